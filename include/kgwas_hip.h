@@ -1,0 +1,127 @@
+/* kgwas_hip.h -- C ABI of the MI355X (gfx950) 1D k-mer association scan.
+ *
+ * The reference has no FFI for this path; the hot path is the in-process seam
+ * of `kgwas --mode cols` (src/kgwas.cpp:183-289): Pearson pre-filter
+ * (src/kgwas.cpp:183-218), compute_cols_stats (src/kgwas.cpp:138-166),
+ * compute_hit + threshold (src/kgwas.cpp:44-56, 239-259).  These entry points
+ * replace that seam; the drop-in `kgwas` binary (supergwas_amd/host) keeps the
+ * process seam (argv/env/files) and calls them.
+ *
+ * Conventions: plain C types; every function returns 0 (KG_OK) or a KG_ERR_*
+ * code and never throws; kg_last_error() gives the message.  The caller owns
+ * host memory, the library owns device memory it allocates.  One context per
+ * host thread.  Device pointers passed in (kg_scan_device, kg_stats_device,
+ * kg_synth_bits) must be 16-byte aligned device allocations.
+ */
+#ifndef KGWAS_HIP_H
+#define KGWAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KG_OK 0
+#define KG_ERR_ARG 1       /* invalid argument / shape */
+#define KG_ERR_DEVICE 2    /* HIP runtime error (no device, launch or copy failure) */
+#define KG_ERR_STATE 3     /* call order (e.g. scan before kg_set_model) */
+#define KG_ERR_CAPACITY 4  /* more hits than `cap`; *n_out holds the required size */
+
+#define KG_PREC_MIXED 0    /* bits x int8-limb weights, exact int32 MFMA accumulation,
+                              FP64 epilogue, FP64 refinement of every reported k-mer */
+#define KG_PREC_FP32 1     /* plain FP32: the reference's cols-mode arithmetic
+                              (src/kgwas.cpp:146-158) on FP32 MFMA */
+
+#define KG_HIT_2D 1u       /* p <= 10^-log10_thresh_2d (src/kgwas.cpp:251-254) */
+
+typedef struct kg_ctx kg_ctx;
+
+typedef struct kg_opts {
+  int32_t device;      /* HIP device ordinal */
+  int32_t precision;   /* KG_PREC_MIXED or KG_PREC_FP32 */
+  int32_t limbs;       /* int8 limbs per weight column for KG_PREC_MIXED (2 or 3; 0 = 3) */
+  int32_t reserved[5];
+} kg_opts;
+
+/* Thresholds of one scan (src/kgwas.cpp:28-30, 101-112, 230-232). */
+typedef struct kg_scan_params {
+  double corr_threshold;   /* CORR_THRESHOLD: keep k-mer iff |cor(x, y)| >= this */
+  double log10_thresh;     /* report iff p <= 10^-log10_thresh */
+  double log10_thresh_2d;  /* KG_HIT_2D iff p <= 10^-log10_thresh_2d; <= 0 disables */
+} kg_scan_params;
+
+/* One reported k-mer: the values compute_hit prints (src/kgwas.cpp:37-56). */
+typedef struct kg_hit {
+  int64_t idx;      /* k-mer row within the scanned batch */
+  int32_t trait;    /* phenotype column */
+  int32_t c;        /* allele count over phenotyped samples (x'x) */
+  uint32_t flags;   /* KG_HIT_2D */
+  uint32_t pad;
+  double xxM;       /* x'x - |U'x|^2 */
+  double xyM;       /* x'y - (U'x)'(U'y) */
+  double effect;    /* xyM / (xxM * yyM) */
+  double T;         /* n ln(1 / (1 - r^2)) */
+  double pval;      /* erfc(sqrt(T / 2)) */
+} kg_hit;
+
+/* Timings of the last scan on the context's stream (HIP events), ms. */
+typedef struct kg_timing {
+  double bulk_ms;       /* bulk scan kernel */
+  double refine_ms;     /* FP64 refinement kernel (0 for KG_PREC_FP32) */
+  double total_ms;      /* first to last device operation of the scan */
+  int64_t candidates;   /* k-mers sent to refinement / host decision */
+  int64_t kmers;        /* k-mers scanned */
+} kg_timing;
+
+/* Device selection, stream creation.  Replaces the OpenBLAS/OpenMP thread
+ * set-up of src/kgwas.cpp:143-144, 227-228. */
+int kg_create(kg_ctx** out, const kg_opts* opts);
+void kg_destroy(kg_ctx* ctx);
+/* Message of the last failing call on ctx (never NULL; "" when none). */
+const char* kg_last_error(const kg_ctx* ctx);
+
+/* Null model.  n_acc accessions (bits per k-mer); idxs[n] = accession
+ * positions that have a phenotype, increasing (build_overlap,
+ * src/kgwas_utils.hpp:291-314); U[n*p] column-major FP64 in overlap order
+ * (load_Ubin, src/kgwas_utils.hpp:135-157); Y[n*t] column-major FP64
+ * phenotypes.  The library copies everything; FP64 residualisation and the
+ * weight quantisation happen here. */
+int kg_set_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int32_t n, const double* U,
+                 int32_t p, const double* Y, int32_t t);
+
+/* Scan m packed k-mer rows (row_words uint64 words each, row_words a multiple
+ * of 8 and >= ceil(n_acc/64); accession a is bit a%64 of word a/64).
+ * Reports every k-mer (and trait) that passes the pre-filter and has
+ * p <= 10^-log10_thresh, sorted by (idx, trait), into out[0..*n_out).
+ * n_passed[t] receives the pre-filter pass count per trait (the `.count`
+ * value, src/kgwas.cpp:292-304).  If more than `cap` hits exist, returns
+ * KG_ERR_CAPACITY with *n_out = required count.
+ * kg_scan: host bits (copied in slices); kg_scan_device: bits already on the
+ * context's device. */
+int kg_scan(kg_ctx* ctx, const uint64_t* bits, int64_t m, int64_t row_words,
+            const kg_scan_params* prm, kg_hit* out, int64_t cap, int64_t* n_out,
+            int64_t* n_passed);
+int kg_scan_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t row_words,
+                   const kg_scan_params* prm, kg_hit* out, int64_t cap, int64_t* n_out,
+                   int64_t* n_passed);
+
+/* Full-statistic mode (validation and checksums): the bulk kernel's values for
+ * every k-mer, written to host arrays: c[m], xxM[m], xyM[m*t], S[m*t]
+ * (S = x'(y - mean y), the pre-filter numerator), k-mer-major. */
+int kg_stats_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t row_words,
+                    int32_t* c, double* xxM, double* xyM, double* S);
+
+int kg_last_timing(const kg_ctx* ctx, kg_timing* out);
+
+/* Seeded synthetic bits on the device (benchmarks and tests; no reference
+ * counterpart): rows kmer0 .. kmer0+m-1 of the model in synth_core.h. */
+int kg_synth_bits(kg_ctx* ctx, uint64_t* d_out, uint64_t seed, int64_t kmer0, int64_t m,
+                  int32_t n_acc, int64_t row_words, const uint8_t* pop, const uint32_t* thr,
+                  int32_t npop, int32_t n_profiles);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KGWAS_HIP_H */

@@ -1,0 +1,173 @@
+// FP64 ORACLE for the 1D k-mer association scan -- TEST INFRASTRUCTURE ONLY.
+//
+// This file is a CPU restatement of SuperGWAS's per-k-mer statistic in double
+// precision.  It is the checker the HIP kernels are compared against and is
+// never linked into, loaded by, or called from the product library or the
+// kgwas drop-in binary.  Only tests/, __graft_entry__.smoke() and bench.py's
+// CPU-baseline leg use it.
+//
+// Followed, line for line in FP64:
+//   src/kgwas.cpp:44-56    compute_hit (r^2 clamp, T = n ln(1/(1-r^2)),
+//                          p = erfc(sqrt(T/2)), effect = xyM/(xxM yyM))
+//   src/kgwas.cpp:387-393  centred phenotype, Uy = U'y, yyM = y'y - |Uy|^2
+//   src/kgwas.cpp:454-459  x(j) = bit at accession idxs[j]
+//   src/kgwas.cpp:462-468  Pearson pre-filter (|x_c|<1e-12 or |y_c|<1e-12
+//                          -> drop; |cor| < tau -> drop)
+//   src/kgwas.cpp:472-474  Ux = U'x, xxM = x'x - |Ux|^2, xyM = x'y - Ux'Uy
+//   src/chisq.hpp:6-9      chi2_sf_k1 (identical to the erfc above)
+//
+// Pinned by tests/test_oracle.py: golden vectors written by
+// tests/golden/make_golden.py from an independent least-squares fit
+// (T = n ln(RSS0/RSS1), effect = beta_x / yyM), scipy's chi2.sf, numpy's
+// corrcoef, brute force over all bit patterns of tiny n, and invariances
+// (sample permutation, U -> UQ).
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+// src/kgwas.cpp:44-56
+void or_compute_hit(double xxM, double xyM, double yyM, int n, double* effect, double* T,
+                    double* pval) {
+  const double eps = 1e-12;
+  double e = 0.0, t = 0.0, p = 1.0;
+  if (xxM > eps && yyM > eps) {
+    double r2 = (xyM * xyM) / (xxM * yyM);
+    if (r2 < 0) r2 = 0;
+    if (r2 >= 1) r2 = 1 - 1e-15;
+    t = (double)n * log(1.0 / (1.0 - r2));
+    p = erfc(sqrt(0.5 * t));
+    e = xyM / (xxM * yyM);
+  }
+  *effect = e;
+  *T = t;
+  *pval = p;
+}
+
+// src/chisq.hpp:6-9
+double or_chi2_sf_k1(double T) {
+  if (T <= 0.0) return 1.0;
+  return erfc(sqrt(0.5 * T));
+}
+
+// yyM = y'y - |U'y|^2 (src/kgwas.cpp:391-392, src/kgwas_utils.hpp:151-152).
+double or_yyM(const double* y, int n, const double* U, int p) {
+  double yy = 0.0, uu = 0.0;
+  for (int j = 0; j < n; ++j) yy += y[j] * y[j];
+  for (int k = 0; k < p; ++k) {
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s += U[(size_t)k * n + j] * y[j];
+    uu += s * s;
+  }
+  return yy - uu;
+}
+
+// Full FP64 scan over m packed k-mer rows (row_words uint64 words each,
+// accession a = bit a%64 of word a/64).  idxs[n] are the phenotyped accession
+// positions in accession order, y[n] the aligned phenotype, U[n*p]
+// column-major.  For every k-mer the statistic is evaluated (whether or not it
+// passes the pre-filter) and pass[i] records the pre-filter decision.
+// Returns the number of k-mers passing the pre-filter (the .count value).
+int64_t or_scan_f64(const uint64_t* bits, int64_t m, int64_t row_words, const int32_t* idxs,
+                    int n, const double* y, const double* U, int p, double corr_thr,
+                    int32_t* c_out, uint8_t* pass_out, double* xxM_out, double* xyM_out,
+                    double* T_out, double* p_out, double* eff_out) {
+  // kgwas.cpp:387-393
+  double ymean = 0.0;
+  for (int j = 0; j < n; ++j) ymean += y[j];
+  ymean /= n;
+  double* yc = (double*)malloc(sizeof(double) * (size_t)n);
+  double ycn = 0.0;
+  for (int j = 0; j < n; ++j) {
+    yc[j] = y[j] - ymean;
+    ycn += yc[j] * yc[j];
+  }
+  ycn = sqrt(ycn);
+  double* Uy = (double*)malloc(sizeof(double) * (size_t)(p > 0 ? p : 1));
+  for (int k = 0; k < p; ++k) {
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s += U[(size_t)k * n + j] * y[j];
+    Uy[k] = s;
+  }
+  const double yyM = or_yyM(y, n, U, p);
+  const double eps = 1e-12;
+  int64_t passed = 0;
+
+#pragma omp parallel
+  {
+    double* x = (double*)malloc(sizeof(double) * (size_t)n);
+    double* Ux = (double*)malloc(sizeof(double) * (size_t)(p > 0 ? p : 1));
+#pragma omp for schedule(dynamic, 256) reduction(+ : passed)
+    for (int64_t i = 0; i < m; ++i) {
+      const uint64_t* row = bits + (size_t)i * (size_t)row_words;
+      // kgwas.cpp:455-459
+      double xs = 0.0;
+      int32_t c = 0;
+      for (int j = 0; j < n; ++j) {
+        const int a = idxs[j];
+        x[j] = (double)((row[a >> 6] >> (a & 63)) & 1ULL);
+        xs += x[j];
+        c += (int32_t)x[j];
+      }
+      // kgwas.cpp:462-468
+      const double xmean = xs / n;
+      double xcn = 0.0, dot = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double d = x[j] - xmean;
+        xcn += d * d;
+        dot += d * yc[j];
+      }
+      xcn = sqrt(xcn);
+      int ok = 1;
+      if (xcn < eps || ycn < eps) ok = 0;
+      else if (fabs(dot / (xcn * ycn)) < corr_thr) ok = 0;
+      // kgwas.cpp:472-474
+      for (int k = 0; k < p; ++k) Ux[k] = 0.0;
+      double xx = 0.0, xy = 0.0;
+      for (int j = 0; j < n; ++j) {
+        if (x[j] != 0.0) {
+          xx += x[j] * x[j];
+          xy += x[j] * y[j];
+          for (int k = 0; k < p; ++k) Ux[k] += U[(size_t)k * n + j];
+        }
+      }
+      double uu = 0.0, uy = 0.0;
+      for (int k = 0; k < p; ++k) {
+        uu += Ux[k] * Ux[k];
+        uy += Ux[k] * Uy[k];
+      }
+      const double xxM = xx - uu, xyM = xy - uy;
+      double e, t, pv;
+      or_compute_hit(xxM, xyM, yyM, n, &e, &t, &pv);
+      c_out[i] = c;
+      pass_out[i] = (uint8_t)ok;
+      xxM_out[i] = xxM;
+      xyM_out[i] = xyM;
+      T_out[i] = t;
+      p_out[i] = pv;
+      eff_out[i] = e;
+      passed += ok;
+    }
+    free(x);
+    free(Ux);
+  }
+  free(yc);
+  free(Uy);
+  return passed;
+}
+
+// Smallest double T with erfc(sqrt(T/2)) <= p_thresh (bisection on the
+// reference's own p-value expression).  Used by tests to state thresholds in
+// T space.
+double or_T_threshold(double p_thresh) {
+  if (p_thresh >= 1.0) return 0.0;
+  double lo = 0.0, hi = 1.0;
+  while (or_chi2_sf_k1(hi) > p_thresh) hi *= 2.0;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (or_chi2_sf_k1(mid) <= p_thresh) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}

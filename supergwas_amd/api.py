@@ -1,0 +1,197 @@
+"""Python host binding of libkgwas_hip's C ABI (include/kgwas_hip.h) via ctypes.
+
+This mirrors the reference's in-process seam of `kgwas --mode cols`
+(src/kgwas.cpp:183-289): set the null model once (U.bin + aligned phenotype),
+then scan packed k-mer rows and receive the reported k-mers with the values
+compute_hit prints (src/kgwas.cpp:44-56) plus the pre-filter pass count
+(`.count`, src/kgwas.cpp:292-304).  Errors raise KgError carrying the
+library's message, as die() ends the reference with "[ERROR] msg".
+
+There is no CPU fallback: without the built HIP library every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libkgwas_hip.so")
+
+KG_OK, KG_ERR_ARG, KG_ERR_DEVICE, KG_ERR_STATE, KG_ERR_CAPACITY = 0, 1, 2, 3, 4
+PRECISIONS = {"mixed": 0, "fp32": 1}
+HIT_2D = 1
+
+HIT_DTYPE = np.dtype([("idx", np.int64), ("trait", np.int32), ("c", np.int32),
+                      ("flags", np.uint32), ("pad", np.uint32), ("xxM", np.float64),
+                      ("xyM", np.float64), ("effect", np.float64), ("T", np.float64),
+                      ("pval", np.float64)])
+
+
+class KgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[kgwas_hip error {code}] {msg}")
+        self.code = code
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("limbs", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+
+
+class ScanParams(ctypes.Structure):
+    """CORR_THRESHOLD, log10 threshold, 2D log10 threshold (src/kgwas.cpp:28-30)."""
+    _fields_ = [("corr_threshold", ctypes.c_double), ("log10_thresh", ctypes.c_double),
+                ("log10_thresh_2d", ctypes.c_double)]
+
+    def __init__(self, corr_threshold=0.2, log10_thresh=6.0, log10_thresh_2d=15.0):
+        super().__init__(corr_threshold, log10_thresh, log10_thresh_2d)
+
+
+class _Timing(ctypes.Structure):
+    _fields_ = [("bulk_ms", ctypes.c_double), ("refine_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("candidates", ctypes.c_int64),
+                ("kmers", ctypes.c_int64)]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+
+# Every entry point the header declares, with its ctypes signature.
+SIGNATURES = {
+    "kg_create": ([ctypes.POINTER(_vp), ctypes.POINTER(_Opts)], ctypes.c_int),
+    "kg_destroy": ([_vp], None),
+    "kg_last_error": ([_vp], ctypes.c_char_p),
+    "kg_set_model": ([_vp, _i32, _vp, _i32, _vp, _i32, _vp, _i32], ctypes.c_int),
+    "kg_scan": ([_vp, _vp, _i64, _i64, ctypes.POINTER(ScanParams), _vp, _i64,
+                 ctypes.POINTER(_i64), _vp], ctypes.c_int),
+    "kg_scan_device": ([_vp, _vp, _i64, _i64, ctypes.POINTER(ScanParams), _vp, _i64,
+                        ctypes.POINTER(_i64), _vp], ctypes.c_int),
+    "kg_stats_device": ([_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "kg_last_timing": ([_vp, ctypes.POINTER(_Timing)], ctypes.c_int),
+    "kg_synth_bits": ([_vp, _vp, ctypes.c_uint64, _i64, _i64, _i32, _i64, _vp, _vp, _i32, _i32],
+                      ctypes.c_int),
+}
+
+
+def lib():
+    """Load libkgwas_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise KgError(KG_ERR_STATE, f"{LIB_PATH} missing: build the HIP library first "
+                          "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+class KgwasHip:
+    """One device context (kg_create) with one null model (kg_set_model)."""
+
+    def __init__(self, device=0, precision="mixed", limbs=3):
+        L = lib()
+        opts = _Opts(device, PRECISIONS[precision], limbs)
+        h = _vp()
+        rc = L.kg_create(ctypes.byref(h), ctypes.byref(opts))
+        self._h = h
+        if rc != KG_OK:
+            msg = L.kg_last_error(h).decode() if h.value else "kg_create failed"
+            if h.value:
+                L.kg_destroy(h)
+            self._h = _vp()
+            raise KgError(rc, msg)
+        self.precision = precision
+        self.t = 0
+        self.n = 0
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().kg_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != KG_OK:
+            raise KgError(rc, lib().kg_last_error(self._h).decode())
+
+    def set_model(self, n_acc, idxs, U, Y):
+        """idxs: phenotyped accession positions (increasing); U: n x p; Y: n or n x t."""
+        idxs = np.ascontiguousarray(idxs, np.int32)
+        U = np.asfortranarray(np.asarray(U, np.float64).reshape(len(idxs), -1))
+        Y = np.asarray(Y, np.float64)
+        if Y.ndim == 1:
+            Y = Y[:, None]
+        Y = np.asfortranarray(Y)
+        self.t = Y.shape[1]
+        self.n = len(idxs)
+        self._check(lib().kg_set_model(self._h, int(n_acc), _p(idxs), len(idxs),
+                                       _p(U) if U.shape[1] else None, U.shape[1], _p(Y),
+                                       Y.shape[1]))
+
+    def _scan(self, fn, ptr, m, row_words, params, cap):
+        params = params or ScanParams()
+        np_ = np.zeros(max(self.t, 1), np.int64)
+        while True:
+            out = np.zeros(max(cap, 1), HIT_DTYPE)
+            n_out = _i64(0)
+            rc = fn(self._h, ptr, int(m), int(row_words), ctypes.byref(params), _p(out), cap,
+                    ctypes.byref(n_out), _p(np_))
+            if rc == KG_ERR_CAPACITY:
+                cap = int(n_out.value)
+                continue
+            self._check(rc)
+            return out[:n_out.value], np_.copy()
+
+    def scan(self, bits, params=None, cap=1 << 16):
+        """Host rows (uint64 [m, row_words]) -> (hits, n_passed per trait)."""
+        bits = np.ascontiguousarray(bits, np.uint64)
+        return self._scan(lib().kg_scan, _p(bits), bits.shape[0], bits.shape[1], params, cap)
+
+    def scan_device(self, dev_ptr, m, row_words, params=None, cap=1 << 16):
+        """Rows already in device memory at dev_ptr (e.g. a torch tensor's data_ptr())."""
+        return self._scan(lib().kg_scan_device, dev_ptr, m, row_words, params, cap)
+
+    def stats_device(self, dev_ptr, m, row_words):
+        """Bulk-kernel values for every k-mer: c, xxM, xyM[m, t], S[m, t]."""
+        t = max(self.t, 1)
+        c = np.zeros(m, np.int32)
+        xx = np.zeros(m)
+        xy = np.zeros((m, t))
+        S = np.zeros((m, t))
+        self._check(lib().kg_stats_device(self._h, dev_ptr, int(m), int(row_words), _p(c), _p(xx),
+                                          _p(xy), _p(S)))
+        return dict(c=c, xxM=xx, xyM=xy, S=S)
+
+    def synth_bits(self, dev_ptr, ds, kmer0, m, row_words):
+        pop = np.ascontiguousarray(ds.pop, np.uint8)
+        thr = np.ascontiguousarray(ds.thr, np.uint32)
+        self._check(lib().kg_synth_bits(self._h, dev_ptr, ds.seed, int(kmer0), int(m),
+                                        ds.n_acc, int(row_words), _p(pop), _p(thr), ds.npop,
+                                        ds.n_profiles))
+
+    def timing(self):
+        t = _Timing()
+        self._check(lib().kg_last_timing(self._h, ctypes.byref(t)))
+        return dict(bulk_ms=t.bulk_ms, refine_ms=t.refine_ms, total_ms=t.total_ms,
+                    candidates=t.candidates, kmers=t.kmers)

@@ -1,0 +1,73 @@
+// Shared definitions between the host side of libkgwas_hip (model set-up,
+// decisions) and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+namespace kg {
+
+constexpr int kMaxTraits = 8;
+
+// Bulk-kernel candidate: a k-mer/trait whose pre-filter or 1D significance is
+// not decided "no" by the bulk pass.
+struct Cand {
+  int64_t idx;     // k-mer row (global within the scan)
+  int32_t trait;
+  uint32_t flags;  // kCand* bits below
+  float xx32;      // plain FP32 path: the reference-arithmetic xxM, xyM
+  float xy32;
+};
+constexpr uint32_t kPfPass = 1u;  // pre-filter decided pass
+constexpr uint32_t kPfAmb = 2u;   // pre-filter within the error bound of tau
+constexpr uint32_t kSigDef = 4u;  // 1D threshold decided pass
+constexpr uint32_t kSigAmb = 8u;  // 1D threshold within the error bound
+
+// Per-scan outputs of the bulk kernels.
+struct ScanOut {
+  Cand* cand;                    // candidate records (unordered)
+  unsigned long long* counters;  // [0] candidates, [1 + tr] decided pre-filter passes
+  int64_t cap;                   // capacity of cand
+  // full-statistic mode (null pointers otherwise)
+  int32_t* st_c;
+  double* st_xx;
+  double* st_xy;  // [m][t]
+  double* st_S;   // [m][t]
+};
+
+// Epilogue constants of the int8-limb kernel (device pointers for arrays).
+struct EpiI8 {
+  int n;            // phenotyped samples
+  int nz;           // projected structure columns kept (z columns)
+  int t;            // traits (r columns follow the z columns)
+  int L;            // limbs per column
+  int NU;           // weight columns in the MFMA tiles (multiple of 16)
+  int tau_trivial;  // corr threshold <= 0: every polymorphic k-mer passes
+  int all_sig;      // p threshold >= 1: every pre-filter pass is reported
+  int has_d;        // z column 0 is the kept intercept-remainder column d
+  double sigma;     // (Q'U'1)_0: the rotated U'1 = sigma e_0
+  double base_coef; // sigma^2 / n^2  (x'x - |cn m'|^2 = c - c^2 base_coef)
+  double dmax_drop; // max|d| of the dropped intercept-remainder column (0 if kept)
+  double tau;
+  double r2star;    // 1D threshold in r^2
+  const double* colw;   // [nb][L] limb weights
+  const double* colh;   // [nb] per-set-bit quantisation bound
+  const double* bp;     // [t][nz] rotated coefficients b' = Q'U'y (S = s + z'b')
+  const double* bp0;    // [t] b'_0 of the dropped column (bound only)
+  const double* rbar;   // [t] mean of r = y - U U'y
+  const double* ycn;    // [t] |y - mean y|
+  const double* yyM;    // [t]
+};
+
+// Epilogue constants of the plain FP32 kernel.
+struct EpiF32 {
+  int n, p, t, NU;
+  int tau_trivial, all_sig;
+  float tau;
+  float ymean_f[kMaxTraits];    // FP32 mean of y (reference pre-filter)
+  float ycsum_f[kMaxTraits];    // FP32 sum of y - mean
+  float ynorm_f[kMaxTraits];    // FP32 |y - mean|
+  double yyM[kMaxTraits];
+  double Tstar_lo;              // loose 1D threshold in T (host decides exactly)
+  const float* Uyf;             // [t][p] FP32 U'y
+};
+
+}  // namespace kg

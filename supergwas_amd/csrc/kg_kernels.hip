@@ -1,0 +1,608 @@
+// gfx950 kernels of the 1D k-mer association scan.
+//
+//  scan_i8   : packed bits x int8-limb weights on v_mfma_i32_16x16x64_i8 (exact
+//              int32 sums), FP64 epilogue with a certified error bound per
+//              k-mer, wave-ballot compaction of candidates.
+//  scan_f32  : the reference's cols-mode arithmetic (src/kgwas.cpp:146-158,
+//              183-200) on v_mfma_f32_16x16x4_f32, FP32 epilogue.
+//  refine_f64: exact FP64 statistic of listed k-mers from their bits.
+//  synth_bits: seeded synthetic rows (synth_core.h).
+//
+// Sample (K) order.  Both bulk kernels contract over accessions in an order
+// chosen so that every lane streams contiguous bits:
+//   scan_i8 : accession 512*C + 128*g + 16*j + e  <->  chunk C, lane group
+//             g = lane/16, MFMA step j (0..7), operand byte e (0..15)
+//   scan_f32: accession 256*C + 64*g + e          <->  chunk C, lane group g,
+//             MFMA step e (0..63)
+// The host lays the weights out in the same order (kg_api.cpp), so the
+// contraction is the plain sum over accessions.
+#include <hip/hip_runtime.h>
+
+#include "kg_internal.h"
+#include "synth_core.h"
+
+namespace kg {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;        // waves per workgroup
+constexpr int kThreads = 64 * kWaves;
+constexpr int kI8Steps = 8;      // 64-sample MFMA steps per 512-sample chunk
+
+// 4 bits -> 4 bytes of 0/1: bit b of nib lands in byte b.
+__device__ __forceinline__ int spread4(uint32_t nib) {
+  return (int)(__umul24(nib, 0x00204081u) & 0x01010101u);
+}
+
+__device__ __forceinline__ double wave_sum4(double v) {  // over lanes l^1, l^2
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  return v;
+}
+
+// Wave-aggregated append of candidate records.  Must be reached by every lane
+// of the wave (uniform control flow).
+__device__ __forceinline__ void append_cand(bool want, const Cand& rec, const ScanOut& o) {
+  const unsigned long long ballot = __ballot(want);
+  if (ballot == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)ballot) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(o.counters, (unsigned long long)__popcll(ballot));
+  base = __shfl(base, leader);
+  if (want) {
+    const unsigned long long below = ballot & ((1ull << lane) - 1ull);
+    const unsigned long long pos = base + (unsigned long long)__popcll(below);
+    if ((long long)pos < o.cap) o.cand[pos] = rec;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Pre-filter + 1D threshold decision of the int8 path, with error bounds.
+// Returns candidate flags (0 = decided "not reported and not counted").
+// *pf_def is set when the pre-filter is decided "pass" (counted on device).
+__device__ uint32_t classify_i8(int c, double xxM, double bxx, double S, double bS,
+                                double xyM, double bxy, const EpiI8& ep, int tr,
+                                bool* pf_def) {
+  *pf_def = false;
+  const int n = ep.n;
+  if (c <= 0 || c >= n) return 0;               // |x - mean x| = 0 (src/kgwas.cpp:465)
+  const double ycn = ep.ycn[tr];
+  if (!(ycn >= 1e-12)) return 0;                // |y - mean y| < eps
+  uint32_t f;
+  if (ep.tau_trivial) {
+    f = kPfPass;
+  } else {
+    const double v = (double)c * (double)(n - c) / (double)n;
+    const double thr = ep.tau * sqrt(v) * ycn;  // |cor| >= tau  <=>  |S| >= thr
+    const double aS = fabs(S);
+    if (aS - bS >= thr * (1.0 + 1e-12)) f = kPfPass;
+    else if (aS + bS < thr * (1.0 - 1e-12)) return 0;
+    else f = kPfAmb;
+  }
+  *pf_def = (f == kPfPass);
+  uint32_t s;
+  const double yy = ep.yyM[tr];
+  if (ep.all_sig) {
+    s = kSigDef;
+  } else if (!(yy > 1e-12)) {
+    s = 0;                                      // compute_hit -> p = 1
+  } else if (xxM + bxx <= 1e-12) {
+    s = 0;
+  } else if (xxM - bxx <= 1e-12) {
+    s = kSigAmb;
+  } else {
+    const double axy = fabs(xyM);
+    const double hi = axy + bxy;
+    const double lo = fmax(axy - bxy, 0.0);
+    const double r2hi = hi * hi / ((xxM - bxx) * yy);
+    const double r2lo = lo * lo / ((xxM + bxx) * yy);
+    if (r2lo >= ep.r2star * (1.0 + 1e-9)) s = kSigDef;
+    else if (r2hi < ep.r2star * (1.0 - 1e-9)) s = 0;
+    else s = kSigAmb;
+  }
+  if (s == 0 && f != kPfAmb) return 0;
+  return f | s;
+}
+
+// ----------------------------------------------------------------------------
+// Bulk int8-limb scan.  Workgroup = 4 waves; wave w of block `blk` owns
+// k-mers [ (blk*4 + w) * 16R, +16R ) as R tiles of 16.  Persistent: blocks
+// are strided over the grid.
+template <int NT, int R>
+__global__ __launch_bounds__(kThreads, 2) void scan_i8(
+    const uint64_t* __restrict__ bits, int64_t m, int64_t row_words, int64_t row0,
+    int nchunks, const v4i* __restrict__ wts, const uint4* __restrict__ maskv, EpiI8 ep,
+    ScanOut out, int64_t nblocks) {
+  constexpr int NU = 16 * NT;
+  constexpr int FRAG = kI8Steps * NT * 64;  // v4i per chunk of weights
+  extern __shared__ v4i lds[];
+  __shared__ unsigned int wg_pass[kMaxTraits];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  const int kk = lane >> 2, q = lane & 3;
+  if (threadIdx.x < kMaxTraits) wg_pass[threadIdx.x] = 0;
+
+  for (int64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+    const int64_t kbase = (blk * kWaves + wave) * (16 * R);
+    v4i acc[R][NT];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[r][t] = v4i{0, 0, 0, 0};
+    uint32_t cnt[R];
+    const uint64_t* rowp[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cnt[r] = 0;
+      int64_t k = kbase + 16 * r + col;
+      if (k > m - 1) k = m - 1;                 // clamp: rows past m are read, never reported
+      rowp[r] = bits + k * row_words + 2 * g;
+    }
+
+    for (int C = 0; C < nchunks; ++C) {
+      uint4 raw[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) raw[r] = *reinterpret_cast<const uint4*>(rowp[r] + 8 * C);
+      const uint4 mk = maskv[4 * C + g];
+      __syncthreads();                          // previous chunk fully consumed
+      const v4i* src = wts + (size_t)C * FRAG;
+      for (int i = threadIdx.x; i < FRAG; i += kThreads) lds[i] = src[i];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        raw[r].x &= mk.x;
+        raw[r].y &= mk.y;
+        raw[r].z &= mk.z;
+        raw[r].w &= mk.w;
+        cnt[r] += __popc(raw[r].x) + __popc(raw[r].y) + __popc(raw[r].z) + __popc(raw[r].w);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kI8Steps; ++j) {
+        v4i a[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) a[t] = lds[(j * NT + t) * 64 + lane];
+        v4i bf[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t w32 = (j < 2) ? raw[r].x : (j < 4) ? raw[r].y : (j < 6) ? raw[r].z : raw[r].w;
+          const uint32_t w16 = w32 >> (16 * (j & 1));
+          bf[r] = v4i{spread4(w16 & 15u), spread4((w16 >> 4) & 15u), spread4((w16 >> 8) & 15u),
+                      spread4((w16 >> 12) & 15u)};
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[t], bf[r], acc[r][t], 0, 0, 0);
+      }
+    }
+
+    // ---- epilogue: per tile, spill D to LDS as [k-mer][weight column] ----
+    __syncthreads();
+    int* sc = reinterpret_cast<int*>(lds) + wave * 16 * NU;
+    const double inv_n = 1.0 / (double)ep.n;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      uint32_t cc = cnt[r];
+      cc += __shfl_xor(cc, 16);
+      cc += __shfl_xor(cc, 32);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        *reinterpret_cast<v4i*>(sc + col * NU + 16 * t + 4 * g) = acc[r][t];
+      __syncthreads();
+      const int c = (int)__shfl(cc, kk);
+      const int64_t kidx = kbase + 16 * r + kk;
+      const bool valid = (kidx < m);
+      const int* drow = sc + kk * NU;
+      const double cd = (double)c, cn = cd * inv_n;
+
+      double zz = 0.0, bzz = 0.0, cross = 0.0, bcross = 0.0;
+      for (int b = q; b < ep.nz; b += 4) {
+        double z = 0.0;
+        for (int l = 0; l < ep.L; ++l) z += ep.colw[b * ep.L + l] * (double)drow[ep.L * b + l];
+        const double e = cd * ep.colh[b];
+        zz += z * z;
+        bzz += 2.0 * fabs(z) * e + e * e;
+        if (ep.has_d && b == 0) {
+          cross = 2.0 * cn * ep.sigma * z;
+          bcross = 2.0 * cn * fabs(ep.sigma) * e;
+        }
+      }
+      zz = wave_sum4(zz);
+      bzz = wave_sum4(bzz);
+      cross = wave_sum4(cross);
+      bcross = wave_sum4(bcross);
+      const double xxM = cd - cd * cd * ep.base_coef - cross - zz;
+      double bxx = bzz + bcross + 1e-11 * (cd + zz + fabs(cross) + 1.0);
+      if (ep.dmax_drop > 0.0) {
+        const double e0 = cd * ep.dmax_drop;
+        bxx += 2.0 * cn * fabs(ep.sigma) * e0 + e0 * e0;
+      }
+
+      for (int tr = 0; tr < ep.t; ++tr) {
+        double S = 0.0, bS = 0.0, sv = 0.0, es = 0.0;
+        for (int b = q; b < ep.nz; b += 4) {
+          double z = 0.0;
+          for (int l = 0; l < ep.L; ++l) z += ep.colw[b * ep.L + l] * (double)drow[ep.L * b + l];
+          const double bpk = ep.bp[tr * ep.nz + b];
+          S += z * bpk;
+          bS += cd * ep.colh[b] * fabs(bpk);
+        }
+        if (q == 0) {
+          const int b = ep.nz + tr;
+          for (int l = 0; l < ep.L; ++l) sv += ep.colw[b * ep.L + l] * (double)drow[ep.L * b + l];
+          es = cd * ep.colh[b];
+        }
+        S = wave_sum4(S + sv);
+        bS = wave_sum4(bS + es);
+        sv = wave_sum4(sv);
+        es = wave_sum4(es);
+        if (ep.dmax_drop > 0.0) bS += cd * ep.dmax_drop * fabs(ep.bp0[tr]);
+        bS += 1e-12 * (fabs(S) + fabs(sv) + 1.0);
+        const double xyM = sv + cd * ep.rbar[tr];
+        const double bxy = es + 1e-12 * (fabs(sv) + cd * fabs(ep.rbar[tr]) + 1.0);
+        bool pf_def = false;
+        uint32_t fl = 0;
+        if (q == 0 && valid) fl = classify_i8(c, xxM, bxx, S, bS, xyM, bxy, ep, tr, &pf_def);
+        if (out.st_c != nullptr && q == 0 && valid) {
+          const int64_t o = kidx - row0;
+          if (tr == 0) {
+            out.st_c[o] = c;
+            out.st_xx[o] = xxM;
+          }
+          out.st_xy[o * ep.t + tr] = xyM;
+          out.st_S[o * ep.t + tr] = S;
+        }
+        const unsigned long long pb = __ballot(pf_def);
+        if (lane == 0 && pb) atomicAdd(&wg_pass[tr], (unsigned int)__popcll(pb));
+        Cand rec;
+        rec.idx = kidx;
+        rec.trait = tr;
+        rec.flags = fl;
+        rec.xx32 = 0.f;
+        rec.xy32 = 0.f;
+        append_cand(fl != 0, rec, out);
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < ep.t && wg_pass[threadIdx.x])
+    atomicAdd(&out.counters[1 + threadIdx.x], (unsigned long long)wg_pass[threadIdx.x]);
+}
+
+// ----------------------------------------------------------------------------
+// Plain FP32 scan.  Columns: [0,p) U, [p,p+t) y, [p+t,p+2t) y - mean(y).
+template <int NT, int R>
+__global__ __launch_bounds__(kThreads, 2) void scan_f32(
+    const uint64_t* __restrict__ bits, int64_t m, int64_t row_words, int64_t row0,
+    int nchunks, const float* __restrict__ wts, const uint64_t* __restrict__ maskw,
+    EpiF32 ep, ScanOut out, int64_t nblocks) {
+  constexpr int NU = 16 * NT;
+  constexpr int FRAG = 64 * NT * 64;  // floats per 256-sample chunk
+  extern __shared__ float ldsf[];
+  __shared__ unsigned int wg_pass[kMaxTraits];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  const int kk = lane >> 2, q = lane & 3;
+  if (threadIdx.x < kMaxTraits) wg_pass[threadIdx.x] = 0;
+
+  for (int64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+    const int64_t kbase = (blk * kWaves + wave) * (16 * R);
+    v4f acc[R][NT];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[r][t] = v4f{0.f, 0.f, 0.f, 0.f};
+    uint32_t cnt[R];
+    const uint64_t* rowp[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cnt[r] = 0;
+      int64_t k = kbase + 16 * r + col;
+      if (k > m - 1) k = m - 1;
+      rowp[r] = bits + k * row_words + g;
+    }
+    for (int C = 0; C < nchunks; ++C) {
+      uint64_t raw[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) raw[r] = rowp[r][4 * C];
+      const uint64_t mk = maskw[4 * C + g];
+      __syncthreads();
+      const float* src = wts + (size_t)C * FRAG;
+      for (int i = threadIdx.x; i < FRAG / 4; i += kThreads)
+        reinterpret_cast<float4*>(ldsf)[i] = reinterpret_cast<const float4*>(src)[i];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        raw[r] &= mk;
+        cnt[r] += (uint32_t)__popcll(raw[r]);
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int e = 0; e < 64; ++e) {
+        float a[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) a[t] = ldsf[(e * NT + t) * 64 + lane];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const float b = (float)(uint32_t)((raw[r] >> e) & 1ull);
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], b, acc[r][t], 0, 0, 0);
+        }
+      }
+    }
+
+    __syncthreads();
+    float* sc = ldsf + wave * 16 * NU;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      uint32_t cc = cnt[r];
+      cc += __shfl_xor(cc, 16);
+      cc += __shfl_xor(cc, 32);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        *reinterpret_cast<v4f*>(sc + col * NU + 16 * t + 4 * g) = acc[r][t];
+      __syncthreads();
+      const int c = (int)__shfl(cc, kk);
+      const int64_t kidx = kbase + 16 * r + kk;
+      const bool valid = (kidx < m);
+      const float* drow = sc + kk * NU;
+      for (int tr = 0; tr < ep.t; ++tr) {
+        // src/kgwas.cpp:153-158 in FP32 (partial sums over lanes q)
+        float sux = 0.f, uyux = 0.f;
+        for (int k = q; k < ep.p; k += 4) {
+          const float v = drow[k];
+          sux += v * v;
+          uyux += v * ep.Uyf[tr * ep.p + k];
+        }
+        sux += __shfl_xor(sux, 1);
+        sux += __shfl_xor(sux, 2);
+        uyux += __shfl_xor(uyux, 1);
+        uyux += __shfl_xor(uyux, 2);
+        const float Xy = drow[ep.p + tr];
+        const float Syc = drow[ep.p + ep.t + tr];
+        const float xyM = Xy - uyux;
+        const float xxM = (float)c - sux;  // colwise squaredNorm of 0/1 = c
+        // pre-filter, src/kgwas.cpp:187-199 in FP32
+        const float fn = (float)ep.n;
+        const float mean = (float)c / fn;
+        const float xn2 = (float)c * (1.f - mean) * (1.f - mean) + (fn - (float)c) * mean * mean;
+        const float xnorm = sqrtf(xn2);
+        const float dot = Syc - mean * ep.ycsum_f[tr];
+        bool pass = false;
+        if (!(xnorm < 1e-12f || ep.ynorm_f[tr] < 1e-12f)) {
+          const float cor = dot / (xnorm * ep.ynorm_f[tr]);
+          pass = ep.tau_trivial ? true : (fabsf(cor) >= ep.tau);
+        }
+        pass = pass && valid && q == 0;
+        bool cand = false;
+        if (pass) {
+          if (ep.all_sig) {
+            cand = true;
+          } else {
+            const double xx = (double)xxM, xy = (double)xyM, yy = ep.yyM[tr];
+            if (xx > 1e-12 && yy > 1e-12) {
+              double r2 = (xy * xy) / (xx * yy);
+              if (r2 < 0) r2 = 0;
+              if (r2 >= 1) r2 = 1 - 1e-15;
+              const double T = (double)ep.n * log(1.0 / (1.0 - r2));
+              cand = (T >= ep.Tstar_lo);
+            }
+          }
+        }
+        if (out.st_c != nullptr && q == 0 && valid) {
+          const int64_t o = kidx - row0;
+          if (tr == 0) {
+            out.st_c[o] = c;
+            out.st_xx[o] = (double)xxM;
+          }
+          out.st_xy[o * ep.t + tr] = (double)xyM;
+          out.st_S[o * ep.t + tr] = (double)dot;
+        }
+        const unsigned long long pb = __ballot(pass);
+        if (lane == 0 && pb) atomicAdd(&wg_pass[tr], (unsigned int)__popcll(pb));
+        Cand rec;
+        rec.idx = kidx;
+        rec.trait = tr;
+        rec.flags = kPfPass | kSigAmb;
+        rec.xx32 = xxM;
+        rec.xy32 = xyM;
+        append_cand(cand, rec, out);
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < ep.t && wg_pass[threadIdx.x])
+    atomicAdd(&out.counters[1 + threadIdx.x], (unsigned long long)wg_pass[threadIdx.x]);
+}
+
+// ----------------------------------------------------------------------------
+// Exact FP64 statistic of each candidate.  One wave per candidate; lane b
+// accumulates base column b (and b + 64) over the set, phenotyped bits in
+// accession order.  B64 is [n_acc][nbf] (zero rows for accessions without a
+// phenotype).  Columns: [0,p) rotated centred U, [p, p+t) centred residuals.
+// res[i] = {c, xxM, xyM, S}.
+__global__ __launch_bounds__(kThreads) void refine_f64(
+    const Cand* __restrict__ cand, int64_t ncand, const uint64_t* __restrict__ bits,
+    int64_t row_words, int64_t row0, const uint64_t* __restrict__ maskw, int nwords,
+    const double* __restrict__ B64, int nbf, int p, int t, int n, double sigma,
+    double base_coef, const double* __restrict__ bpfull, const double* __restrict__ rbar,
+    double* __restrict__ res) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int64_t nw = (int64_t)gridDim.x * kWaves;
+  for (int64_t i = wid; i < ncand; i += nw) {
+    const Cand cd = cand[i];
+    const uint64_t* row = bits + (cd.idx - row0) * row_words;
+    double a0 = 0.0, a1 = 0.0;
+    int c = 0;
+    for (int w = 0; w < nwords; ++w) {
+      uint64_t word = row[w] & maskw[w];
+      c += __popcll(word);
+      while (word) {
+        const int bit = __ffsll((long long)word) - 1;
+        word &= word - 1;
+        const double* Br = B64 + (size_t)(64 * w + bit) * nbf;
+        if (lane < nbf) a0 += Br[lane];
+        if (lane + 64 < nbf) a1 += Br[lane + 64];
+      }
+    }
+    const int tr = cd.trait;
+    double zz = 0.0, zb = 0.0, z0 = 0.0, sv = 0.0;
+    if (lane < p) {
+      zz += a0 * a0;
+      zb += a0 * bpfull[tr * p + lane];
+      if (lane == 0) z0 = a0;
+    }
+    if (lane + 64 < p) {
+      zz += a1 * a1;
+      zb += a1 * bpfull[tr * p + lane + 64];
+    }
+    if (lane == p + tr) sv = a0;
+    if (lane + 64 == p + tr) sv = a1;
+    for (int off = 32; off > 0; off >>= 1) {
+      zz += __shfl_xor(zz, off);
+      zb += __shfl_xor(zb, off);
+      z0 += __shfl_xor(z0, off);
+      sv += __shfl_xor(sv, off);
+    }
+    if (lane == 0) {
+      const double cdv = (double)c, cn = cdv / (double)n;
+      const double xxM = cdv - cdv * cdv * base_coef - 2.0 * cn * sigma * z0 - zz;
+      res[4 * i + 0] = cdv;
+      res[4 * i + 1] = xxM;
+      res[4 * i + 2] = sv + cdv * rbar[tr];
+      res[4 * i + 3] = sv + zb;
+    }
+  }
+}
+
+__global__ void zero_u64(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+__global__ void synth_bits(uint64_t* __restrict__ out, uint64_t seed, int64_t kmer0, int64_t m,
+                           uint32_t n_acc, int64_t row_words, const uint8_t* __restrict__ pop,
+                           const uint32_t* __restrict__ thr, uint32_t npop, uint32_t nprof) {
+  const int64_t total = m * row_words;
+  const uint32_t used = (n_acc + 63u) / 64u;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = i / row_words;
+    const uint32_t w = (uint32_t)(i - k * row_words);
+    out[i] = (w < used) ? ks_word(seed, (uint64_t)(kmer0 + k), w, n_acc, pop, thr, npop, nprof)
+                        : 0ull;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Launchers (host).  The tile count NT is a template parameter.
+
+template <int NT>
+static hipError_t launch_i8_nt(int R, dim3 grid, size_t shm, hipStream_t s, const uint64_t* bits,
+                               int64_t m, int64_t rw, int64_t row0, int nch, const void* wts,
+                               const void* mask, const EpiI8& ep, const ScanOut& o,
+                               int64_t nblocks) {
+  if (R == 4) {
+    auto kfn = scan_i8<NT, 4>;
+    (void)hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL(kfn, grid, dim3(kThreads), shm, s, bits, m, rw, row0, nch,
+                       (const v4i*)wts, (const uint4*)mask, ep, o, nblocks);
+  } else if (R == 2) {
+    auto kfn = scan_i8<NT, 2>;
+    (void)hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL(kfn, grid, dim3(kThreads), shm, s, bits, m, rw, row0, nch,
+                       (const v4i*)wts, (const uint4*)mask, ep, o, nblocks);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const uint64_t* bits,
+                          int64_t m, int64_t rw, int64_t row0, const void* wts, const void* mask,
+                          const EpiI8& ep, const ScanOut& o) {
+  const int nch = (int)(rw / 8);
+  const int64_t per_block = (int64_t)kWaves * 16 * R;
+  const int64_t nblocks = (m + per_block - 1) / per_block;
+  const size_t shm = (size_t)kI8Steps * NT * 64 * 16;
+  dim3 g((unsigned)(nblocks < grid ? nblocks : grid));
+  switch (NT) {
+#define KG_CASE(X) \
+  case X:          \
+    return launch_i8_nt<X>(R, g, shm, s, bits, m, rw, row0, nch, wts, mask, ep, o, nblocks);
+    KG_CASE(1) KG_CASE(2) KG_CASE(3) KG_CASE(4) KG_CASE(5) KG_CASE(6) KG_CASE(7) KG_CASE(8)
+#undef KG_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <int NT>
+static hipError_t launch_f32_nt(dim3 grid, size_t shm, hipStream_t s, const uint64_t* bits,
+                                int64_t m, int64_t rw, int64_t row0, int nch, const float* wts,
+                                const uint64_t* mask, const EpiF32& ep, const ScanOut& o,
+                                int64_t nblocks) {
+  auto kfn = scan_f32<NT, 4>;
+  (void)hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+  hipLaunchKernelGGL(kfn, grid, dim3(kThreads), shm, s, bits, m, rw, row0, nch, wts, mask, ep, o,
+                     nblocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_f32(int NT, int grid, hipStream_t s, const uint64_t* bits, int64_t m,
+                           int64_t rw, int64_t row0, const float* wts, const uint64_t* mask,
+                           const EpiF32& ep, const ScanOut& o) {
+  const int nch = (int)(rw / 4);
+  const int64_t per_block = (int64_t)kWaves * 16 * 4;
+  const int64_t nblocks = (m + per_block - 1) / per_block;
+  const size_t shm = (size_t)64 * NT * 64 * 4;
+  dim3 g((unsigned)(nblocks < grid ? nblocks : grid));
+  switch (NT) {
+#define KG_CASE(X) \
+  case X:          \
+    return launch_f32_nt<X>(g, shm, s, bits, m, rw, row0, nch, wts, mask, ep, o, nblocks);
+    KG_CASE(1) KG_CASE(2) KG_CASE(3) KG_CASE(4) KG_CASE(5) KG_CASE(6)
+#undef KG_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_refine(hipStream_t s, const Cand* cand, int64_t ncand, const uint64_t* bits,
+                         int64_t rw, int64_t row0, const uint64_t* maskw, int nwords,
+                         const double* B64, int nbf, int p, int t, int n, double sigma,
+                         double base_coef, const double* bpfull, const double* rbar,
+                         double* res) {
+  if (ncand <= 0) return hipSuccess;
+  int64_t blocks = (ncand + kWaves - 1) / kWaves;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(refine_f64, dim3((unsigned)blocks), dim3(kThreads), 0, s, cand, ncand, bits,
+                     rw, row0, maskw, nwords, B64, nbf, p, t, n, sigma, base_coef, bpfull, rbar,
+                     res);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(hipStream_t s, unsigned long long* p, int n) {
+  hipLaunchKernelGGL(zero_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_synth(hipStream_t s, uint64_t* out, uint64_t seed, int64_t kmer0, int64_t m,
+                        uint32_t n_acc, int64_t rw, const uint8_t* pop, const uint32_t* thr,
+                        uint32_t npop, uint32_t nprof) {
+  const int64_t total = m * rw;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 1 << 20) blocks = 1 << 20;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(synth_bits, dim3((unsigned)blocks), dim3(256), 0, s, out, seed, kmer0, m,
+                     n_acc, rw, pop, thr, npop, nprof);
+  return hipGetLastError();
+}
+
+}  // namespace kg

@@ -1,0 +1,62 @@
+"""CPU checks of the C ABI: the library loads, exports exactly what the header
+declares, and fails cleanly (no crash, a message) when no GPU is present."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from supergwas_amd import api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "kgwas_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(kg_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for s in ("kg_create", "kg_destroy", "kg_last_error", "kg_set_model", "kg_scan",
+              "kg_scan_device", "kg_stats_device", "kg_last_timing", "kg_synth_bits"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    L = api.lib()
+    for s in declared_symbols():
+        assert hasattr(L, s), s
+    assert set(declared_symbols()) == set(api.SIGNATURES)
+
+
+def test_hit_struct_layout_matches_header():
+    # kg_hit: int64, int32, int32, uint32, uint32, 5 doubles
+    assert api.HIT_DTYPE.itemsize == 64
+
+
+def test_create_without_gpu_reports_error():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(api.KgError) as ei:
+        api.KgwasHip()
+    assert ei.value.code == api.KG_ERR_DEVICE
+    assert "device" in str(ei.value)
+
+
+def test_bad_options_rejected():
+    L = api.lib()
+    h = ctypes.c_void_p()
+    opts = api._Opts(0, 7, 3)
+    assert L.kg_create(ctypes.byref(h), ctypes.byref(opts)) == api.KG_ERR_ARG
+    opts = api._Opts(0, 0, 9)
+    assert L.kg_create(ctypes.byref(h), ctypes.byref(opts)) == api.KG_ERR_ARG
+
+
+def test_null_context_calls():
+    L = api.lib()
+    assert L.kg_last_error(None) == b"null context"
+    assert L.kg_set_model(None, 1, None, 1, None, 0, None, 1) == api.KG_ERR_ARG
+    L.kg_destroy(None)

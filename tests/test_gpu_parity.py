@@ -1,0 +1,198 @@
+"""GPU parity of the HIP scan against the FP64 oracle (and the FP32 CPU restatement).
+
+Bar (per the workload definition): allele counts bit-exact, reported k-mers
+identical to the FP64 oracle's outside a +-1e-6 relative band around the
+thresholds, reported statistics equal to FP64 (they are FP64-refined), and the
+bulk statistic of every k-mer within 1e-5 relative error (|T - T64| / max(T64, 1)).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from supergwas_amd import synth
+from tests.helpers import T_from, expected_hits, pearson, rel_err_T
+
+pytestmark = pytest.mark.gpu
+
+STAT_TOL = 1e-5      # bulk-kernel statistic, relative (|T - T64| / max(T64, 1))
+REPORT_TOL = 1e-9    # FP64-refined reported values
+BAND = 1e-6          # threshold band inside which a flip is not counted
+
+_DS = {}
+
+
+def dataset(key, **kw):
+    if key not in _DS:
+        _DS[key] = synth.make_dataset(**kw)
+    return _DS[key]
+
+
+CASES = {
+    # name: (dataset kwargs, m, corr, log10, log10_2d)
+    "small_n190_p5": (dict(seed=0, n_acc=200, n_missing=10, npop=3, p=5, m=30000, n_causal=5,
+                           effect=0.5), 30000, 0.2, 2.0, 3.0),
+    "n1024_p20": (dict(seed=1, n_acc=1024, n_missing=50, npop=8, p=20, m=8269, n_causal=20,
+                       effect_sd=0.3), 8269, 0.2, 3.0, 5.0),
+    "n2048_p32": (dict(seed=2, n_acc=2048, n_missing=0, npop=8, p=32, m=12000, n_causal=50,
+                       effect_sd=0.3), 12000, 0.2, 4.0, 8.0),
+    "n333_nointercept": (dict(seed=3, n_acc=333, n_missing=7, npop=4, p=6, m=5001, n_causal=5,
+                              effect=0.5, intercept=False), 5001, 0.1, 2.0, 4.0),
+}
+
+
+def _run_case(name, precision="mixed", limbs=3):
+    from supergwas_amd.api import KgwasHip, ScanParams
+    kw, m, corr, l10, l2d = CASES[name]
+    ds = dataset(name, **kw)
+    bits = ds.fill_bits(0, m)
+    ref = oracle.scan_f64(bits, ds.idxs, ds.y, ds.U, corr)
+    with KgwasHip(precision=precision, limbs=limbs) as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        hits, npass = k.scan(bits, ScanParams(corr, l10, l2d))
+        tim = k.timing()
+    return ds, bits, ref, hits, npass, (corr, l10, l2d), tim
+
+
+def _band_sets(ds, bits, ref, corr, l10):
+    cor = pearson(bits, ds.idxs, ds.y)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        near_c = np.abs(np.abs(cor) / corr - 1.0) <= BAND
+        near_p = np.abs(ref["p"] / 10.0 ** (-l10) - 1.0) <= BAND
+    return near_c, near_p
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("limbs", [3, 2])
+def test_mixed_scan_matches_oracle(built, name, limbs):
+    ds, bits, ref, hits, npass, (corr, l10, l2d), tim = _run_case(name, "mixed", limbs)
+    print(f"[case] {name} limbs={limbs} n={ds.n} p={ds.p} m={len(bits)} "
+          f"hits={len(hits)} passed={npass[0]} oracle_passed={ref['n_passed']} "
+          f"candidates={tim['candidates']}", flush=True)
+    near_c, near_p = _band_sets(ds, bits, ref, corr, l10)
+    # pre-filter count: identical up to k-mers inside the band
+    assert abs(int(npass[0]) - ref["n_passed"]) <= int(near_c.sum())
+    exp_idx, exp_2d = expected_hits(ref, l10, l2d)
+    got_idx = hits["idx"]
+    assert np.all(np.diff(got_idx) > 0), "hits must be sorted by k-mer index"
+    sym = np.setxor1d(exp_idx, got_idx)
+    assert np.all(near_c[sym] | near_p[sym]), f"flips outside band: {sym[:10]}"
+    common, ie, ig = np.intersect1d(exp_idx, got_idx, return_indices=True)
+    h = hits[ig]
+    assert np.array_equal(h["c"], ref["c"][common])
+    for key, tol in (("T", REPORT_TOL), ("effect", REPORT_TOL)):
+        err = np.abs(h[key] - ref[key][common]) / np.maximum(np.abs(ref[key][common]), 1e-300)
+        assert err.max(initial=0) <= tol, (key, err.max())
+    perr = np.abs(h["pval"] - ref["p"][common]) / np.maximum(ref["p"][common], 1e-300)
+    assert perr.max(initial=0) <= 1e-7
+    assert np.array_equal((h["flags"] & 1).astype(bool), exp_2d[ie])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_mixed_bulk_statistic_error(built, name):
+    """Full-statistic mode: bulk (int8-limb MFMA + FP64 epilogue) values of every k-mer."""
+    import torch
+    from supergwas_amd.api import KgwasHip
+    kw, m, corr, l10, l2d = CASES[name]
+    ds = dataset(name, **kw)
+    bits = ds.fill_bits(0, m)
+    ref = oracle.scan_f64(bits, ds.idxs, ds.y, ds.U, corr)
+    dev = torch.from_numpy(bits.view(np.int64)).to("cuda")
+    torch.cuda.synchronize()
+    with KgwasHip(precision="mixed") as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        st = k.stats_device(dev.data_ptr(), m, bits.shape[1])
+    assert np.array_equal(st["c"], ref["c"]), "allele counts must be bit-exact"
+    yyM = oracle.yyM(ds.y, ds.U)
+    T = T_from(st["xxM"], st["xyM"][:, 0], yyM, ds.n)
+    err = rel_err_T(T, ref["T"])
+    print(f"[stat] {name}: max rel err T = {err.max():.3e}", flush=True)
+    assert err.max() <= STAT_TOL
+
+
+def test_fp32_matches_cpu_restatement(built):
+    """Plain FP32 kernel vs the FP32 CPU restatement of run_cols: same semantics,
+    summation order differs, so compare within FP32 tolerance."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    name = "small_n190_p5"
+    kw, m, corr, l10, l2d = CASES[name]
+    ds = dataset(name, **kw)
+    bits = ds.fill_bits(0, m)
+    X = oracle.unpack_rows(bits, ds.idxs)
+    cpu = oracle.cols_fp32(X, ds.y, ds.U, corr, l10, l2d, 4)
+    import torch
+    dev = torch.from_numpy(bits.view(np.int64)).to("cuda")
+    torch.cuda.synchronize()
+    with KgwasHip(precision="fp32") as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        hits, npass = k.scan(bits, ScanParams(corr, l10, l2d))
+        st = k.stats_device(dev.data_ptr(), m, bits.shape[1])
+    assert abs(int(npass[0]) - cpu["n_passed"]) <= max(2, cpu["n_passed"] // 1000)
+    kept = cpu["keep"]
+    scale = np.maximum(np.abs(ref_c := st["c"][kept]).astype(np.float64), 1.0)
+    assert np.max(np.abs(st["xxM"][kept] - cpu["xxM"]) / scale) < 1e-4
+    assert np.max(np.abs(st["xyM"][kept, 0] - cpu["xyM"]) / scale) < 1e-4
+    assert abs(len(hits) - cpu["hits_1d"]) <= max(2, cpu["hits_1d"] // 100)
+
+
+def test_synth_device_matches_host(built):
+    import torch
+    from supergwas_amd.api import KgwasHip
+    ds = dataset("n333", seed=5, n_acc=333, n_missing=3, npop=4, p=3, m=1000)
+    rw = synth.row_words_for(ds.n_acc)
+    m = 777
+    dev = torch.zeros((m, rw), dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    with KgwasHip() as k:
+        k.synth_bits(dev.data_ptr(), ds, 12345, m, rw)
+    got = dev.cpu().numpy().view(np.uint64)
+    assert np.array_equal(got, ds.fill_bits(12345, m, rw))
+
+
+def test_edge_cases(built):
+    from supergwas_amd.api import KgwasHip, ScanParams
+    ds = dataset("small_n190_p5", **CASES["small_n190_p5"][0])
+    rw = synth.row_words_for(ds.n_acc)
+    bits = ds.fill_bits(0, 300)
+    # monomorphic rows over the phenotyped samples: all zero / all one
+    bits[0, :] = 0
+    allone = np.zeros(rw, np.uint64)
+    for a in ds.idxs:
+        allone[a >> 6] |= np.uint64(1) << np.uint64(a & 63)
+    bits[1, :] = allone
+    ref = oracle.scan_f64(bits, ds.idxs, ds.y, ds.U, 0.0)
+    with KgwasHip() as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        # empty scan
+        h, npass = k.scan(bits[:0], ScanParams(0.2, 6, 15))
+        assert len(h) == 0 and npass[0] == 0
+        # tau = 0: every polymorphic k-mer passes; L = 0: every pass is reported
+        h, npass = k.scan(bits, ScanParams(0.0, 0.0, 0.0))
+        poly = (ref["c"] > 0) & (ref["c"] < ds.n)
+        assert npass[0] == poly.sum() == ref["n_passed"]
+        assert np.array_equal(h["idx"], np.nonzero(poly)[0])
+        # one k-mer
+        h, npass = k.scan(bits[5:6], ScanParams(0.0, 0.0, 0.0))
+        assert len(h) == 1 and h["idx"][0] == 0 and h["c"][0] == ref["c"][5]
+    # constant phenotype: |y - mean| = 0 -> nothing passes (src/kgwas.cpp:465)
+    with KgwasHip() as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, np.full(ds.n, 2.5))
+        h, npass = k.scan(bits, ScanParams(0.2, 0.0, 0.0))
+        assert npass[0] == 0 and len(h) == 0
+
+
+def test_multi_trait_equals_single_trait_runs(built):
+    from supergwas_amd.api import KgwasHip, ScanParams
+    ds = synth.make_dataset(seed=7, n_acc=512, n_missing=20, npop=4, p=8, m=4000, n_causal=10,
+                            effect_sd=0.5, traits=3, trait_corr=0.2)
+    bits = ds.fill_bits(0, 4000)
+    prm = ScanParams(0.15, 2.0, 4.0)
+    with KgwasHip() as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.Y)
+        hits, npass = k.scan(bits, prm)
+    for tr in range(3):
+        ref = oracle.scan_f64(bits, ds.idxs, ds.Y[:, tr], ds.U, 0.15)
+        exp_idx, _ = expected_hits(ref, 2.0, 4.0)
+        got = hits[hits["trait"] == tr]
+        assert npass[tr] == ref["n_passed"]
+        assert np.array_equal(got["idx"], exp_idx)
+        assert np.allclose(got["T"], ref["T"][exp_idx], rtol=1e-9, atol=0)
