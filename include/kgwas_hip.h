@@ -58,7 +58,7 @@ typedef struct kg_hit {
   int32_t trait;    /* phenotype column */
   int32_t c;        /* allele count over phenotyped samples (x'x) */
   uint32_t flags;   /* KG_HIT_2D */
-  uint32_t pad;
+  int32_t shard;    /* shard index (kg_scan_shards), else 0 */
   double xxM;       /* x'x - |U'x|^2 */
   double xyM;       /* x'y - (U'x)'(U'y) */
   double effect;    /* xyM / (xxM * yyM) */
@@ -106,6 +106,34 @@ int kg_scan(kg_ctx* ctx, const uint64_t* bits, int64_t m, int64_t row_words,
 int kg_scan_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t row_words,
                    const kg_scan_params* prm, kg_hit* out, int64_t cap, int64_t* n_out,
                    int64_t* n_passed);
+
+/* A further null model (its own sample mask, U and phenotypes) in the same
+ * context; *model_id receives its index (kg_set_model's model is 0).  Used by
+ * batched scans whose shards have different phenotyped samples, as separate
+ * kgwas runs over batch_* directories would. */
+int kg_add_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int32_t n, const double* U,
+                 int32_t p, const double* Y, int32_t t, int32_t* model_id);
+
+/* One shard of a batched scan: device rows and the model they are tested with. */
+typedef struct kg_shard {
+  const uint64_t* d_bits;
+  int64_t m;
+  int64_t row_words;
+  int32_t model;
+  int32_t pad;
+} kg_shard;
+
+#define KG_SHARDS_PER_LAUNCH 1u  /* one bulk launch per shard (comparison mode) */
+#define KG_SHARDS_GRAPH 2u       /* capture the device work once as a HIP graph, replay it */
+
+/* Scans many shards of uneven size in one persistent launch (the reference
+ * runs one process per batch_* directory: scripts/run_pipeline.sh:243-253).
+ * All shards' models must share p, t and the limb count.  Hits carry their
+ * shard index and are sorted by (shard, idx, trait); n_passed[s * t + tr] is
+ * the per-shard `.count` value. */
+int kg_scan_shards(kg_ctx* ctx, const kg_shard* shards, int32_t nshards,
+                   const kg_scan_params* prm, uint32_t flags, kg_hit* out, int64_t cap,
+                   int64_t* n_out, int64_t* n_passed);
 
 /* Full-statistic mode (validation and checksums): the bulk kernel's values for
  * every k-mer, written to host arrays: c[m], xxM[m], xyM[m*t], S[m*t]

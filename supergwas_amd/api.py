@@ -22,7 +22,7 @@ PRECISIONS = {"mixed": 0, "fp32": 1}
 HIT_2D = 1
 
 HIT_DTYPE = np.dtype([("idx", np.int64), ("trait", np.int32), ("c", np.int32),
-                      ("flags", np.uint32), ("pad", np.uint32), ("xxM", np.float64),
+                      ("flags", np.uint32), ("shard", np.int32), ("xxM", np.float64),
                       ("xyM", np.float64), ("effect", np.float64), ("T", np.float64),
                       ("pval", np.float64)])
 
@@ -47,6 +47,16 @@ class ScanParams(ctypes.Structure):
         super().__init__(corr_threshold, log10_thresh, log10_thresh_2d)
 
 
+class Shard(ctypes.Structure):
+    """One shard of a batched scan (kg_shard): device rows + model id."""
+    _fields_ = [("d_bits", ctypes.c_void_p), ("m", ctypes.c_int64), ("row_words", ctypes.c_int64),
+                ("model", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+SHARDS_PER_LAUNCH = 1
+SHARDS_GRAPH = 2
+
+
 class _Timing(ctypes.Structure):
     _fields_ = [("bulk_ms", ctypes.c_double), ("refine_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("candidates", ctypes.c_int64),
@@ -67,6 +77,10 @@ SIGNATURES = {
     "kg_scan": ([_vp, _vp, _i64, _i64, ctypes.POINTER(ScanParams), _vp, _i64,
                  ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "kg_scan_device": ([_vp, _vp, _i64, _i64, ctypes.POINTER(ScanParams), _vp, _i64,
+                        ctypes.POINTER(_i64), _vp], ctypes.c_int),
+    "kg_add_model": ([_vp, _i32, _vp, _i32, _vp, _i32, _vp, _i32, ctypes.POINTER(_i32)],
+                     ctypes.c_int),
+    "kg_scan_shards": ([_vp, _vp, _i32, ctypes.POINTER(ScanParams), ctypes.c_uint32, _vp, _i64,
                         ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "kg_stats_device": ([_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp], ctypes.c_int),
     "kg_last_timing": ([_vp, ctypes.POINTER(_Timing)], ctypes.c_int),
@@ -148,6 +162,37 @@ class KgwasHip:
         self._check(lib().kg_set_model(self._h, int(n_acc), _p(idxs), len(idxs),
                                        _p(U) if U.shape[1] else None, U.shape[1], _p(Y),
                                        Y.shape[1]))
+
+    def add_model(self, n_acc, idxs, U, Y):
+        """A further null model (own sample mask / U / phenotypes); returns its id."""
+        idxs = np.ascontiguousarray(idxs, np.int32)
+        U = np.asfortranarray(np.asarray(U, np.float64).reshape(len(idxs), -1))
+        Y = np.asarray(Y, np.float64)
+        if Y.ndim == 1:
+            Y = Y[:, None]
+        Y = np.asfortranarray(Y)
+        mid = _i32(0)
+        self._check(lib().kg_add_model(self._h, int(n_acc), _p(idxs), len(idxs),
+                                       _p(U) if U.shape[1] else None, U.shape[1], _p(Y),
+                                       Y.shape[1], ctypes.byref(mid)))
+        return mid.value
+
+    def scan_shards(self, shards, params=None, flags=0, cap=1 << 16):
+        """shards: list of (dev_ptr, m, row_words, model_id).  Returns (hits, n_passed[s, t])."""
+        params = params or ScanParams()
+        arr = (Shard * len(shards))(*[Shard(p, m, rw, mid, 0) for (p, m, rw, mid) in shards])
+        np_ = np.zeros(len(shards) * max(self.t, 1), np.int64)
+        while True:
+            out = np.zeros(max(cap, 1), HIT_DTYPE)
+            n_out = _i64(0)
+            rc = lib().kg_scan_shards(self._h, ctypes.cast(arr, _vp), len(shards),
+                                      ctypes.byref(params), flags, _p(out), cap,
+                                      ctypes.byref(n_out), _p(np_))
+            if rc == KG_ERR_CAPACITY:
+                cap = int(n_out.value)
+                continue
+            self._check(rc)
+            return out[:n_out.value], np_.reshape(len(shards), -1).copy()
 
     def _scan(self, fn, ptr, m, row_words, params, cap):
         params = params or ScanParams()

@@ -1,6 +1,7 @@
 // C ABI of libkgwas_hip: null-model set-up in FP64, weight quantisation and
-// layout, scan orchestration on one HIP stream, and the final per-k-mer
-// decisions with the reference's own compute_hit arithmetic.
+// layout, scan orchestration on one HIP stream (optionally replayed as a HIP
+// graph), and the final per-k-mer decisions with the reference's own
+// compute_hit arithmetic.
 //
 // Reference seam replaced (src/kgwas.cpp):
 //   183-218  Pearson pre-filter      -> bulk kernel bound test + FP64 host test
@@ -13,6 +14,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -20,17 +22,15 @@
 #include "kg_internal.h"
 
 namespace kg {
-hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const uint64_t* bits,
-                          int64_t m, int64_t rw, int64_t row0, const void* wts, const void* mask,
-                          const EpiI8& ep, const ScanOut& o);
+hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const ShardDesc* sd,
+                          int nshards, int64_t total_blocks, const ScanOut& o);
+int64_t i8_block_kmers(int R);
+hipError_t set_kernel_attributes();
 hipError_t launch_scan_f32(int NT, int grid, hipStream_t s, const uint64_t* bits, int64_t m,
                            int64_t rw, int64_t row0, const float* wts, const uint64_t* mask,
                            const EpiF32& ep, const ScanOut& o);
-hipError_t launch_refine(hipStream_t s, const Cand* cand, int64_t ncand, const uint64_t* bits,
-                         int64_t rw, int64_t row0, const uint64_t* maskw, int nwords,
-                         const double* B64, int nbf, int p, int t, int n, double sigma,
-                         double base_coef, const double* bpfull, const double* rbar,
-                         double* res);
+hipError_t launch_refine(hipStream_t s, const Cand* cand, const unsigned long long* count,
+                         int64_t cap, const RefDesc* rd, double* res, int blocks);
 hipError_t launch_zero(hipStream_t s, unsigned long long* p, int n);
 hipError_t launch_synth(hipStream_t s, uint64_t* out, uint64_t seed, int64_t kmer0, int64_t m,
                         uint32_t n_acc, int64_t rw, const uint8_t* pop, const uint32_t* thr,
@@ -45,6 +45,9 @@ namespace {
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
   ~DBuf() {
     if (p) (void)hipFree(p);
   }
@@ -110,6 +113,16 @@ struct Model {
   int nch_i8 = 0, nch_f32 = 0;
 };
 
+// A null model with its device-side weights and constants.
+struct ModelDev {
+  Model md;
+  int NT = 0, nz = 0, nb = 0;
+  DBuf wts, mask, colw, colh, bp, bp0, rbar, ycn, yyM, B64, bpfull;  // int8 path
+  DBuf wf32, uyf;                                                     // FP32 path
+  EpiI8 epi{};
+  EpiF32 epf{};
+};
+
 }  // namespace
 
 struct kg_ctx {
@@ -120,22 +133,18 @@ struct kg_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::string err;
-  bool have_model = false;
-  Model md;
-  // i8 path
-  int NT = 0, nz = 0, nb = 0;
-  DBuf wts, mask, colw, colh, bp, bp0, rbar, ycn, yyM;
-  DBuf B64, bpfull;
-  // f32 path
-  DBuf wf32, uyf;
-  EpiF32 epf{};
-  EpiI8 epi{};
+  std::vector<std::unique_ptr<ModelDev>> models;
   // scratch
-  DBuf counters, cand, res, stage, st_c, st_xx, st_xy, st_S;
+  DBuf counters, cand, res, stage, st_c, st_xx, st_xy, st_S, sdesc, sdesc1, rdesc;
   int64_t cand_cap = 0;
+  size_t counters_len = 0;
   kg_timing timing{};
   int R = 4;
   int grid_mult = 8;
+  // captured graph of the last batched scan
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<char> graph_key;
 };
 
 #define KG_CK(ctx, expr)                                                            \
@@ -295,8 +304,8 @@ static void quantize(const double* col, int n, int L, std::vector<int8_t>& limbs
   *h = 0.5 * w[L - 1] + 1e-15 * scale;
 }
 
-static int build_i8(kg_ctx* ctx) {
-  Model& md = ctx->md;
+static int build_i8(kg_ctx* ctx, ModelDev& M) {
+  Model& md = M.md;
   const int n = md.n, p = md.p, t = md.t, L = ctx->limbs;
   std::vector<int> zc;
   for (int k = 0; k < p; ++k)
@@ -309,9 +318,9 @@ static int build_i8(kg_ctx* ctx) {
     return fail(ctx, KG_ERR_ARG,
                 "weight columns exceed the int8 kernel's 128-column tile budget (p + t too "
                 "large for this limb count)");
-  ctx->NT = NT;
-  ctx->nz = nz;
-  ctx->nb = nb;
+  M.NT = NT;
+  M.nz = nz;
+  M.nb = nb;
   std::vector<int8_t> lim;
   std::vector<double> colw((size_t)nb * L), colh(nb);
   std::vector<int8_t> ucol((size_t)n * NU, 0);  // [sample][u-col]
@@ -357,18 +366,18 @@ static int build_i8(kg_ctx* ctx) {
     if (e != hipSuccess) return e;
     return hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice);
   };
-  KG_CK(ctx, up(ctx->wts, W.data(), W.size()));
-  KG_CK(ctx, up(ctx->mask, md.maskw.data(), md.maskw.size() * 8));
-  KG_CK(ctx, up(ctx->colw, colw.data(), colw.size() * 8));
-  KG_CK(ctx, up(ctx->colh, colh.data(), colh.size() * 8));
-  KG_CK(ctx, up(ctx->bp, bp.data(), bp.size() * 8));
-  KG_CK(ctx, up(ctx->bp0, bp0.data(), bp0.size() * 8));
-  KG_CK(ctx, up(ctx->rbar, md.rbar.data(), t * 8));
-  KG_CK(ctx, up(ctx->ycn, md.ycn.data(), t * 8));
-  KG_CK(ctx, up(ctx->yyM, md.yyM.data(), t * 8));
-  KG_CK(ctx, up(ctx->B64, B.data(), B.size() * 8));
-  KG_CK(ctx, up(ctx->bpfull, md.bprime.data(), std::max<size_t>(md.bprime.size(), 1) * 8));
-  EpiI8& ep = ctx->epi;
+  KG_CK(ctx, up(M.wts, W.data(), W.size()));
+  KG_CK(ctx, up(M.mask, md.maskw.data(), md.maskw.size() * 8));
+  KG_CK(ctx, up(M.colw, colw.data(), colw.size() * 8));
+  KG_CK(ctx, up(M.colh, colh.data(), colh.size() * 8));
+  KG_CK(ctx, up(M.bp, bp.data(), bp.size() * 8));
+  KG_CK(ctx, up(M.bp0, bp0.data(), bp0.size() * 8));
+  KG_CK(ctx, up(M.rbar, md.rbar.data(), t * 8));
+  KG_CK(ctx, up(M.ycn, md.ycn.data(), t * 8));
+  KG_CK(ctx, up(M.yyM, md.yyM.data(), t * 8));
+  KG_CK(ctx, up(M.B64, B.data(), B.size() * 8));
+  KG_CK(ctx, up(M.bpfull, md.bprime.data(), std::max<size_t>(md.bprime.size(), 1) * 8));
+  EpiI8& ep = M.epi;
   ep.n = n;
   ep.nz = nz;
   ep.t = t;
@@ -378,26 +387,26 @@ static int build_i8(kg_ctx* ctx) {
   ep.sigma = md.sigma;
   ep.base_coef = md.base_coef;
   ep.dmax_drop = (md.rotated && md.drop_d) ? md.dmax : 0.0;
-  ep.colw = ctx->colw.as<double>();
-  ep.colh = ctx->colh.as<double>();
-  ep.bp = ctx->bp.as<double>();
-  ep.bp0 = ctx->bp0.as<double>();
-  ep.rbar = ctx->rbar.as<double>();
-  ep.ycn = ctx->ycn.as<double>();
-  ep.yyM = ctx->yyM.as<double>();
+  ep.colw = M.colw.as<double>();
+  ep.colh = M.colh.as<double>();
+  ep.bp = M.bp.as<double>();
+  ep.bp0 = M.bp0.as<double>();
+  ep.rbar = M.rbar.as<double>();
+  ep.ycn = M.ycn.as<double>();
+  ep.yyM = M.yyM.as<double>();
   return KG_OK;
 }
 
-static int build_f32(kg_ctx* ctx) {
-  Model& md = ctx->md;
+static int build_f32(kg_ctx* ctx, ModelDev& M) {
+  Model& md = M.md;
   const int n = md.n, p = md.p, t = md.t;
   const int ncol = p + 2 * t;
   const int NT = (ncol + 15) / 16;
   if (NT > 6) return fail(ctx, KG_ERR_ARG, "p + 2t exceeds the FP32 kernel's 96 columns");
-  ctx->NT = NT;
+  M.NT = NT;
   std::vector<float> colv((size_t)n * 16 * NT, 0.f);  // [sample][col]
   const int NU = 16 * NT;
-  EpiF32& ep = ctx->epf;
+  EpiF32& ep = M.epf;
   memset(&ep, 0, sizeof(ep));
   std::vector<float> Uyf((size_t)t * (p > 0 ? p : 1), 0.f);
   for (int tr = 0; tr < t; ++tr) {
@@ -444,17 +453,17 @@ static int build_f32(kg_ctx* ctx) {
             W[(size_t)C * frag + ((size_t)e * NT + tt) * 64 + lane] =
                 colv[(size_t)samp[a] * NU + 16 * tt + i];
         }
-  KG_CK(ctx, ctx->wf32.ensure(W.size() * 4));
-  KG_CK(ctx, hipMemcpy(ctx->wf32.p, W.data(), W.size() * 4, hipMemcpyHostToDevice));
-  KG_CK(ctx, ctx->mask.ensure(md.maskw.size() * 8));
-  KG_CK(ctx, hipMemcpy(ctx->mask.p, md.maskw.data(), md.maskw.size() * 8, hipMemcpyHostToDevice));
-  KG_CK(ctx, ctx->uyf.ensure(Uyf.size() * 4));
-  KG_CK(ctx, hipMemcpy(ctx->uyf.p, Uyf.data(), Uyf.size() * 4, hipMemcpyHostToDevice));
+  KG_CK(ctx, M.wf32.ensure(W.size() * 4));
+  KG_CK(ctx, hipMemcpy(M.wf32.p, W.data(), W.size() * 4, hipMemcpyHostToDevice));
+  KG_CK(ctx, M.mask.ensure(md.maskw.size() * 8));
+  KG_CK(ctx, hipMemcpy(M.mask.p, md.maskw.data(), md.maskw.size() * 8, hipMemcpyHostToDevice));
+  KG_CK(ctx, M.uyf.ensure(Uyf.size() * 4));
+  KG_CK(ctx, hipMemcpy(M.uyf.p, Uyf.data(), Uyf.size() * 4, hipMemcpyHostToDevice));
   ep.n = n;
   ep.p = p;
   ep.t = t;
   ep.NU = NU;
-  ep.Uyf = ctx->uyf.as<float>();
+  ep.Uyf = M.uyf.as<float>();
   return KG_OK;
 }
 
@@ -491,7 +500,7 @@ extern "C" int kg_create(kg_ctx** out, const kg_opts* opts) {
   ctx->cus = prop.multiProcessorCount;
   KG_CK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   for (auto& e : ctx->ev) KG_CK(ctx, hipEventCreate(&e));
-  KG_CK(ctx, ctx->counters.ensure(sizeof(unsigned long long) * (1 + kMaxTraits)));
+  KG_CK(ctx, set_kernel_attributes());
   if (const char* v = std::getenv("KG_TILES_PER_WAVE")) {
     const int r = atoi(v);
     if (r == 2 || r == 4) ctx->R = r;
@@ -503,12 +512,22 @@ extern "C" int kg_create(kg_ctx** out, const kg_opts* opts) {
   return KG_OK;
 }
 
+static void drop_graph(kg_ctx* ctx) {
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->gexec = nullptr;
+  ctx->graph = nullptr;
+  ctx->graph_key.clear();
+}
+
 extern "C" void kg_destroy(kg_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
   }
+  drop_graph(ctx);
+  ctx->models.clear();
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -519,20 +538,21 @@ extern "C" const char* kg_last_error(const kg_ctx* ctx) {
   return ctx ? ctx->err.c_str() : "null context";
 }
 
-extern "C" int kg_set_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int32_t n,
-                            const double* U, int32_t p, const double* Y, int32_t t) {
+static int add_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int32_t n, const double* U,
+                     int32_t p, const double* Y, int32_t t, int32_t* model_id) {
   if (!ctx) return KG_ERR_ARG;
   if (!ctx->stream) return fail(ctx, KG_ERR_STATE, "context has no device");
   if (n_acc <= 0 || n <= 0 || n > n_acc || p < 0 || t < 1 || t > kMaxTraits || !idxs || !Y ||
       (p > 0 && !U))
-    return fail(ctx, KG_ERR_ARG, "kg_set_model: bad shape (need 0 < n <= n_acc, p >= 0, 1 <= t <= 8)");
+    return fail(ctx, KG_ERR_ARG,
+                "kg_set_model: bad shape (need 0 < n <= n_acc, p >= 0, 1 <= t <= 8)");
   for (int j = 0; j < n; ++j) {
     if (idxs[j] < 0 || idxs[j] >= n_acc || (j > 0 && idxs[j] <= idxs[j - 1]))
       return fail(ctx, KG_ERR_ARG, "kg_set_model: idxs must be increasing accession positions");
   }
   KG_CK(ctx, hipSetDevice(ctx->device));
-  Model& md = ctx->md;
-  md = Model();
+  auto M = std::make_unique<ModelDev>();
+  Model& md = M->md;
   md.n_acc = n_acc;
   md.n = n;
   md.p = p;
@@ -545,11 +565,27 @@ extern "C" int kg_set_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int
   md.maskw.assign((size_t)8 * md.nch_i8, 0ull);
   for (int j = 0; j < n; ++j) md.maskw[idxs[j] >> 6] |= 1ull << (idxs[j] & 63);
   prepare_model(md);
-  ctx->have_model = false;
-  const int rc = (ctx->precision == KG_PREC_MIXED) ? build_i8(ctx) : build_f32(ctx);
+  const int rc = (ctx->precision == KG_PREC_MIXED) ? build_i8(ctx, *M) : build_f32(ctx, *M);
   if (rc != KG_OK) return rc;
-  ctx->have_model = true;
+  drop_graph(ctx);
+  ctx->models.push_back(std::move(M));
+  if (model_id) *model_id = (int32_t)ctx->models.size() - 1;
   return KG_OK;
+}
+
+extern "C" int kg_set_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int32_t n,
+                            const double* U, int32_t p, const double* Y, int32_t t) {
+  if (!ctx) return KG_ERR_ARG;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
+  ctx->models.clear();
+  return add_model(ctx, n_acc, idxs, n, U, p, Y, t, nullptr);
+}
+
+extern "C" int kg_add_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int32_t n,
+                            const double* U, int32_t p, const double* Y, int32_t t,
+                            int32_t* model_id) {
+  return add_model(ctx, n_acc, idxs, n, U, p, Y, t, model_id);
 }
 
 // Final decision for one candidate (k-mer idx, trait) with FP64 values.
@@ -559,9 +595,8 @@ struct Decided {
   kg_hit h;
 };
 
-static Decided decide(const kg_ctx* ctx, int64_t idx, int tr, int c, double xxM, double xyM,
+static Decided decide(const Model& md, int64_t idx, int tr, int c, double xxM, double xyM,
                       double S, bool have_S, const kg_scan_params& prm) {
-  const Model& md = ctx->md;
   Decided d{};
   d.pass = true;
   if (have_S) {  // FP64 pre-filter, src/kgwas.cpp:462-468
@@ -593,22 +628,126 @@ static Decided decide(const kg_ctx* ctx, int64_t idx, int tr, int c, double xxM,
   return d;
 }
 
-// One bulk pass over device rows [0, m) (+ refinement), appending hits with
-// k-mer indices offset by idx0.
-static int scan_core(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t rw, int64_t idx0,
-                     const kg_scan_params& prm, std::vector<kg_hit>& hits, int64_t* n_passed,
-                     bool stats) {
-  Model& md = ctx->md;
-  const int t = md.t;
+// A shard as the host sees it.
+struct HShard {
+  const uint64_t* bits;
+  int64_t m, rw, idx0;
+  int model;
+};
+
+static int check_shard(kg_ctx* ctx, const HShard& s) {
+  if (s.model < 0 || s.model >= (int)ctx->models.size())
+    return fail(ctx, KG_ERR_ARG, "shard refers to an unknown model");
+  const Model& md = ctx->models[s.model]->md;
+  if (s.m < 0) return fail(ctx, KG_ERR_ARG, "negative k-mer count");
+  if (s.m > 0 && !s.bits) return fail(ctx, KG_ERR_ARG, "null bits");
+  if (s.rw % 8 != 0 || s.rw < 8 * (int64_t)md.nch_i8)
+    return fail(ctx, KG_ERR_ARG,
+                "row_words must be a multiple of 8 and cover all accessions (>= " +
+                    std::to_string(8 * md.nch_i8) + ")");
+  if (((uintptr_t)s.bits & 15) != 0) return fail(ctx, KG_ERR_ARG, "bits must be 16-byte aligned");
+  return KG_OK;
+}
+
+// Device part of a batched int8 scan: zero counters, bulk kernel(s), refine.
+static int enqueue_i8(kg_ctx* ctx, int nsh, int64_t total_blocks, bool per_launch,
+                      const std::vector<int64_t>& nblk, const ScanOut& o, int ref_blocks) {
+  const int NT = ctx->models[0]->NT;
+  const int grid = ctx->cus * ctx->grid_mult;
+  KG_CK(ctx, launch_zero(ctx->stream, o.counters, (int)ctx->counters_len));
+  KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  if (!per_launch) {
+    KG_CK(ctx, launch_scan_i8(NT, ctx->R, grid, ctx->stream, ctx->sdesc.as<ShardDesc>(), nsh,
+                              total_blocks, o));
+  } else {
+    for (int s = 0; s < nsh; ++s) {
+      if (nblk[s] == 0) continue;
+      KG_CK(ctx, launch_scan_i8(NT, ctx->R, grid, ctx->stream, ctx->sdesc1.as<ShardDesc>() + s, 1,
+                                nblk[s], o));
+    }
+  }
+  KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  KG_CK(ctx, launch_refine(ctx->stream, o.cand, o.counters, o.cap, ctx->rdesc.as<RefDesc>(),
+                           ctx->res.as<double>(), ref_blocks));
+  KG_CK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  return KG_OK;
+}
+
+enum { kPerLaunch = 1, kGraph = 2 };
+
+// Scans every shard (int8 path), appending hits (idx offset by shard idx0,
+// shard index set) and adding pre-filter passes to n_passed[s * t + tr].
+static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_scan_params& prm,
+                          unsigned flags, std::vector<kg_hit>& hits, int64_t* n_passed,
+                          bool stats) {
+  const int nsh = (int)sh.size();
+  const int t = ctx->models[sh[0].model]->md.t;
+  const int NT = ctx->models[sh[0].model]->NT;
+  for (const auto& s : sh) {
+    const ModelDev& M = *ctx->models[s.model];
+    if (M.NT != NT || M.md.t != t)
+      return fail(ctx, KG_ERR_ARG, "all shards of one scan need the same p, t and limb count");
+  }
   const double p_thr = std::pow(10.0, -prm.log10_thresh);
   const double Ts = t_star(p_thr);
-  const double Tmax = (double)md.n * std::log(1.0 / (1.0 - (1 - 1e-15)));
-  if (ctx->cand_cap == 0) {
-    ctx->cand_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(m / 16 + 1024, 1 << 24));
-    KG_CK(ctx, ctx->cand.ensure(sizeof(Cand) * ctx->cand_cap));
+  const int64_t bk = i8_block_kmers(ctx->R);
+  std::vector<ShardDesc> sd(nsh), sd1(nsh);
+  std::vector<RefDesc> rd(nsh);
+  std::vector<int64_t> nblk(nsh);
+  int64_t total_blocks = 0;
+  ctx->counters_len = 1 + (size_t)nsh * t;
+  KG_CK(ctx, ctx->counters.ensure(sizeof(unsigned long long) * ctx->counters_len));
+  unsigned long long* dc = ctx->counters.as<unsigned long long>();
+  for (int s = 0; s < nsh; ++s) {
+    const ModelDev& M = *ctx->models[sh[s].model];
+    const Model& md = M.md;
+    ShardDesc& D = sd[s];
+    D.bits = sh[s].bits;
+    D.m = sh[s].m;
+    D.rw = sh[s].rw;
+    D.blk0 = total_blocks;
+    nblk[s] = (sh[s].m + bk - 1) / bk;
+    total_blocks += nblk[s];
+    D.wts = M.wts.p;
+    D.mask = M.mask.p;
+    D.nchunks = md.nch_i8;
+    D.pad = 0;
+    D.pass = dc + 1 + (size_t)s * t;
+    D.ep = M.epi;
+    D.ep.tau = prm.corr_threshold;
+    D.ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;
+    D.ep.all_sig = p_thr >= 1.0 ? 1 : 0;
+    const double Tmax = (double)md.n * std::log(1.0 / (1.0 - (1 - 1e-15)));
+    D.ep.r2star = (Ts > Tmax) ? INFINITY : -std::expm1(-Ts / (double)md.n);
+    sd1[s] = D;
+    sd1[s].blk0 = 0;
+    RefDesc& R = rd[s];
+    R.bits = sh[s].bits;
+    R.rw = sh[s].rw;
+    R.maskw = M.mask.as<uint64_t>();
+    R.B64 = M.B64.as<double>();
+    R.bpfull = M.bpfull.as<double>();
+    R.rbar = M.rbar.as<double>();
+    R.sigma = md.sigma;
+    R.base_coef = md.base_coef;
+    R.nwords = (md.n_acc + 63) / 64;
+    R.p = md.p;
+    R.t = md.t;
+    R.n = md.n;
   }
+  if (ctx->cand_cap == 0) {
+    int64_t mt = 0;
+    for (const auto& s : sh) mt += s.m;
+    ctx->cand_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(mt / 16 + 1024, 1 << 24));
+  }
+  KG_CK(ctx, ctx->cand.ensure(sizeof(Cand) * ctx->cand_cap));
+  KG_CK(ctx, ctx->res.ensure(sizeof(double) * 4 * ctx->cand_cap));
+  KG_CK(ctx, ctx->sdesc.ensure(sizeof(ShardDesc) * nsh));
+  KG_CK(ctx, ctx->sdesc1.ensure(sizeof(ShardDesc) * nsh));
+  KG_CK(ctx, ctx->rdesc.ensure(sizeof(RefDesc) * nsh));
   ScanOut o{};
   if (stats) {
+    const int64_t m = sh[0].m;
     KG_CK(ctx, ctx->st_c.ensure(sizeof(int32_t) * m));
     KG_CK(ctx, ctx->st_xx.ensure(sizeof(double) * m));
     KG_CK(ctx, ctx->st_xy.ensure(sizeof(double) * m * t));
@@ -618,32 +757,153 @@ static int scan_core(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t rw,
     o.st_xy = ctx->st_xy.as<double>();
     o.st_S = ctx->st_S.as<double>();
   }
+  const bool per_launch = (flags & kPerLaunch) != 0;
+  const bool use_graph = (flags & kGraph) != 0 && !stats;
+  const int ref_blocks = ctx->cus * 4;
+  std::vector<unsigned long long> cnt(ctx->counters_len);
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    o.cand = ctx->cand.as<Cand>();
+    o.counters = dc;
+    o.cap = ctx->cand_cap;
+    // graph key: everything the captured launches depend on
+    std::vector<char> key;
+    if (use_graph) {
+      const size_t kb = sizeof(ShardDesc) * nsh * 2 + sizeof(RefDesc) * nsh + sizeof(o) + 16;
+      key.resize(kb);
+      char* kp = key.data();
+      memcpy(kp, sd.data(), sizeof(ShardDesc) * nsh);
+      kp += sizeof(ShardDesc) * nsh;
+      memcpy(kp, sd1.data(), sizeof(ShardDesc) * nsh);
+      kp += sizeof(ShardDesc) * nsh;
+      memcpy(kp, rd.data(), sizeof(RefDesc) * nsh);
+      kp += sizeof(RefDesc) * nsh;
+      memcpy(kp, &o, sizeof(o));
+      kp += sizeof(o);
+      const int64_t extra[2] = {(int64_t)flags, (int64_t)ctx->R};
+      memcpy(kp, extra, 16);
+    }
+    const bool replay = use_graph && ctx->gexec && key == ctx->graph_key;
+    if (!replay) {
+      KG_CK(ctx, hipMemcpy(ctx->sdesc.p, sd.data(), sizeof(ShardDesc) * nsh, hipMemcpyHostToDevice));
+      KG_CK(ctx, hipMemcpy(ctx->sdesc1.p, sd1.data(), sizeof(ShardDesc) * nsh,
+                           hipMemcpyHostToDevice));
+      KG_CK(ctx, hipMemcpy(ctx->rdesc.p, rd.data(), sizeof(RefDesc) * nsh, hipMemcpyHostToDevice));
+    }
+    KG_CK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if (use_graph) {
+      if (!replay) {
+        drop_graph(ctx);
+        KG_CK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_i8(ctx, nsh, total_blocks, per_launch, nblk, o, ref_blocks);
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        if (rc != KG_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        KG_CK(ctx, ec);
+        ctx->graph = g;
+        KG_CK(ctx, hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
+        ctx->graph_key = key;
+      }
+      KG_CK(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
+      KG_CK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+    } else {
+      int rc = enqueue_i8(ctx, nsh, total_blocks, per_launch, nblk, o, ref_blocks);
+      if (rc != KG_OK) return rc;
+    }
+    KG_CK(ctx, hipMemcpyAsync(cnt.data(), dc, sizeof(unsigned long long) * cnt.size(),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    KG_CK(ctx, hipStreamSynchronize(ctx->stream));
+    if ((int64_t)cnt[0] <= ctx->cand_cap) break;
+    ctx->cand_cap = (int64_t)cnt[0] + (int64_t)cnt[0] / 4 + 1024;
+    KG_CK(ctx, ctx->cand.ensure(sizeof(Cand) * ctx->cand_cap));
+    KG_CK(ctx, ctx->res.ensure(sizeof(double) * 4 * ctx->cand_cap));
+    drop_graph(ctx);
+  }
+  const int64_t nc = std::min<int64_t>((int64_t)cnt[0], ctx->cand_cap);
+  float ms = 0.f;
+  if (use_graph) {
+    KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+    ctx->timing.bulk_ms = ms;  // graph: bulk + refine together
+    ctx->timing.refine_ms = 0.0;
+    ctx->timing.total_ms = ms;
+  } else {
+    KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+    ctx->timing.bulk_ms = ms;
+    KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+    ctx->timing.refine_ms = ms;
+    KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+    ctx->timing.total_ms = ms;
+  }
+  ctx->timing.candidates = nc;
+  int64_t mt = 0;
+  for (const auto& s : sh) mt += s.m;
+  ctx->timing.kmers = mt;
+  for (int s = 0; s < nsh; ++s)
+    for (int tr = 0; tr < t; ++tr) n_passed[s * t + tr] += (int64_t)cnt[1 + (size_t)s * t + tr];
+  if (nc == 0) return KG_OK;
+  std::vector<Cand> cands(nc);
+  std::vector<double> res((size_t)4 * nc);
+  KG_CK(ctx, hipMemcpy(cands.data(), ctx->cand.p, sizeof(Cand) * nc, hipMemcpyDeviceToHost));
+  KG_CK(ctx, hipMemcpy(res.data(), ctx->res.p, sizeof(double) * 4 * nc, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < nc; ++i) {
+    const Cand& cd = cands[i];
+    const HShard& s = sh[cd.shard];
+    const Model& md = ctx->models[s.model]->md;
+    const int c = (int)res[4 * i + 0];
+    Decided d = decide(md, s.idx0 + cd.idx, cd.trait, c, res[4 * i + 1], res[4 * i + 2],
+                       res[4 * i + 3], true, prm);
+    if ((cd.flags & kPfAmb) && d.pass) n_passed[cd.shard * t + cd.trait] += 1;
+    if (d.hit) {
+      d.h.shard = cd.shard;
+      hits.push_back(d.h);
+    }
+  }
+  return KG_OK;
+}
+
+// Plain FP32 path, one shard at a time.
+static int scan_f32_shard(kg_ctx* ctx, const HShard& s, int sidx, const kg_scan_params& prm,
+                          std::vector<kg_hit>& hits, int64_t* n_passed, bool stats) {
+  const ModelDev& M = *ctx->models[s.model];
+  const Model& md = M.md;
+  const int t = md.t;
+  const double p_thr = std::pow(10.0, -prm.log10_thresh);
+  const double Ts = t_star(p_thr);
+  const double Tmax = (double)md.n * std::log(1.0 / (1.0 - (1 - 1e-15)));
+  ctx->counters_len = 1 + (size_t)kMaxTraits;
+  KG_CK(ctx, ctx->counters.ensure(sizeof(unsigned long long) * ctx->counters_len));
+  if (ctx->cand_cap == 0)
+    ctx->cand_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(s.m / 16 + 1024, 1 << 24));
+  KG_CK(ctx, ctx->cand.ensure(sizeof(Cand) * ctx->cand_cap));
+  ScanOut o{};
+  if (stats) {
+    KG_CK(ctx, ctx->st_c.ensure(sizeof(int32_t) * s.m));
+    KG_CK(ctx, ctx->st_xx.ensure(sizeof(double) * s.m));
+    KG_CK(ctx, ctx->st_xy.ensure(sizeof(double) * s.m * t));
+    KG_CK(ctx, ctx->st_S.ensure(sizeof(double) * s.m * t));
+    o.st_c = ctx->st_c.as<int32_t>();
+    o.st_xx = ctx->st_xx.as<double>();
+    o.st_xy = ctx->st_xy.as<double>();
+    o.st_S = ctx->st_S.as<double>();
+  }
+  EpiF32 ep = M.epf;
+  ep.tau = (float)prm.corr_threshold;
+  ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;
+  ep.all_sig = p_thr >= 1.0 ? 1 : 0;
+  ep.Tstar_lo = (Ts > Tmax) ? INFINITY : Ts * (1.0 - 1e-9);
   unsigned long long cnt[1 + kMaxTraits];
   const int grid = ctx->cus * ctx->grid_mult;
-  for (int attempt = 0; attempt < 2; ++attempt) {
+  for (int attempt = 0; attempt < 3; ++attempt) {
     o.cand = ctx->cand.as<Cand>();
     o.counters = ctx->counters.as<unsigned long long>();
     o.cap = ctx->cand_cap;
     KG_CK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     KG_CK(ctx, launch_zero(ctx->stream, o.counters, 1 + kMaxTraits));
     KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    if (ctx->precision == KG_PREC_MIXED) {
-      EpiI8 ep = ctx->epi;
-      ep.tau = prm.corr_threshold;
-      ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;
-      ep.all_sig = p_thr >= 1.0 ? 1 : 0;
-      ep.r2star = (Ts > Tmax) ? INFINITY : -std::expm1(-Ts / (double)md.n);
-      KG_CK(ctx, launch_scan_i8(ctx->NT, ctx->R, grid, ctx->stream, d_bits, m, rw, 0,
-                                ctx->wts.p, ctx->mask.p, ep, o));
-    } else {
-      EpiF32 ep = ctx->epf;
-      ep.tau = (float)prm.corr_threshold;
-      ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;
-      ep.all_sig = p_thr >= 1.0 ? 1 : 0;
-      ep.Tstar_lo = (Ts > Tmax) ? INFINITY : Ts * (1.0 - 1e-9);
-      KG_CK(ctx, launch_scan_f32(ctx->NT, grid, ctx->stream, d_bits, m, rw, 0,
-                                 ctx->wf32.as<float>(), ctx->mask.as<uint64_t>(), ep, o));
-    }
+    KG_CK(ctx, launch_scan_f32(M.NT, grid, ctx->stream, s.bits, s.m, s.rw, 0, M.wf32.as<float>(),
+                               M.mask.as<uint64_t>(), ep, o));
     KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     KG_CK(ctx, hipMemcpyAsync(cnt, o.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
     KG_CK(ctx, hipStreamSynchronize(ctx->stream));
@@ -651,92 +911,59 @@ static int scan_core(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t rw,
     ctx->cand_cap = (int64_t)cnt[0] + (int64_t)cnt[0] / 4 + 1024;
     KG_CK(ctx, ctx->cand.ensure(sizeof(Cand) * ctx->cand_cap));
   }
-  const int64_t nc = (int64_t)cnt[0];
-  float ms_bulk = 0.f;
-  KG_CK(ctx, hipEventElapsedTime(&ms_bulk, ctx->ev[1], ctx->ev[2]));
-  ctx->timing.bulk_ms = ms_bulk;
+  float ms = 0.f;
+  KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+  ctx->timing.bulk_ms = ms;
   ctx->timing.refine_ms = 0.0;
+  KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]));
+  ctx->timing.total_ms = ms;
+  const int64_t nc = std::min<int64_t>((int64_t)cnt[0], ctx->cand_cap);
   ctx->timing.candidates = nc;
-  ctx->timing.kmers = m;
-  for (int tr = 0; tr < t; ++tr) n_passed[tr] += (int64_t)cnt[1 + tr];
-
+  ctx->timing.kmers = s.m;
+  for (int tr = 0; tr < t; ++tr) n_passed[sidx * t + tr] += (int64_t)cnt[1 + tr];
   std::vector<Cand> cands(nc);
-  if (nc) {
+  if (nc)
     KG_CK(ctx, hipMemcpy(cands.data(), ctx->cand.p, sizeof(Cand) * nc, hipMemcpyDeviceToHost));
-  }
-  if (ctx->precision == KG_PREC_MIXED) {
-    std::vector<double> res((size_t)4 * nc);
-    if (nc) {
-      KG_CK(ctx, ctx->res.ensure(sizeof(double) * 4 * nc));
-      KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-      KG_CK(ctx, launch_refine(ctx->stream, ctx->cand.as<Cand>(), nc, d_bits, rw, 0,
-                               ctx->mask.as<uint64_t>(), (md.n_acc + 63) / 64,
-                               ctx->B64.as<double>(), md.p + t, md.p, t, md.n, md.sigma,
-                               md.base_coef, ctx->bpfull.as<double>(), ctx->rbar.as<double>(),
-                               ctx->res.as<double>()));
-      KG_CK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-      KG_CK(ctx, hipMemcpyAsync(res.data(), ctx->res.p, sizeof(double) * 4 * nc,
-                                hipMemcpyDeviceToHost, ctx->stream));
-      KG_CK(ctx, hipStreamSynchronize(ctx->stream));
-      float ms_ref = 0.f;
-      KG_CK(ctx, hipEventElapsedTime(&ms_ref, ctx->ev[2], ctx->ev[3]));
-      ctx->timing.refine_ms = ms_ref;
-    }
-    for (int64_t i = 0; i < nc; ++i) {
-      const Cand& cd = cands[i];
-      const int c = (int)res[4 * i + 0];
-      Decided d = decide(ctx, idx0 + cd.idx, cd.trait, c, res[4 * i + 1], res[4 * i + 2],
-                         res[4 * i + 3], true, prm);
-      if ((cd.flags & kPfAmb) && d.pass) n_passed[cd.trait] += 1;
-      if (d.hit) hits.push_back(d.h);
-    }
-  } else {
-    for (int64_t i = 0; i < nc; ++i) {
-      const Cand& cd = cands[i];
-      Decided d = decide(ctx, idx0 + cd.idx, cd.trait, -1, (double)cd.xx32, (double)cd.xy32,
-                         0.0, false, prm);
-      if (d.hit) hits.push_back(d.h);
-    }
-    if (!hits.empty()) {  // allele counts of reported k-mers from the bits
-      // (the FP32 bulk pass keeps only xxM/xyM in its records)
-      std::vector<uint64_t> row(rw);
-      for (auto& h : hits) {
-        if (h.c >= 0) continue;
-        KG_CK(ctx, hipMemcpy(row.data(), d_bits + (h.idx - idx0) * rw, rw * 8,
-                             hipMemcpyDeviceToHost));
-        int c = 0;
-        for (size_t w = 0; w < md.maskw.size() && (int64_t)w < rw; ++w)
-          c += __builtin_popcountll(row[w] & md.maskw[w]);
-        h.c = c;
-      }
+  const size_t h0 = hits.size();
+  for (int64_t i = 0; i < nc; ++i) {
+    const Cand& cd = cands[i];
+    Decided d = decide(md, s.idx0 + cd.idx, cd.trait, -1, (double)cd.xx32, (double)cd.xy32, 0.0,
+                       false, prm);
+    if (d.hit) {
+      d.h.shard = sidx;
+      hits.push_back(d.h);
     }
   }
-  float ms_tot = 0.f;
-  KG_CK(ctx, hipEventElapsedTime(&ms_tot, ctx->ev[0],
-                                 (ctx->precision == KG_PREC_MIXED && nc) ? ctx->ev[3] : ctx->ev[2]));
-  ctx->timing.total_ms = ms_tot;
+  // allele counts of reported k-mers from their bits (the FP32 records carry
+  // only xxM/xyM)
+  std::vector<uint64_t> row(s.rw);
+  for (size_t i = h0; i < hits.size(); ++i) {
+    kg_hit& h = hits[i];
+    KG_CK(ctx, hipMemcpy(row.data(), s.bits + (h.idx - s.idx0) * s.rw, s.rw * 8,
+                         hipMemcpyDeviceToHost));
+    int c = 0;
+    for (size_t w = 0; w < md.maskw.size() && (int64_t)w < s.rw; ++w)
+      c += __builtin_popcountll(row[w] & md.maskw[w]);
+    h.c = c;
+  }
   return KG_OK;
 }
 
-static int check_scan_args(kg_ctx* ctx, const void* bits, int64_t m, int64_t rw,
-                           const kg_scan_params* prm) {
-  if (!ctx) return KG_ERR_ARG;
-  if (!ctx->have_model) return fail(ctx, KG_ERR_STATE, "scan before kg_set_model");
-  if (!prm) return fail(ctx, KG_ERR_ARG, "null scan params");
-  if (m < 0) return fail(ctx, KG_ERR_ARG, "negative k-mer count");
-  if (m > 0 && !bits) return fail(ctx, KG_ERR_ARG, "null bits");
-  if (rw % 8 != 0 || rw < 8 * (int64_t)ctx->md.nch_i8)
-    return fail(ctx, KG_ERR_ARG,
-                "row_words must be a multiple of 8 and cover all accessions (>= " +
-                    std::to_string(8 * ctx->md.nch_i8) + ")");
-  if (((uintptr_t)bits & 15) != 0) return fail(ctx, KG_ERR_ARG, "bits must be 16-byte aligned");
-  if (!(prm->log10_thresh >= 0.0)) return fail(ctx, KG_ERR_ARG, "log10_thresh must be >= 0");
+static int scan_dispatch(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_scan_params& prm,
+                         unsigned flags, std::vector<kg_hit>& hits, int64_t* n_passed,
+                         bool stats) {
+  if (ctx->precision == KG_PREC_MIXED) return scan_i8_shards(ctx, sh, prm, flags, hits, n_passed, stats);
+  for (size_t i = 0; i < sh.size(); ++i) {
+    int rc = scan_f32_shard(ctx, sh[i], (int)i, prm, hits, n_passed, stats);
+    if (rc) return rc;
+  }
   return KG_OK;
 }
 
 static int finish_hits(kg_ctx* ctx, std::vector<kg_hit>& hits, kg_hit* out, int64_t cap,
                        int64_t* n_out) {
   std::sort(hits.begin(), hits.end(), [](const kg_hit& a, const kg_hit& b) {
+    if (a.shard != b.shard) return a.shard < b.shard;
     return a.idx != b.idx ? a.idx < b.idx : a.trait < b.trait;
   });
   if (n_out) *n_out = (int64_t)hits.size();
@@ -750,49 +977,87 @@ static int finish_hits(kg_ctx* ctx, std::vector<kg_hit>& hits, kg_hit* out, int6
   return KG_OK;
 }
 
+static int check_common(kg_ctx* ctx, const kg_scan_params* prm) {
+  if (!ctx) return KG_ERR_ARG;
+  if (ctx->models.empty()) return fail(ctx, KG_ERR_STATE, "scan before kg_set_model");
+  if (!prm) return fail(ctx, KG_ERR_ARG, "null scan params");
+  if (!(prm->log10_thresh >= 0.0)) return fail(ctx, KG_ERR_ARG, "log10_thresh must be >= 0");
+  return KG_OK;
+}
+
 extern "C" int kg_scan_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t row_words,
                               const kg_scan_params* prm, kg_hit* out, int64_t cap,
                               int64_t* n_out, int64_t* n_passed) {
-  int rc = check_scan_args(ctx, d_bits, m, row_words, prm);
+  int rc = check_common(ctx, prm);
+  if (rc) return rc;
+  HShard s{d_bits, m, row_words, 0, 0};
+  rc = check_shard(ctx, s);
   if (rc) return rc;
   KG_CK(ctx, hipSetDevice(ctx->device));
+  const int t = ctx->models[0]->md.t;
   std::vector<kg_hit> hits;
-  std::vector<int64_t> np(ctx->md.t, 0);
+  std::vector<int64_t> np(t, 0);
   if (m > 0) {
-    rc = scan_core(ctx, d_bits, m, row_words, 0, *prm, hits, np.data(), false);
+    rc = scan_dispatch(ctx, {s}, *prm, 0, hits, np.data(), false);
     if (rc) return rc;
   }
   if (n_passed)
-    for (int tr = 0; tr < ctx->md.t; ++tr) n_passed[tr] = np[tr];
+    for (int tr = 0; tr < t; ++tr) n_passed[tr] = np[tr];
   return finish_hits(ctx, hits, out, cap, n_out);
 }
 
 extern "C" int kg_scan(kg_ctx* ctx, const uint64_t* bits, int64_t m, int64_t row_words,
                        const kg_scan_params* prm, kg_hit* out, int64_t cap, int64_t* n_out,
                        int64_t* n_passed) {
-  if (!ctx) return KG_ERR_ARG;
-  if (!ctx->have_model) return fail(ctx, KG_ERR_STATE, "scan before kg_set_model");
+  int rc = check_common(ctx, prm);
+  if (rc) return rc;
   if (m > 0 && !bits) return fail(ctx, KG_ERR_ARG, "null bits");
-  if (row_words % 8 != 0 || row_words < 8 * (int64_t)ctx->md.nch_i8)
+  const Model& md = ctx->models[0]->md;
+  if (row_words % 8 != 0 || row_words < 8 * (int64_t)md.nch_i8)
     return fail(ctx, KG_ERR_ARG, "row_words must be a multiple of 8 and cover all accessions");
-  if (!prm) return fail(ctx, KG_ERR_ARG, "null scan params");
   KG_CK(ctx, hipSetDevice(ctx->device));
+  const int t = md.t;
   // slices of at most 512 MiB of bits
   const int64_t slice = std::max<int64_t>(1, (int64_t)(512ll << 20) / (row_words * 8));
   const int64_t rows = std::min<int64_t>(slice, std::max<int64_t>(m, 1));
   KG_CK(ctx, ctx->stage.ensure((size_t)rows * row_words * 8));
   std::vector<kg_hit> hits;
-  std::vector<int64_t> np(ctx->md.t, 0);
-  for (int64_t s = 0; s < m; s += rows) {
-    const int64_t mm = std::min<int64_t>(rows, m - s);
-    KG_CK(ctx, hipMemcpyAsync(ctx->stage.p, bits + s * row_words, (size_t)mm * row_words * 8,
+  std::vector<int64_t> np(t, 0);
+  for (int64_t s0 = 0; s0 < m; s0 += rows) {
+    const int64_t mm = std::min<int64_t>(rows, m - s0);
+    KG_CK(ctx, hipMemcpyAsync(ctx->stage.p, bits + s0 * row_words, (size_t)mm * row_words * 8,
                               hipMemcpyHostToDevice, ctx->stream));
-    int rc = scan_core(ctx, ctx->stage.as<uint64_t>(), mm, row_words, s, *prm, hits, np.data(),
-                       false);
+    HShard s{ctx->stage.as<uint64_t>(), mm, row_words, s0, 0};
+    rc = scan_dispatch(ctx, {s}, *prm, 0, hits, np.data(), false);
     if (rc) return rc;
   }
   if (n_passed)
-    for (int tr = 0; tr < ctx->md.t; ++tr) n_passed[tr] = np[tr];
+    for (int tr = 0; tr < t; ++tr) n_passed[tr] = np[tr];
+  return finish_hits(ctx, hits, out, cap, n_out);
+}
+
+extern "C" int kg_scan_shards(kg_ctx* ctx, const kg_shard* shards, int32_t nshards,
+                              const kg_scan_params* prm, uint32_t flags, kg_hit* out,
+                              int64_t cap, int64_t* n_out, int64_t* n_passed) {
+  int rc = check_common(ctx, prm);
+  if (rc) return rc;
+  if (nshards <= 0 || !shards) return fail(ctx, KG_ERR_ARG, "no shards");
+  std::vector<HShard> sh(nshards);
+  for (int s = 0; s < nshards; ++s) {
+    sh[s] = HShard{shards[s].d_bits, shards[s].m, shards[s].row_words, 0, shards[s].model};
+    rc = check_shard(ctx, sh[s]);
+    if (rc) return rc;
+  }
+  KG_CK(ctx, hipSetDevice(ctx->device));
+  const int t = ctx->models[sh[0].model]->md.t;
+  std::vector<int64_t> np((size_t)nshards * t, 0);
+  std::vector<kg_hit> hits;
+  unsigned f = 0;
+  if (flags & KG_SHARDS_PER_LAUNCH) f |= kPerLaunch;
+  if (flags & KG_SHARDS_GRAPH) f |= kGraph;
+  rc = scan_dispatch(ctx, sh, *prm, f, hits, np.data(), false);
+  if (rc) return rc;
+  if (n_passed) memcpy(n_passed, np.data(), sizeof(int64_t) * np.size());
   return finish_hits(ctx, hits, out, cap, n_out);
 }
 
@@ -800,16 +1065,19 @@ extern "C" int kg_stats_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m,
                                int64_t row_words, int32_t* c, double* xxM, double* xyM,
                                double* S) {
   kg_scan_params prm{0.2, 6.0, 0.0};
-  int rc = check_scan_args(ctx, d_bits, m, row_words, &prm);
+  int rc = check_common(ctx, &prm);
+  if (rc) return rc;
+  HShard s{d_bits, m, row_words, 0, 0};
+  rc = check_shard(ctx, s);
   if (rc) return rc;
   if (m == 0) return KG_OK;
   if (!c || !xxM || !xyM || !S) return fail(ctx, KG_ERR_ARG, "null output array");
   KG_CK(ctx, hipSetDevice(ctx->device));
+  const int t = ctx->models[0]->md.t;
   std::vector<kg_hit> hits;
-  std::vector<int64_t> np(ctx->md.t, 0);
-  rc = scan_core(ctx, d_bits, m, row_words, 0, prm, hits, np.data(), true);
+  std::vector<int64_t> np(t, 0);
+  rc = scan_dispatch(ctx, {s}, prm, 0, hits, np.data(), true);
   if (rc) return rc;
-  const int t = ctx->md.t;
   KG_CK(ctx, hipMemcpy(c, ctx->st_c.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
   KG_CK(ctx, hipMemcpy(xxM, ctx->st_xx.p, sizeof(double) * m, hipMemcpyDeviceToHost));
   KG_CK(ctx, hipMemcpy(xyM, ctx->st_xy.p, sizeof(double) * m * t, hipMemcpyDeviceToHost));
@@ -830,14 +1098,14 @@ extern "C" int kg_synth_bits(kg_ctx* ctx, uint64_t* d_out, uint64_t seed, int64_
   if (m < 0 || n_acc <= 0 || row_words * 64 < n_acc || npop <= 0 || n_profiles <= 0 || !pop ||
       !thr || (m > 0 && !d_out))
     return fail(ctx, KG_ERR_ARG, "kg_synth_bits: bad arguments");
+  for (int i = 0; i < n_acc; ++i)
+    if (pop[i] >= npop) return fail(ctx, KG_ERR_ARG, "kg_synth_bits: population out of range");
   KG_CK(ctx, hipSetDevice(ctx->device));
   DBuf dpop, dthr;
   KG_CK(ctx, dpop.ensure(n_acc));
   KG_CK(ctx, dthr.ensure((size_t)4 * npop * n_profiles));
   KG_CK(ctx, hipMemcpy(dpop.p, pop, n_acc, hipMemcpyHostToDevice));
   KG_CK(ctx, hipMemcpy(dthr.p, thr, (size_t)4 * npop * n_profiles, hipMemcpyHostToDevice));
-  for (int i = 0; i < n_acc; ++i)
-    if (pop[i] >= npop) return fail(ctx, KG_ERR_ARG, "kg_synth_bits: population out of range");
   if (m > 0)
     KG_CK(ctx, launch_synth(ctx->stream, d_out, seed, kmer0, m, (uint32_t)n_acc, row_words,
                             dpop.as<uint8_t>(), dthr.as<uint32_t>(), (uint32_t)npop,

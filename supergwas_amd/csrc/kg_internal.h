@@ -10,11 +10,13 @@ constexpr int kMaxTraits = 8;
 // Bulk-kernel candidate: a k-mer/trait whose pre-filter or 1D significance is
 // not decided "no" by the bulk pass.
 struct Cand {
-  int64_t idx;     // k-mer row (global within the scan)
+  int64_t idx;     // k-mer row within its shard
   int32_t trait;
-  uint32_t flags;  // kCand* bits below
+  uint32_t flags;  // kPf* / kSig* bits below
+  int32_t shard;
   float xx32;      // plain FP32 path: the reference-arithmetic xxM, xyM
   float xy32;
+  int32_t pad;
 };
 constexpr uint32_t kPfPass = 1u;  // pre-filter decided pass
 constexpr uint32_t kPfAmb = 2u;   // pre-filter within the error bound of tau
@@ -24,7 +26,7 @@ constexpr uint32_t kSigAmb = 8u;  // 1D threshold within the error bound
 // Per-scan outputs of the bulk kernels.
 struct ScanOut {
   Cand* cand;                    // candidate records (unordered)
-  unsigned long long* counters;  // [0] candidates, [1 + tr] decided pre-filter passes
+  unsigned long long* counters;  // [0] candidates; FP32 path: [1 + tr] pre-filter passes
   int64_t cap;                   // capacity of cand
   // full-statistic mode (null pointers otherwise)
   int32_t* st_c;
@@ -55,6 +57,32 @@ struct EpiI8 {
   const double* rbar;   // [t] mean of r = y - U U'y
   const double* ycn;    // [t] |y - mean y|
   const double* yyM;    // [t]
+};
+
+// One shard of a batched int8 scan: its rows and its null model.  The
+// persistent kernel walks the concatenation of all shards' 256-k-mer blocks.
+struct ShardDesc {
+  const uint64_t* bits;          // rows [m][rw]
+  int64_t m, rw;
+  int64_t blk0;                  // first global block of this shard
+  const void* wts;               // model weights (int8 limbs, chunk-major)
+  const void* mask;              // accession mask words
+  int nchunks;                   // 512-sample chunks of the model
+  int pad;
+  unsigned long long* pass;      // [t] decided pre-filter passes of this shard
+  EpiI8 ep;
+};
+
+// FP64 refinement inputs of one shard.
+struct RefDesc {
+  const uint64_t* bits;
+  int64_t rw;
+  const uint64_t* maskw;
+  const double* B64;             // [n_acc][p + t]
+  const double* bpfull;          // [t][p]
+  const double* rbar;            // [t]
+  double sigma, base_coef;
+  int nwords, p, t, n;
 };
 
 // Epilogue constants of the plain FP32 kernel.

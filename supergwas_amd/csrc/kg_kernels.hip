@@ -107,14 +107,15 @@ __device__ uint32_t classify_i8(int c, double xxM, double bxx, double S, double 
 }
 
 // ----------------------------------------------------------------------------
-// Bulk int8-limb scan.  Workgroup = 4 waves; wave w of block `blk` owns
-// k-mers [ (blk*4 + w) * 16R, +16R ) as R tiles of 16.  Persistent: blocks
-// are strided over the grid.
+// Bulk int8-limb scan over a table of shards.  Workgroup = 4 waves; global
+// block `w` belongs to the shard s with sd[s].blk0 <= w < sd[s+1].blk0, and
+// wave v of that block owns the shard's k-mers [(b*4 + v) * 16R, +16R)
+// (b = w - blk0) as R tiles of 16.  Persistent: blocks are strided over the
+// grid, so one launch covers every shard.
 template <int NT, int R>
-__global__ __launch_bounds__(kThreads, 2) void scan_i8(
-    const uint64_t* __restrict__ bits, int64_t m, int64_t row_words, int64_t row0,
-    int nchunks, const v4i* __restrict__ wts, const uint4* __restrict__ maskv, EpiI8 ep,
-    ScanOut out, int64_t nblocks) {
+__global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
+                                                        int nshards, int64_t total_blocks,
+                                                        ScanOut out) {
   constexpr int NU = 16 * NT;
   constexpr int FRAG = kI8Steps * NT * 64;  // v4i per chunk of weights
   extern __shared__ v4i lds[];
@@ -123,9 +124,20 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(
   const int g = lane >> 4, col = lane & 15;
   const int kk = lane >> 2, q = lane & 3;
   if (threadIdx.x < kMaxTraits) wg_pass[threadIdx.x] = 0;
+  int s = 0;
 
-  for (int64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
-    const int64_t kbase = (blk * kWaves + wave) * (16 * R);
+  for (int64_t w = blockIdx.x; w < total_blocks; w += gridDim.x) {
+    // shard of block w (blocks are visited in increasing order per workgroup)
+    while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
+    const ShardDesc& D = sd[s];
+    const EpiI8& ep = D.ep;
+    const uint64_t* __restrict__ bits = D.bits;
+    const int64_t m = D.m, row_words = D.rw;
+    const v4i* __restrict__ wts = reinterpret_cast<const v4i*>(D.wts);
+    const uint4* __restrict__ maskv = reinterpret_cast<const uint4*>(D.mask);
+    const int nchunks = D.nchunks;
+    const int64_t kbase = ((w - D.blk0) * kWaves + wave) * (16 * R);
+
     v4i acc[R][NT];
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -247,13 +259,12 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(
         uint32_t fl = 0;
         if (q == 0 && valid) fl = classify_i8(c, xxM, bxx, S, bS, xyM, bxy, ep, tr, &pf_def);
         if (out.st_c != nullptr && q == 0 && valid) {
-          const int64_t o = kidx - row0;
           if (tr == 0) {
-            out.st_c[o] = c;
-            out.st_xx[o] = xxM;
+            out.st_c[kidx] = c;
+            out.st_xx[kidx] = xxM;
           }
-          out.st_xy[o * ep.t + tr] = xyM;
-          out.st_S[o * ep.t + tr] = S;
+          out.st_xy[kidx * ep.t + tr] = xyM;
+          out.st_S[kidx * ep.t + tr] = S;
         }
         const unsigned long long pb = __ballot(pf_def);
         if (lane == 0 && pb) atomicAdd(&wg_pass[tr], (unsigned int)__popcll(pb));
@@ -261,16 +272,21 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(
         rec.idx = kidx;
         rec.trait = tr;
         rec.flags = fl;
+        rec.shard = s;
         rec.xx32 = 0.f;
         rec.xy32 = 0.f;
+        rec.pad = 0;
         append_cand(fl != 0, rec, out);
       }
       __syncthreads();
     }
+    // flush this block's pre-filter passes to its shard
+    if (threadIdx.x < ep.t) {
+      const unsigned int v = wg_pass[threadIdx.x];
+      if (v) atomicAdd(&D.pass[threadIdx.x], (unsigned long long)v);
+      wg_pass[threadIdx.x] = 0;
+    }
   }
-  __syncthreads();
-  if (threadIdx.x < ep.t && wg_pass[threadIdx.x])
-    atomicAdd(&out.counters[1 + threadIdx.x], (unsigned long long)wg_pass[threadIdx.x]);
 }
 
 // ----------------------------------------------------------------------------
@@ -408,6 +424,8 @@ __global__ __launch_bounds__(kThreads, 2) void scan_f32(
         rec.idx = kidx;
         rec.trait = tr;
         rec.flags = kPfPass | kSigAmb;
+        rec.shard = 0;
+        rec.pad = 0;
         rec.xx32 = xxM;
         rec.xy32 = xyM;
         append_cand(cand, rec, out);
@@ -423,30 +441,33 @@ __global__ __launch_bounds__(kThreads, 2) void scan_f32(
 // ----------------------------------------------------------------------------
 // Exact FP64 statistic of each candidate.  One wave per candidate; lane b
 // accumulates base column b (and b + 64) over the set, phenotyped bits in
-// accession order.  B64 is [n_acc][nbf] (zero rows for accessions without a
-// phenotype).  Columns: [0,p) rotated centred U, [p, p+t) centred residuals.
-// res[i] = {c, xxM, xyM, S}.
-__global__ __launch_bounds__(kThreads) void refine_f64(
-    const Cand* __restrict__ cand, int64_t ncand, const uint64_t* __restrict__ bits,
-    int64_t row_words, int64_t row0, const uint64_t* __restrict__ maskw, int nwords,
-    const double* __restrict__ B64, int nbf, int p, int t, int n, double sigma,
-    double base_coef, const double* __restrict__ bpfull, const double* __restrict__ rbar,
-    double* __restrict__ res) {
+// accession order.  The candidate count is read on the device (so the launch
+// can sit in a captured graph).  Columns of RefDesc::B64: [0,p) rotated
+// centred U, [p, p+t) centred residuals.  res[i] = {c, xxM, xyM, S}.
+__global__ __launch_bounds__(kThreads) void refine_f64(const Cand* __restrict__ cand,
+                                                       const unsigned long long* __restrict__ count,
+                                                       int64_t cap,
+                                                       const RefDesc* __restrict__ rd,
+                                                       double* __restrict__ res) {
   const int lane = threadIdx.x & 63;
   const int64_t wid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
   const int64_t nw = (int64_t)gridDim.x * kWaves;
+  int64_t ncand = (int64_t)count[0];
+  if (ncand > cap) ncand = cap;
   for (int64_t i = wid; i < ncand; i += nw) {
     const Cand cd = cand[i];
-    const uint64_t* row = bits + (cd.idx - row0) * row_words;
+    const RefDesc& R = rd[cd.shard];
+    const uint64_t* row = R.bits + cd.idx * R.rw;
+    const int nbf = R.p + R.t, p = R.p;
     double a0 = 0.0, a1 = 0.0;
     int c = 0;
-    for (int w = 0; w < nwords; ++w) {
-      uint64_t word = row[w] & maskw[w];
+    for (int w = 0; w < R.nwords; ++w) {
+      uint64_t word = row[w] & R.maskw[w];
       c += __popcll(word);
       while (word) {
         const int bit = __ffsll((long long)word) - 1;
         word &= word - 1;
-        const double* Br = B64 + (size_t)(64 * w + bit) * nbf;
+        const double* Br = R.B64 + (size_t)(64 * w + bit) * nbf;
         if (lane < nbf) a0 += Br[lane];
         if (lane + 64 < nbf) a1 += Br[lane + 64];
       }
@@ -455,12 +476,12 @@ __global__ __launch_bounds__(kThreads) void refine_f64(
     double zz = 0.0, zb = 0.0, z0 = 0.0, sv = 0.0;
     if (lane < p) {
       zz += a0 * a0;
-      zb += a0 * bpfull[tr * p + lane];
+      zb += a0 * R.bpfull[tr * p + lane];
       if (lane == 0) z0 = a0;
     }
     if (lane + 64 < p) {
       zz += a1 * a1;
-      zb += a1 * bpfull[tr * p + lane + 64];
+      zb += a1 * R.bpfull[tr * p + lane + 64];
     }
     if (lane == p + tr) sv = a0;
     if (lane + 64 == p + tr) sv = a1;
@@ -471,11 +492,11 @@ __global__ __launch_bounds__(kThreads) void refine_f64(
       sv += __shfl_xor(sv, off);
     }
     if (lane == 0) {
-      const double cdv = (double)c, cn = cdv / (double)n;
-      const double xxM = cdv - cdv * cdv * base_coef - 2.0 * cn * sigma * z0 - zz;
+      const double cdv = (double)c, cn = cdv / (double)R.n;
+      const double xxM = cdv - cdv * cdv * R.base_coef - 2.0 * cn * R.sigma * z0 - zz;
       res[4 * i + 0] = cdv;
       res[4 * i + 1] = xxM;
-      res[4 * i + 2] = sv + cdv * rbar[tr];
+      res[4 * i + 2] = sv + cdv * R.rbar[tr];
       res[4 * i + 3] = sv + zb;
     }
   }
@@ -503,39 +524,27 @@ __global__ void synth_bits(uint64_t* __restrict__ out, uint64_t seed, int64_t km
 // ----------------------------------------------------------------------------
 // Launchers (host).  The tile count NT is a template parameter.
 
-template <int NT>
-static hipError_t launch_i8_nt(int R, dim3 grid, size_t shm, hipStream_t s, const uint64_t* bits,
-                               int64_t m, int64_t rw, int64_t row0, int nch, const void* wts,
-                               const void* mask, const EpiI8& ep, const ScanOut& o,
-                               int64_t nblocks) {
-  if (R == 4) {
-    auto kfn = scan_i8<NT, 4>;
-    (void)hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    hipLaunchKernelGGL(kfn, grid, dim3(kThreads), shm, s, bits, m, rw, row0, nch,
-                       (const v4i*)wts, (const uint4*)mask, ep, o, nblocks);
-  } else if (R == 2) {
-    auto kfn = scan_i8<NT, 2>;
-    (void)hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    hipLaunchKernelGGL(kfn, grid, dim3(kThreads), shm, s, bits, m, rw, row0, nch,
-                       (const v4i*)wts, (const uint4*)mask, ep, o, nblocks);
-  } else {
-    return hipErrorInvalidValue;
-  }
+template <int NT, int R>
+static hipError_t launch_i8_tr(dim3 grid, size_t shm, hipStream_t s, const ShardDesc* sd,
+                               int nshards, int64_t total_blocks, const ScanOut& o) {
+  auto kfn = scan_i8<NT, R>;
+  hipLaunchKernelGGL(kfn, grid, dim3(kThreads), shm, s, sd, nshards, total_blocks, o);
   return hipGetLastError();
 }
 
-hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const uint64_t* bits,
-                          int64_t m, int64_t rw, int64_t row0, const void* wts, const void* mask,
-                          const EpiI8& ep, const ScanOut& o) {
-  const int nch = (int)(rw / 8);
-  const int64_t per_block = (int64_t)kWaves * 16 * R;
-  const int64_t nblocks = (m + per_block - 1) / per_block;
+// k-mers per workgroup block for tiles-per-wave R
+int64_t i8_block_kmers(int R) { return (int64_t)kWaves * 16 * R; }
+
+hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const ShardDesc* sd,
+                          int nshards, int64_t total_blocks, const ScanOut& o) {
   const size_t shm = (size_t)kI8Steps * NT * 64 * 16;
-  dim3 g((unsigned)(nblocks < grid ? nblocks : grid));
-  switch (NT) {
-#define KG_CASE(X) \
-  case X:          \
-    return launch_i8_nt<X>(R, g, shm, s, bits, m, rw, row0, nch, wts, mask, ep, o, nblocks);
+  dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
+  switch (NT * 8 + R) {
+#define KG_CASE(X)                                                                  \
+  case X * 8 + 4:                                                                   \
+    return launch_i8_tr<X, 4>(g, shm, s, sd, nshards, total_blocks, o);             \
+  case X * 8 + 2:                                                                   \
+    return launch_i8_tr<X, 2>(g, shm, s, sd, nshards, total_blocks, o);
     KG_CASE(1) KG_CASE(2) KG_CASE(3) KG_CASE(4) KG_CASE(5) KG_CASE(6) KG_CASE(7) KG_CASE(8)
 #undef KG_CASE
     default:
@@ -549,7 +558,6 @@ static hipError_t launch_f32_nt(dim3 grid, size_t shm, hipStream_t s, const uint
                                 const uint64_t* mask, const EpiF32& ep, const ScanOut& o,
                                 int64_t nblocks) {
   auto kfn = scan_f32<NT, 4>;
-  (void)hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
   hipLaunchKernelGGL(kfn, grid, dim3(kThreads), shm, s, bits, m, rw, row0, nch, wts, mask, ep, o,
                      nblocks);
   return hipGetLastError();
@@ -574,18 +582,36 @@ hipError_t launch_scan_f32(int NT, int grid, hipStream_t s, const uint64_t* bits
   }
 }
 
-hipError_t launch_refine(hipStream_t s, const Cand* cand, int64_t ncand, const uint64_t* bits,
-                         int64_t rw, int64_t row0, const uint64_t* maskw, int nwords,
-                         const double* B64, int nbf, int p, int t, int n, double sigma,
-                         double base_coef, const double* bpfull, const double* rbar,
-                         double* res) {
-  if (ncand <= 0) return hipSuccess;
-  int64_t blocks = (ncand + kWaves - 1) / kWaves;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(refine_f64, dim3((unsigned)blocks), dim3(kThreads), 0, s, cand, ncand, bits,
-                     rw, row0, maskw, nwords, B64, nbf, p, t, n, sigma, base_coef, bpfull, rbar,
-                     res);
+hipError_t launch_refine(hipStream_t s, const Cand* cand, const unsigned long long* count,
+                         int64_t cap, const RefDesc* rd, double* res, int blocks) {
+  hipLaunchKernelGGL(refine_f64, dim3((unsigned)blocks), dim3(kThreads), 0, s, cand, count, cap,
+                     rd, res);
   return hipGetLastError();
+}
+
+// Dynamic-LDS limits of every bulk-kernel instantiation, set once per process
+// (outside any stream capture).
+template <int NT>
+static hipError_t set_attr_nt() {
+  hipError_t e = hipFuncSetAttribute((const void*)scan_i8<NT, 4>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     kI8Steps * NT * 64 * 16);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)scan_i8<NT, 2>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kI8Steps * NT * 64 * 16);
+  if (e == hipSuccess && NT <= 6)
+    e = hipFuncSetAttribute((const void*)scan_f32<(NT <= 6 ? NT : 6), 4>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 64 * NT * 64 * 4);
+  return e;
+}
+
+hipError_t set_kernel_attributes() {
+  hipError_t e = hipSuccess;
+#define KG_ATTR(X) \
+  if (e == hipSuccess) e = set_attr_nt<X>();
+  KG_ATTR(1) KG_ATTR(2) KG_ATTR(3) KG_ATTR(4) KG_ATTR(5) KG_ATTR(6) KG_ATTR(7) KG_ATTR(8)
+#undef KG_ATTR
+  return e;
 }
 
 hipError_t launch_zero(hipStream_t s, unsigned long long* p, int n) {

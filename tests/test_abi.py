@@ -20,7 +20,8 @@ def declared_symbols():
 def test_header_declares_entry_points():
     syms = declared_symbols()
     for s in ("kg_create", "kg_destroy", "kg_last_error", "kg_set_model", "kg_scan",
-              "kg_scan_device", "kg_stats_device", "kg_last_timing", "kg_synth_bits"):
+              "kg_scan_device", "kg_stats_device", "kg_last_timing", "kg_synth_bits",
+              "kg_add_model", "kg_scan_shards"):
         assert s in syms
 
 

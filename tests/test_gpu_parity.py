@@ -196,3 +196,67 @@ def test_multi_trait_equals_single_trait_runs(built):
         assert npass[tr] == ref["n_passed"]
         assert np.array_equal(got["idx"], exp_idx)
         assert np.allclose(got["T"], ref["T"][exp_idx], rtol=1e-9, atol=0)
+
+
+def _shard_setup(k, nsh=5, seed=30):
+    """Shards of uneven size, each with its own phenotyped-sample mask (own model)."""
+    import torch
+    sizes = [1, 300, 0, 5003, 2500][:nsh]
+    rng = np.random.default_rng(seed)
+    base = synth.make_dataset(seed=seed, n_acc=640, n_missing=0, npop=4, p=8, m=10000,
+                              n_causal=10, effect_sd=0.5)
+    rw = synth.row_words_for(base.n_acc)
+    shards, refs, devs = [], [], []
+    k0 = 0
+    for s, m in enumerate(sizes):
+        miss = rng.choice(base.n_acc, size=int(rng.uniform(0, 0.15) * base.n_acc), replace=False)
+        mask = np.ones(base.n_acc, bool)
+        mask[miss] = False
+        idxs = np.nonzero(mask)[0].astype(np.int32)
+        # per-shard U: orthonormal basis of [1, PCs restricted to the shard's samples]
+        Upcs = base.U[np.isin(base.idxs, idxs)]
+        U, _, _ = np.linalg.svd(Upcs, full_matrices=False)
+        y = rng.standard_normal(len(idxs)) + 0.2
+        mid = k.add_model(base.n_acc, idxs, U, y) if s else None
+        if s == 0:
+            k.set_model(base.n_acc, idxs, U, y)
+            mid = 0
+        bits = base.fill_bits(k0, max(m, 1))[:m]
+        k0 += m
+        dev = torch.from_numpy(bits.view(np.int64).copy()).to("cuda") if m else \
+            torch.zeros((1, rw), dtype=torch.int64, device="cuda")
+        devs.append(dev)
+        shards.append((dev.data_ptr(), m, rw, mid))
+        refs.append((bits, idxs, y, U))
+    torch.cuda.synchronize()
+    return shards, refs, devs
+
+
+def test_batched_shards_match_oracle_and_modes(built):
+    from supergwas_amd.api import SHARDS_GRAPH, SHARDS_PER_LAUNCH, KgwasHip, ScanParams
+    prm = ScanParams(0.1, 1.5, 3.0)
+    with KgwasHip() as k:
+        shards, refs, devs = _shard_setup(k)
+        results = {}
+        for name, flags in (("one", 0), ("per", SHARDS_PER_LAUNCH), ("graph", SHARDS_GRAPH),
+                            ("graph_per", SHARDS_GRAPH | SHARDS_PER_LAUNCH)):
+            reps = 3 if flags & SHARDS_GRAPH else 1
+            for r in range(reps):   # graph replays, each checked
+                hits, npass = k.scan_shards(shards, prm, flags)
+                results[(name, r)] = (hits.copy(), npass.copy())
+                print(f"[shards] mode={name} replay={r} hits={len(hits)}", flush=True)
+    h0, n0 = results[("one", 0)]
+    for key, (h, n) in results.items():
+        assert np.array_equal(n, n0), key
+        assert np.array_equal(h["idx"], h0["idx"]) and np.array_equal(h["shard"], h0["shard"]), key
+        assert np.array_equal(h["T"], h0["T"]), key
+    for s, (bits, idxs, y, U) in enumerate(refs):
+        if len(bits) == 0:
+            assert n0[s, 0] == 0 and not np.any(h0["shard"] == s)
+            continue
+        ref = oracle.scan_f64(bits, idxs, y, U, 0.1)
+        exp, _ = expected_hits(ref, 1.5, 3.0)
+        got = h0[h0["shard"] == s]
+        assert n0[s, 0] == ref["n_passed"], s
+        assert np.array_equal(got["idx"], exp), s
+        assert np.allclose(got["T"], ref["T"][exp], rtol=1e-9, atol=0)
