@@ -143,6 +143,13 @@ int kg_stats_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t row_
 
 int kg_last_timing(const kg_ctx* ctx, kg_timing* out);
 
+/* Device memory on the context's device for callers without their own
+ * allocator (tests, benchmarks, ctypes hosts).  kg_memcpy direction:
+ * 0 = host to device, 1 = device to host, 2 = device to device. */
+int kg_dev_alloc(kg_ctx* ctx, size_t bytes, void** d_ptr);
+int kg_dev_free(kg_ctx* ctx, void* d_ptr);
+int kg_memcpy(kg_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t direction);
+
 /* Seeded synthetic bits on the device (benchmarks and tests; no reference
  * counterpart): rows kmer0 .. kmer0+m-1 of the model in synth_core.h. */
 int kg_synth_bits(kg_ctx* ctx, uint64_t* d_out, uint64_t seed, int64_t kmer0, int64_t m,

@@ -46,6 +46,9 @@ def lib():
                                   _dp, _dp, ctypes.c_int, ctypes.c_double,
                                   _i32p, _u8p, _dp, _dp, _dp, _dp, _dp]
         L.or_scan_f64.restype = ctypes.c_int64
+        L.or_unpack_rows.argtypes = [_u64p, ctypes.c_int64, ctypes.c_int64, _i32p, ctypes.c_int,
+                                     _fp]
+        L.or_unpack_rows.restype = None
         L.cpu_cols_fp32.argtypes = [_fp, ctypes.c_int, ctypes.c_int64, _dp, _dp, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_int, _i64p, _i64p, _dp, _fp, _fp, _i64p, _i64p]
@@ -107,9 +110,11 @@ def unpack_rows(bits, idxs):
     """Row-major float32 matrix X[n, m] (phenotyped samples x k-mers) from packed bits,
     the layout src/kgwas_utils.hpp:225-282 builds."""
     bits = np.ascontiguousarray(bits, dtype=np.uint64)
-    idxs = np.asarray(idxs, dtype=np.int64)
-    words = bits[:, idxs >> 6]
-    return np.ascontiguousarray(((words >> (idxs & 63).astype(np.uint64)) & 1).T.astype(np.float32))
+    m, rw = bits.shape
+    idxs = np.ascontiguousarray(idxs, dtype=np.int32)
+    X = np.empty((len(idxs), m), np.float32)
+    lib().or_unpack_rows(_ptr(bits, _u64p), m, rw, _ptr(idxs, _i32p), len(idxs), _ptr(X, _fp))
+    return X
 
 
 def cols_fp32(X, y, U, corr_thr, log10_thresh, log10_thresh_2d, threads):

@@ -171,3 +171,17 @@ double or_T_threshold(double p_thresh) {
   }
   return hi;
 }
+
+// Row-major float matrix X[n][m] (phenotyped samples x k-mers) from packed
+// rows: the layout load_matrix_rows builds (src/kgwas_utils.hpp:225-282).
+void or_unpack_rows(const uint64_t* bits, int64_t m, int64_t row_words, const int32_t* idxs,
+                    int n, float* X) {
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; ++j) {
+    const int a = idxs[j];
+    const uint64_t* col = bits + (a >> 6);
+    float* xr = X + (size_t)j * (size_t)m;
+    for (int64_t i = 0; i < m; ++i)
+      xr[i] = (float)((col[(size_t)i * (size_t)row_words] >> (a & 63)) & 1ULL);
+  }
+}

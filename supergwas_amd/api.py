@@ -84,6 +84,9 @@ SIGNATURES = {
                         ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "kg_stats_device": ([_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp], ctypes.c_int),
     "kg_last_timing": ([_vp, ctypes.POINTER(_Timing)], ctypes.c_int),
+    "kg_dev_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
+    "kg_dev_free": ([_vp, _vp], ctypes.c_int),
+    "kg_memcpy": ([_vp, _vp, _vp, ctypes.c_size_t, _i32], ctypes.c_int),
     "kg_synth_bits": ([_vp, _vp, ctypes.c_uint64, _i64, _i64, _i32, _i64, _vp, _vp, _i32, _i32],
                       ctypes.c_int),
 }
@@ -107,6 +110,35 @@ def lib():
 
 def _p(a):
     return a.ctypes.data if a is not None else None
+
+
+class DeviceArray:
+    """Device allocation owned by a KgwasHip context (freed on close/garbage)."""
+
+    def __init__(self, ctx, ptr, nbytes):
+        self._ctx = ctx
+        self.ptr = ptr
+        self.nbytes = nbytes
+        self.shape = None
+        self.dtype = None
+
+    def to_host(self, dtype=None, shape=None, nbytes=None, offset=0):
+        dtype = np.dtype(dtype or self.dtype)
+        nb = self.nbytes - offset if nbytes is None else nbytes
+        out = np.empty(nb // dtype.itemsize, dtype)
+        self._ctx._check(lib().kg_memcpy(self._ctx._h, _p(out), self.ptr + offset, out.nbytes, 1))
+        return out.reshape(shape) if shape is not None else out
+
+    def free(self):
+        if self.ptr and self._ctx._h and self._ctx._h.value:
+            lib().kg_dev_free(self._ctx._h, self.ptr)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class KgwasHip:
@@ -234,6 +266,19 @@ class KgwasHip:
         self._check(lib().kg_synth_bits(self._h, dev_ptr, ds.seed, int(kmer0), int(m),
                                         ds.n_acc, int(row_words), _p(pop), _p(thr), ds.npop,
                                         ds.n_profiles))
+
+    # ---- device memory (no PyTorch needed) ----
+    def alloc(self, nbytes):
+        p = _vp()
+        self._check(lib().kg_dev_alloc(self._h, int(nbytes), ctypes.byref(p)))
+        return DeviceArray(self, p.value, int(nbytes))
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        d = self.alloc(arr.nbytes)
+        self._check(lib().kg_memcpy(self._h, d.ptr, _p(arr), arr.nbytes, 0))
+        d.shape, d.dtype = arr.shape, arr.dtype
+        return d
 
     def timing(self):
         t = _Timing()

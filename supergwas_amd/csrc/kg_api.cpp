@@ -1113,3 +1113,37 @@ extern "C" int kg_synth_bits(kg_ctx* ctx, uint64_t* d_out, uint64_t seed, int64_
   KG_CK(ctx, hipStreamSynchronize(ctx->stream));
   return KG_OK;
 }
+
+extern "C" int kg_dev_alloc(kg_ctx* ctx, size_t bytes, void** d_ptr) {
+  if (!ctx || !ctx->stream || !d_ptr) return KG_ERR_ARG;
+  KG_CK(ctx, hipSetDevice(ctx->device));
+  KG_CK(ctx, hipMalloc(d_ptr, bytes ? bytes : 16));
+  return KG_OK;
+}
+
+extern "C" int kg_dev_free(kg_ctx* ctx, void* d_ptr) {
+  if (!ctx || !ctx->stream) return KG_ERR_ARG;
+  if (!d_ptr) return KG_OK;
+  KG_CK(ctx, hipSetDevice(ctx->device));
+  KG_CK(ctx, hipStreamSynchronize(ctx->stream));
+  KG_CK(ctx, hipFree(d_ptr));
+  return KG_OK;
+}
+
+extern "C" int kg_memcpy(kg_ctx* ctx, void* dst, const void* src, size_t bytes,
+                         int32_t direction) {
+  if (!ctx || !ctx->stream) return KG_ERR_ARG;
+  if (bytes == 0) return KG_OK;
+  if (!dst || !src) return fail(ctx, KG_ERR_ARG, "kg_memcpy: null pointer");
+  hipMemcpyKind k;
+  switch (direction) {
+    case 0: k = hipMemcpyHostToDevice; break;
+    case 1: k = hipMemcpyDeviceToHost; break;
+    case 2: k = hipMemcpyDeviceToDevice; break;
+    default: return fail(ctx, KG_ERR_ARG, "kg_memcpy: direction must be 0, 1 or 2");
+  }
+  KG_CK(ctx, hipSetDevice(ctx->device));
+  KG_CK(ctx, hipStreamSynchronize(ctx->stream));
+  KG_CK(ctx, hipMemcpy(dst, src, bytes, k));
+  return KG_OK;
+}

@@ -112,31 +112,83 @@ __device__ uint32_t classify_i8(int c, double xxM, double bxx, double S, double 
 // wave v of that block owns the shard's k-mers [(b*4 + v) * 16R, +16R)
 // (b = w - blk0) as R tiles of 16.  Persistent: blocks are strided over the
 // grid, so one launch covers every shard.
+//
+// Pipeline per block: weights stream through two LDS stage buffers of 256
+// samples (4 MFMA steps) each, filled by direct global->LDS loads for stage
+// S+1 while stage S computes; each lane's bits for the next 512-sample chunk
+// are prefetched into registers one chunk ahead; the next block's first
+// stage and first chunk are prefetched before the epilogue runs.
+constexpr int kStageSteps = 4;
+
+template <int NT>
+__device__ __forceinline__ void dma_stage(const v4i* __restrict__ src, v4i* dst, int wave,
+                                          int lane) {
+#pragma unroll
+  for (int i = 0; i < NT; ++i) {
+    const int qd = i * kWaves + wave;  // 64-v4i block of the stage handled by this wave
+    __builtin_amdgcn_global_load_lds((const void*)(src + qd * 64 + lane),
+                                     (__attribute__((address_space(3))) void*)(dst + qd * 64),
+                                     16, 0, 0);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void load_chunk(const uint64_t* const (&rowp)[R], int C, uint4 (&raw)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) raw[r] = *reinterpret_cast<const uint4*>(rowp[r] + 8 * C);
+}
+
+template <int R>
+__device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wave, int g, int col,
+                                           const uint64_t* (&rowp)[R]) {
+  const int64_t kbase = ((w - D.blk0) * kWaves + wave) * (16 * R);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int64_t k = kbase + 16 * r + col;
+    if (k > D.m - 1) k = D.m - 1;  // clamp: rows past m are read, never reported
+    rowp[r] = D.bits + k * D.rw + 2 * g;
+  }
+}
+
 template <int NT, int R>
-__global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
+__global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8(const ShardDesc* __restrict__ sd,
                                                         int nshards, int64_t total_blocks,
                                                         ScanOut out) {
   constexpr int NU = 16 * NT;
-  constexpr int FRAG = kI8Steps * NT * 64;  // v4i per chunk of weights
+  constexpr int NUP = NU + 4;                        // padded epilogue row (bank spread)
+  constexpr int FRAG2 = kStageSteps * NT * 64;       // v4i per 256-sample stage
   extern __shared__ v4i lds[];
+  v4i* const buf0 = lds;
+  v4i* const buf1 = lds + FRAG2;
+  int* const sc_all = reinterpret_cast<int*>(lds + 2 * FRAG2);
   __shared__ unsigned int wg_pass[kMaxTraits];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, col = lane & 15;
   const int kk = lane >> 2, q = lane & 3;
   if (threadIdx.x < kMaxTraits) wg_pass[threadIdx.x] = 0;
-  int s = 0;
 
-  for (int64_t w = blockIdx.x; w < total_blocks; w += gridDim.x) {
-    // shard of block w (blocks are visited in increasing order per workgroup)
-    while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
+  int64_t w = blockIdx.x;
+  if (w >= total_blocks) return;
+  int s = 0;
+  while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
+  uint4 rawn[R];
+  {
+    const uint64_t* rp[R];
+    block_rows<R>(sd[s], w, wave, g, col, rp);
+    load_chunk<R>(rp, 0, rawn);
+    dma_stage<NT>(reinterpret_cast<const v4i*>(sd[s].wts), buf0, wave, lane);
+  }
+
+  while (w < total_blocks) {
     const ShardDesc& D = sd[s];
     const EpiI8& ep = D.ep;
-    const uint64_t* __restrict__ bits = D.bits;
-    const int64_t m = D.m, row_words = D.rw;
     const v4i* __restrict__ wts = reinterpret_cast<const v4i*>(D.wts);
     const uint4* __restrict__ maskv = reinterpret_cast<const uint4*>(D.mask);
     const int nchunks = D.nchunks;
+    const int NS = 2 * nchunks;
     const int64_t kbase = ((w - D.blk0) * kWaves + wave) * (16 * R);
+    const uint64_t* rowp[R];
+    block_rows<R>(D, w, wave, g, col, rowp);
 
     v4i acc[R][NT];
 #pragma unroll
@@ -144,42 +196,41 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[r][t] = v4i{0, 0, 0, 0};
     uint32_t cnt[R];
-    const uint64_t* rowp[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      cnt[r] = 0;
-      int64_t k = kbase + 16 * r + col;
-      if (k > m - 1) k = m - 1;                 // clamp: rows past m are read, never reported
-      rowp[r] = bits + k * row_words + 2 * g;
-    }
+    for (int r = 0; r < R; ++r) cnt[r] = 0;
+    uint4 raw[R];
 
+    __builtin_amdgcn_s_waitcnt(0);  // stage 0 weights and chunk 0 bits have landed
+    __syncthreads();
     for (int C = 0; C < nchunks; ++C) {
-      uint4 raw[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) raw[r] = *reinterpret_cast<const uint4*>(rowp[r] + 8 * C);
-      const uint4 mk = maskv[4 * C + g];
-      __syncthreads();                          // previous chunk fully consumed
-      const v4i* src = wts + (size_t)C * FRAG;
-      for (int i = threadIdx.x; i < FRAG; i += kThreads) lds[i] = src[i];
+     for (int h = 0; h < 2; ++h) {
+      const int S = 2 * C + h;
+      const v4i* cur = h ? buf1 : buf0;
+      if (S + 1 < NS) dma_stage<NT>(wts + (size_t)(S + 1) * FRAG2, h ? buf0 : buf1, wave, lane);
+      if (h == 0) {
+        const uint4 mk = maskv[4 * C + g];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        raw[r].x &= mk.x;
-        raw[r].y &= mk.y;
-        raw[r].z &= mk.z;
-        raw[r].w &= mk.w;
-        cnt[r] += __popc(raw[r].x) + __popc(raw[r].y) + __popc(raw[r].z) + __popc(raw[r].w);
+        for (int r = 0; r < R; ++r) {
+          raw[r].x = rawn[r].x & mk.x;
+          raw[r].y = rawn[r].y & mk.y;
+          raw[r].z = rawn[r].z & mk.z;
+          raw[r].w = rawn[r].w & mk.w;
+          cnt[r] += __popc(raw[r].x) + __popc(raw[r].y) + __popc(raw[r].z) + __popc(raw[r].w);
+        }
+        if (C + 1 < nchunks) load_chunk<R>(rowp, C + 1, rawn);
       }
-      __syncthreads();
 #pragma unroll
-      for (int j = 0; j < kI8Steps; ++j) {
+      for (int jj = 0; jj < kStageSteps; ++jj) {
         v4i a[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) a[t] = lds[(j * NT + t) * 64 + lane];
+        for (int t = 0; t < NT; ++t) a[t] = cur[(jj * NT + t) * 64 + lane];
         v4i bf[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const uint32_t w32 = (j < 2) ? raw[r].x : (j < 4) ? raw[r].y : (j < 6) ? raw[r].z : raw[r].w;
-          const uint32_t w16 = w32 >> (16 * (j & 1));
+          const uint32_t lo = h ? raw[r].z : raw[r].x;
+          const uint32_t hi = h ? raw[r].w : raw[r].y;
+          const uint32_t w16 = ((jj < 2) ? lo : hi) >> (16 * (jj & 1));
           bf[r] = v4i{spread4(w16 & 15u), spread4((w16 >> 4) & 15u), spread4((w16 >> 8) & 15u),
                       spread4((w16 >> 12) & 15u)};
         }
@@ -189,11 +240,25 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
           for (int r = 0; r < R; ++r)
             acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[t], bf[r], acc[r][t], 0, 0, 0);
       }
+      __builtin_amdgcn_s_waitcnt(0);  // next stage's weights (and next chunk's bits) landed
+      __syncthreads();
+     }
+    }
+
+    // next block: locate its shard, prefetch its first stage (buf0 is free:
+    // the last stage used buf1) and its first bits chunk
+    const int64_t wn = w + gridDim.x;
+    int sn = s;
+    while (sn + 1 < nshards && sd[sn + 1].blk0 <= wn) ++sn;
+    if (wn < total_blocks) {
+      const uint64_t* rp[R];
+      block_rows<R>(sd[sn], wn, wave, g, col, rp);
+      load_chunk<R>(rp, 0, rawn);
+      dma_stage<NT>(reinterpret_cast<const v4i*>(sd[sn].wts), buf0, wave, lane);
     }
 
     // ---- epilogue: per tile, spill D to LDS as [k-mer][weight column] ----
-    __syncthreads();
-    int* sc = reinterpret_cast<int*>(lds) + wave * 16 * NU;
+    int* sc = sc_all + wave * 16 * NUP;
     const double inv_n = 1.0 / (double)ep.n;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -202,21 +267,27 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
       cc += __shfl_xor(cc, 32);
 #pragma unroll
       for (int t = 0; t < NT; ++t)
-        *reinterpret_cast<v4i*>(sc + col * NU + 16 * t + 4 * g) = acc[r][t];
+        *reinterpret_cast<v4i*>(sc + col * NUP + 16 * t + 4 * g) = acc[r][t];
       __syncthreads();
       const int c = (int)__shfl(cc, kk);
       const int64_t kidx = kbase + 16 * r + kk;
-      const bool valid = (kidx < m);
-      const int* drow = sc + kk * NU;
+      const bool valid = (kidx < D.m);
+      const int* drow = sc + kk * NUP;
       const double cd = (double)c, cn = cd * inv_n;
+      const int L = ep.L, nz = ep.nz;
 
-      double zz = 0.0, bzz = 0.0, cross = 0.0, bcross = 0.0;
-      for (int b = q; b < ep.nz; b += 4) {
+      // one pass over the structure columns: |z|^2, its bound, the intercept
+      // cross term, and trait 0's z'b'
+      double zz = 0.0, bzz = 0.0, cross = 0.0, bcross = 0.0, S0 = 0.0, bS0 = 0.0;
+      for (int b = q; b < nz; b += 4) {
         double z = 0.0;
-        for (int l = 0; l < ep.L; ++l) z += ep.colw[b * ep.L + l] * (double)drow[ep.L * b + l];
+        for (int l = 0; l < L; ++l) z += ep.colw[b * L + l] * (double)drow[L * b + l];
         const double e = cd * ep.colh[b];
         zz += z * z;
         bzz += 2.0 * fabs(z) * e + e * e;
+        const double bpk = ep.bp[b];
+        S0 += z * bpk;
+        bS0 += e * fabs(bpk);
         if (ep.has_d && b == 0) {
           cross = 2.0 * cn * ep.sigma * z;
           bcross = 2.0 * cn * fabs(ep.sigma) * e;
@@ -234,17 +305,21 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
       }
 
       for (int tr = 0; tr < ep.t; ++tr) {
-        double S = 0.0, bS = 0.0, sv = 0.0, es = 0.0;
-        for (int b = q; b < ep.nz; b += 4) {
-          double z = 0.0;
-          for (int l = 0; l < ep.L; ++l) z += ep.colw[b * ep.L + l] * (double)drow[ep.L * b + l];
-          const double bpk = ep.bp[tr * ep.nz + b];
-          S += z * bpk;
-          bS += cd * ep.colh[b] * fabs(bpk);
+        double S = S0, bS = bS0, sv = 0.0, es = 0.0;
+        if (tr > 0) {
+          S = 0.0;
+          bS = 0.0;
+          for (int b = q; b < nz; b += 4) {
+            double z = 0.0;
+            for (int l = 0; l < L; ++l) z += ep.colw[b * L + l] * (double)drow[L * b + l];
+            const double bpk = ep.bp[tr * nz + b];
+            S += z * bpk;
+            bS += cd * ep.colh[b] * fabs(bpk);
+          }
         }
         if (q == 0) {
-          const int b = ep.nz + tr;
-          for (int l = 0; l < ep.L; ++l) sv += ep.colw[b * ep.L + l] * (double)drow[ep.L * b + l];
+          const int b = nz + tr;
+          for (int l = 0; l < L; ++l) sv += ep.colw[b * L + l] * (double)drow[L * b + l];
           es = cd * ep.colh[b];
         }
         S = wave_sum4(S + sv);
@@ -286,6 +361,8 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
       if (v) atomicAdd(&D.pass[threadIdx.x], (unsigned long long)v);
       wg_pass[threadIdx.x] = 0;
     }
+    w = wn;
+    s = sn;
   }
 }
 
@@ -532,12 +609,17 @@ static hipError_t launch_i8_tr(dim3 grid, size_t shm, hipStream_t s, const Shard
   return hipGetLastError();
 }
 
+// Dynamic LDS of scan_i8<NT, *>: two weight stages + the epilogue rows.
+static size_t i8_lds_bytes(int NT) {
+  return (size_t)2 * kStageSteps * NT * 64 * 16 + (size_t)kWaves * 16 * (16 * NT + 4) * 4;
+}
+
 // k-mers per workgroup block for tiles-per-wave R
 int64_t i8_block_kmers(int R) { return (int64_t)kWaves * 16 * R; }
 
 hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const ShardDesc* sd,
                           int nshards, int64_t total_blocks, const ScanOut& o) {
-  const size_t shm = (size_t)kI8Steps * NT * 64 * 16;
+  const size_t shm = i8_lds_bytes(NT);
   dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
   switch (NT * 8 + R) {
 #define KG_CASE(X)                                                                  \
@@ -595,10 +677,10 @@ template <int NT>
 static hipError_t set_attr_nt() {
   hipError_t e = hipFuncSetAttribute((const void*)scan_i8<NT, 4>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     kI8Steps * NT * 64 * 16);
+                                     (int)i8_lds_bytes(NT));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)scan_i8<NT, 2>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kI8Steps * NT * 64 * 16);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)i8_lds_bytes(NT));
   if (e == hipSuccess && NT <= 6)
     e = hipFuncSetAttribute((const void*)scan_f32<(NT <= 6 ? NT : 6), 4>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * NT * 64 * 4);

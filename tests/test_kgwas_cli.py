@@ -141,8 +141,7 @@ def test_kmer_count_mismatch(small, tmp_path):
 
 
 def test_no_gpu_fails_cleanly(small, tmp_path):
-    import torch
-    if torch.cuda.is_available():
+    if os.path.exists("/dev/kfd"):
         pytest.skip("a GPU is present")
     ds, bits, p, root = small
     r = run(base_args(p, str(tmp_path / "o.tsv")))
