@@ -117,7 +117,7 @@ struct Model {
 struct ModelDev {
   Model md;
   int NT = 0, nz = 0, nb = 0;
-  DBuf wts, mask, colw, colh, bp, bp0, rbar, ycn, yyM, B64, bpfull;  // int8 path
+  DBuf wts, mask, colw, colh, bp, bp0, rbar, ycn, yyM, B64, bpfull, colrec;  // int8 path
   DBuf wf32, uyf;                                                     // FP32 path
   EpiI8 epi{};
   EpiF32 epf{};
@@ -139,8 +139,8 @@ struct kg_ctx {
   int64_t cand_cap = 0;
   size_t counters_len = 0;
   kg_timing timing{};
-  int R = 4;
-  int grid_mult = 8;
+  int R = 2;          // k-mer tiles per wave (KG_TILES_PER_WAVE)
+  int grid_mult = 4;  // persistent workgroups per CU (KG_GRID_MULT)
   // captured graph of the last batched scan
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -377,7 +377,25 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
   KG_CK(ctx, up(M.yyM, md.yyM.data(), t * 8));
   KG_CK(ctx, up(M.B64, B.data(), B.size() * 8));
   KG_CK(ctx, up(M.bpfull, md.bprime.data(), std::max<size_t>(md.bprime.size(), 1) * 8));
+  // per-column epilogue records {w0, w1, w2, h, b'_0, 0} and bound sums
+  if (nb > kMaxCols) return fail(ctx, KG_ERR_ARG, "too many weight columns for the int8 kernel");
+  std::vector<double> rec((size_t)nb * kColRec, 0.0);
+  double sum_h2 = 0.0;
+  std::vector<double> sum_hbp(kMaxTraits, 0.0);
+  for (int b = 0; b < nb; ++b) {
+    for (int l = 0; l < L; ++l) rec[(size_t)b * kColRec + l] = colw[(size_t)b * L + l];
+    rec[(size_t)b * kColRec + 3] = colh[b];
+    if (b < nz) {
+      rec[(size_t)b * kColRec + 4] = bp[b];  // trait 0
+      sum_h2 += colh[b] * colh[b];
+      for (int tr = 0; tr < t; ++tr) sum_hbp[tr] += colh[b] * std::fabs(bp[(size_t)tr * nz + b]);
+    }
+  }
+  KG_CK(ctx, up(M.colrec, rec.data(), rec.size() * 8));
   EpiI8& ep = M.epi;
+  ep.colrec = M.colrec.as<double>();
+  ep.sum_h2 = sum_h2 * (1.0 + 1e-12);
+  for (int tr = 0; tr < kMaxTraits; ++tr) ep.sum_hbp[tr] = sum_hbp[tr] * (1.0 + 1e-12);
   ep.n = n;
   ep.nz = nz;
   ep.t = t;
@@ -713,6 +731,8 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     D.nchunks = md.nch_i8;
     D.pad = 0;
     D.pass = dc + 1 + (size_t)s * t;
+    D.id = s;
+    D.pad2 = 0;
     D.ep = M.epi;
     D.ep.tau = prm.corr_threshold;
     D.ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;

@@ -57,7 +57,12 @@ struct EpiI8 {
   const double* rbar;   // [t] mean of r = y - U U'y
   const double* ycn;    // [t] |y - mean y|
   const double* yyM;    // [t]
+  const double* colrec; // [nb][kColRec] {w0, w1, w2, h, b'_0 (trait 0), 0}: staged in LDS
+  double sum_h2;        // sum over z columns of h^2   (|z|^2 bound: 2c sum|z|h + c^2 sum h^2)
+  double sum_hbp[8];    // per trait: sum over z columns of h |b'|
 };
+constexpr int kColRec = 6;        // doubles per column record
+constexpr int kMaxCols = 48;      // base columns the int8 kernel supports (NT <= 8, L = 3)
 
 // One shard of a batched int8 scan: its rows and its null model.  The
 // persistent kernel walks the concatenation of all shards' 256-k-mer blocks.
@@ -70,6 +75,8 @@ struct ShardDesc {
   int nchunks;                   // 512-sample chunks of the model
   int pad;
   unsigned long long* pass;      // [t] decided pre-filter passes of this shard
+  int id;                        // index of this shard in the caller's table
+  int pad2;
   EpiI8 ep;
 };
 

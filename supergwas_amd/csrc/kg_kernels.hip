@@ -161,6 +161,7 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
   v4i* const buf0 = lds;
   v4i* const buf1 = lds + FRAG2;
   int* const sc_all = reinterpret_cast<int*>(lds + 2 * FRAG2);
+  double* const colrec_lds = reinterpret_cast<double*>(sc_all + kWaves * 16 * NUP);
   __shared__ unsigned int wg_pass[kMaxTraits];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, col = lane & 15;
@@ -258,6 +259,10 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
     }
 
     // ---- epilogue: per tile, spill D to LDS as [k-mer][weight column] ----
+    // Column records (limb weights, bound, b') are staged in LDS once per
+    // block; lane (kk, q) of a tile evaluates k-mer kk over columns q, q+4, ...
+    const int nz = ep.nz, L = ep.L, nbc = nz + ep.t;
+    for (int i = threadIdx.x; i < nbc * kColRec; i += kThreads) colrec_lds[i] = ep.colrec[i];
     int* sc = sc_all + wave * 16 * NUP;
     const double inv_n = 1.0 / (double)ep.n;
 #pragma unroll
@@ -274,29 +279,31 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
       const bool valid = (kidx < D.m);
       const int* drow = sc + kk * NUP;
       const double cd = (double)c, cn = cd * inv_n;
-      const int L = ep.L, nz = ep.nz;
 
-      // one pass over the structure columns: |z|^2, its bound, the intercept
-      // cross term, and trait 0's z'b'
-      double zz = 0.0, bzz = 0.0, cross = 0.0, bcross = 0.0, S0 = 0.0, bS0 = 0.0;
+      // one pass over the structure columns: |z|^2, sum |z| h (bound),
+      // trait 0's z'b', and z_0 (intercept-remainder column)
+      double zz = 0.0, za = 0.0, S0 = 0.0, z0 = 0.0;
       for (int b = q; b < nz; b += 4) {
-        double z = 0.0;
-        for (int l = 0; l < L; ++l) z += ep.colw[b * L + l] * (double)drow[L * b + l];
-        const double e = cd * ep.colh[b];
+        const double* cr = colrec_lds + b * kColRec;
+        double z = cr[0] * (double)drow[L * b];
+        if (L > 1) z += cr[1] * (double)drow[L * b + 1];
+        if (L > 2) z += cr[2] * (double)drow[L * b + 2];
         zz += z * z;
-        bzz += 2.0 * fabs(z) * e + e * e;
-        const double bpk = ep.bp[b];
-        S0 += z * bpk;
-        bS0 += e * fabs(bpk);
-        if (ep.has_d && b == 0) {
-          cross = 2.0 * cn * ep.sigma * z;
-          bcross = 2.0 * cn * fabs(ep.sigma) * e;
-        }
+        za += fabs(z) * cr[3];
+        S0 += z * cr[4];
+        if (b == 0) z0 = z;
       }
       zz = wave_sum4(zz);
-      bzz = wave_sum4(bzz);
-      cross = wave_sum4(cross);
-      bcross = wave_sum4(bcross);
+      za = wave_sum4(za);
+      S0 = wave_sum4(S0);
+      z0 = wave_sum4(z0);
+      // |dz_k| <= c h_k  =>  |d |z|^2| <= 2c sum|z_k|h_k + c^2 sum h_k^2
+      const double bzz = 2.0 * cd * za + cd * cd * ep.sum_h2;
+      double cross = 0.0, bcross = 0.0;
+      if (ep.has_d) {
+        cross = 2.0 * cn * ep.sigma * z0;
+        bcross = 2.0 * cn * fabs(ep.sigma) * cd * colrec_lds[3];
+      }
       const double xxM = cd - cd * cd * ep.base_coef - cross - zz;
       double bxx = bzz + bcross + 1e-11 * (cd + zz + fabs(cross) + 1.0);
       if (ep.dmax_drop > 0.0) {
@@ -305,27 +312,27 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
       }
 
       for (int tr = 0; tr < ep.t; ++tr) {
-        double S = S0, bS = bS0, sv = 0.0, es = 0.0;
+        double S = S0;
         if (tr > 0) {
           S = 0.0;
-          bS = 0.0;
           for (int b = q; b < nz; b += 4) {
-            double z = 0.0;
-            for (int l = 0; l < L; ++l) z += ep.colw[b * L + l] * (double)drow[L * b + l];
-            const double bpk = ep.bp[tr * nz + b];
-            S += z * bpk;
-            bS += cd * ep.colh[b] * fabs(bpk);
+            const double* cr = colrec_lds + b * kColRec;
+            double z = cr[0] * (double)drow[L * b];
+            if (L > 1) z += cr[1] * (double)drow[L * b + 1];
+            if (L > 2) z += cr[2] * (double)drow[L * b + 2];
+            S += z * ep.bp[tr * nz + b];
           }
+          S = wave_sum4(S);
         }
-        if (q == 0) {
-          const int b = nz + tr;
-          for (int l = 0; l < L; ++l) sv += ep.colw[b * L + l] * (double)drow[L * b + l];
-          es = cd * ep.colh[b];
-        }
-        S = wave_sum4(S + sv);
-        bS = wave_sum4(bS + es);
-        sv = wave_sum4(sv);
-        es = wave_sum4(es);
+        // residual column of this trait (every lane reads it: no reduction)
+        const double* crr = colrec_lds + (nz + tr) * kColRec;
+        const int br = nz + tr;
+        double sv = crr[0] * (double)drow[L * br];
+        if (L > 1) sv += crr[1] * (double)drow[L * br + 1];
+        if (L > 2) sv += crr[2] * (double)drow[L * br + 2];
+        const double es = cd * crr[3];
+        S += sv;
+        double bS = cd * ep.sum_hbp[tr] + es;
         if (ep.dmax_drop > 0.0) bS += cd * ep.dmax_drop * fabs(ep.bp0[tr]);
         bS += 1e-12 * (fabs(S) + fabs(sv) + 1.0);
         const double xyM = sv + cd * ep.rbar[tr];
@@ -347,7 +354,7 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
         rec.idx = kidx;
         rec.trait = tr;
         rec.flags = fl;
-        rec.shard = s;
+        rec.shard = D.id;
         rec.xx32 = 0.f;
         rec.xy32 = 0.f;
         rec.pad = 0;
@@ -611,7 +618,8 @@ static hipError_t launch_i8_tr(dim3 grid, size_t shm, hipStream_t s, const Shard
 
 // Dynamic LDS of scan_i8<NT, *>: two weight stages + the epilogue rows.
 static size_t i8_lds_bytes(int NT) {
-  return (size_t)2 * kStageSteps * NT * 64 * 16 + (size_t)kWaves * 16 * (16 * NT + 4) * 4;
+  return (size_t)2 * kStageSteps * NT * 64 * 16 + (size_t)kWaves * 16 * (16 * NT + 4) * 4 +
+         (size_t)kMaxCols * kColRec * 8;
 }
 
 // k-mers per workgroup block for tiles-per-wave R
