@@ -42,6 +42,10 @@ def main():
     ap.add_argument("--err-sample", type=int, default=50_000, help="k-mers for error vs FP64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--limbs", type=int, default=3)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the mixed scans (for rocprofv3 counter runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="copy this many bytes device-to-device and exit (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -52,6 +56,15 @@ def main():
     from supergwas_amd import synth
     from supergwas_amd.api import KgwasHip, ScanParams
     from tests.helpers import T_from, rel_err_T
+
+    if args.copy_calibration:
+        k = KgwasHip()
+        a, b = k.alloc(args.copy_calibration), k.alloc(args.copy_calibration)
+        from supergwas_amd.api import lib
+        for _ in range(3):
+            k._check(lib().kg_memcpy(k._h, b.ptr, a.ptr, args.copy_calibration, 2))
+        log(f"[bench] copied {args.copy_calibration} bytes device-to-device x3")
+        return
 
     cfg = dict(synth.CONFIGS["mixed"])
     cfg["m"] = args.m
@@ -86,6 +99,9 @@ def main():
     log(f"[bench] mixed: {ms_step:.2f} ms/step, {value:.3e} k-mers/s, bulk {np.mean(bulk_ms):.3f} ms,"
         f" refine {np.mean(ref_ms):.3f} ms, candidates {int(np.mean(cands))}, hits {len(hits)},"
         f" passed {int(npass[0])}")
+
+    if args.profile_only:
+        return
 
     # ---- plain FP32 kernel ----
     fp32 = KgwasHip(precision="fp32")

@@ -132,10 +132,15 @@ __device__ __forceinline__ void dma_stage(const v4i* __restrict__ src, v4i* dst,
   }
 }
 
+// Each lane's 16 bytes of chunk C for every tile, into this wave's LDS slot.
 template <int R>
-__device__ __forceinline__ void load_chunk(const uint64_t* const (&rowp)[R], int C, uint4 (&raw)[R]) {
+__device__ __forceinline__ void dma_bits(const uint64_t* const (&rowp)[R], int C, v4i* dst,
+                                         int lane) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) raw[r] = *reinterpret_cast<const uint4*>(rowp[r] + 8 * C);
+  for (int r = 0; r < R; ++r)
+    __builtin_amdgcn_global_load_lds((const void*)(rowp[r] + 8 * C),
+                                     (__attribute__((address_space(3))) void*)(dst + r * 64), 16,
+                                     0, 0);
 }
 
 template <int R>
@@ -151,32 +156,33 @@ __device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wa
 }
 
 template <int NT, int R>
-__global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8(const ShardDesc* __restrict__ sd,
+__global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
                                                         int nshards, int64_t total_blocks,
                                                         ScanOut out) {
   constexpr int NU = 16 * NT;
-  constexpr int NUP = NU + 4;                        // padded epilogue row (bank spread)
   constexpr int FRAG2 = kStageSteps * NT * 64;       // v4i per 256-sample stage
+  constexpr int SWZ = (NT % 2 == 0) ? 7 : 3;         // epilogue row swizzle (16-byte chunks)
   extern __shared__ v4i lds[];
   v4i* const buf0 = lds;
   v4i* const buf1 = lds + FRAG2;
-  int* const sc_all = reinterpret_cast<int*>(lds + 2 * FRAG2);
-  double* const colrec_lds = reinterpret_cast<double*>(sc_all + kWaves * 16 * NUP);
+  v4i* const bitsbuf = lds + 2 * FRAG2;              // [wave][r][lane] next bits chunk
+  double* const colrec_lds = reinterpret_cast<double*>(bitsbuf + kWaves * R * 64);
+  int* const sc_all = reinterpret_cast<int*>(buf1);  // epilogue rows alias the spent stage
   __shared__ unsigned int wg_pass[kMaxTraits];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, col = lane & 15;
   const int kk = lane >> 2, q = lane & 3;
+  v4i* const mybits = bitsbuf + wave * R * 64;
   if (threadIdx.x < kMaxTraits) wg_pass[threadIdx.x] = 0;
 
   int64_t w = blockIdx.x;
   if (w >= total_blocks) return;
   int s = 0;
   while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
-  uint4 rawn[R];
   {
     const uint64_t* rp[R];
     block_rows<R>(sd[s], w, wave, g, col, rp);
-    load_chunk<R>(rp, 0, rawn);
+    dma_bits<R>(rp, 0, mybits, lane);
     dma_stage<NT>(reinterpret_cast<const v4i*>(sd[s].wts), buf0, wave, lane);
   }
 
@@ -203,43 +209,43 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
 
     __builtin_amdgcn_s_waitcnt(0);  // stage 0 weights and chunk 0 bits have landed
     __syncthreads();
-    for (int C = 0; C < nchunks; ++C) {
-#pragma unroll
-     for (int h = 0; h < 2; ++h) {
-      const int S = 2 * C + h;
+    for (int S = 0; S < NS; ++S) {
+     {
+      const int h = S & 1, C = S >> 1;  // stage parity, 512-sample chunk
       const v4i* cur = h ? buf1 : buf0;
       if (S + 1 < NS) dma_stage<NT>(wts + (size_t)(S + 1) * FRAG2, h ? buf0 : buf1, wave, lane);
       if (h == 0) {
         const uint4 mk = maskv[4 * C + g];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          raw[r].x = rawn[r].x & mk.x;
-          raw[r].y = rawn[r].y & mk.y;
-          raw[r].z = rawn[r].z & mk.z;
-          raw[r].w = rawn[r].w & mk.w;
+          const v4i v = mybits[r * 64 + lane];
+          raw[r].x = (uint32_t)v.x & mk.x;
+          raw[r].y = (uint32_t)v.y & mk.y;
+          raw[r].z = (uint32_t)v.z & mk.z;
+          raw[r].w = (uint32_t)v.w & mk.w;
           cnt[r] += __popc(raw[r].x) + __popc(raw[r].y) + __popc(raw[r].z) + __popc(raw[r].w);
         }
-        if (C + 1 < nchunks) load_chunk<R>(rowp, C + 1, rawn);
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's bits reads are done
+        if (C + 1 < nchunks) dma_bits<R>(rowp, C + 1, mybits, lane);
       }
 #pragma unroll
       for (int jj = 0; jj < kStageSteps; ++jj) {
-        v4i a[NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) a[t] = cur[(jj * NT + t) * 64 + lane];
         v4i bf[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const uint32_t lo = h ? raw[r].z : raw[r].x;
-          const uint32_t hi = h ? raw[r].w : raw[r].y;
+          const uint32_t lo = raw[r].x ^ ((raw[r].x ^ raw[r].z) & (0u - (uint32_t)h));
+          const uint32_t hi = raw[r].y ^ ((raw[r].y ^ raw[r].w) & (0u - (uint32_t)h));
           const uint32_t w16 = ((jj < 2) ? lo : hi) >> (16 * (jj & 1));
           bf[r] = v4i{spread4(w16 & 15u), spread4((w16 >> 4) & 15u), spread4((w16 >> 8) & 15u),
                       spread4((w16 >> 12) & 15u)};
         }
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
+        for (int t = 0; t < NT; ++t) {
+          const v4i a = cur[(jj * NT + t) * 64 + lane];
 #pragma unroll
           for (int r = 0; r < R; ++r)
-            acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[t], bf[r], acc[r][t], 0, 0, 0);
+            acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bf[r], acc[r][t], 0, 0, 0);
+        }
       }
       __builtin_amdgcn_s_waitcnt(0);  // next stage's weights (and next chunk's bits) landed
       __syncthreads();
@@ -254,7 +260,7 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
     if (wn < total_blocks) {
       const uint64_t* rp[R];
       block_rows<R>(sd[sn], wn, wave, g, col, rp);
-      load_chunk<R>(rp, 0, rawn);
+      dma_bits<R>(rp, 0, mybits, lane);
       dma_stage<NT>(reinterpret_cast<const v4i*>(sd[sn].wts), buf0, wave, lane);
     }
 
@@ -263,7 +269,7 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
     // block; lane (kk, q) of a tile evaluates k-mer kk over columns q, q+4, ...
     const int nz = ep.nz, L = ep.L, nbc = nz + ep.t;
     for (int i = threadIdx.x; i < nbc * kColRec; i += kThreads) colrec_lds[i] = ep.colrec[i];
-    int* sc = sc_all + wave * 16 * NUP;
+    int* sc = sc_all + wave * 16 * NU;
     const double inv_n = 1.0 / (double)ep.n;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -272,12 +278,15 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
       cc += __shfl_xor(cc, 32);
 #pragma unroll
       for (int t = 0; t < NT; ++t)
-        *reinterpret_cast<v4i*>(sc + col * NUP + 16 * t + 4 * g) = acc[r][t];
+        *reinterpret_cast<v4i*>(sc + col * NU + 4 * ((4 * t + g) ^ (col & SWZ))) = acc[r][t];
       __syncthreads();
       const int c = (int)__shfl(cc, kk);
       const int64_t kidx = kbase + 16 * r + kk;
       const bool valid = (kidx < D.m);
-      const int* drow = sc + kk * NUP;
+      const int* drow = sc + kk * NU;
+      const int ksw = kk & SWZ;
+      // column uc of this k-mer's row (16-byte chunks XOR-swizzled by row)
+#define KG_D(uc) drow[4 * (((uc) >> 2) ^ ksw) + ((uc) & 3)]
       const double cd = (double)c, cn = cd * inv_n;
 
       // one pass over the structure columns: |z|^2, sum |z| h (bound),
@@ -285,9 +294,9 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
       double zz = 0.0, za = 0.0, S0 = 0.0, z0 = 0.0;
       for (int b = q; b < nz; b += 4) {
         const double* cr = colrec_lds + b * kColRec;
-        double z = cr[0] * (double)drow[L * b];
-        if (L > 1) z += cr[1] * (double)drow[L * b + 1];
-        if (L > 2) z += cr[2] * (double)drow[L * b + 2];
+        double z = cr[0] * (double)KG_D(L * b);
+        if (L > 1) z += cr[1] * (double)KG_D(L * b + 1);
+        if (L > 2) z += cr[2] * (double)KG_D(L * b + 2);
         zz += z * z;
         za += fabs(z) * cr[3];
         S0 += z * cr[4];
@@ -317,9 +326,9 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
           S = 0.0;
           for (int b = q; b < nz; b += 4) {
             const double* cr = colrec_lds + b * kColRec;
-            double z = cr[0] * (double)drow[L * b];
-            if (L > 1) z += cr[1] * (double)drow[L * b + 1];
-            if (L > 2) z += cr[2] * (double)drow[L * b + 2];
+            double z = cr[0] * (double)KG_D(L * b);
+            if (L > 1) z += cr[1] * (double)KG_D(L * b + 1);
+            if (L > 2) z += cr[2] * (double)KG_D(L * b + 2);
             S += z * ep.bp[tr * nz + b];
           }
           S = wave_sum4(S);
@@ -327,9 +336,9 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
         // residual column of this trait (every lane reads it: no reduction)
         const double* crr = colrec_lds + (nz + tr) * kColRec;
         const int br = nz + tr;
-        double sv = crr[0] * (double)drow[L * br];
-        if (L > 1) sv += crr[1] * (double)drow[L * br + 1];
-        if (L > 2) sv += crr[2] * (double)drow[L * br + 2];
+        double sv = crr[0] * (double)KG_D(L * br);
+        if (L > 1) sv += crr[1] * (double)KG_D(L * br + 1);
+        if (L > 2) sv += crr[2] * (double)KG_D(L * br + 2);
         const double es = cd * crr[3];
         S += sv;
         double bS = cd * ep.sum_hbp[tr] + es;
@@ -362,6 +371,7 @@ __global__ __launch_bounds__(kThreads, (R >= 4 && NT >= 6) ? 1 : 2) void scan_i8
       }
       __syncthreads();
     }
+#undef KG_D
     // flush this block's pre-filter passes to its shard
     if (threadIdx.x < ep.t) {
       const unsigned int v = wg_pass[threadIdx.x];
@@ -617,8 +627,8 @@ static hipError_t launch_i8_tr(dim3 grid, size_t shm, hipStream_t s, const Shard
 }
 
 // Dynamic LDS of scan_i8<NT, *>: two weight stages + the epilogue rows.
-static size_t i8_lds_bytes(int NT) {
-  return (size_t)2 * kStageSteps * NT * 64 * 16 + (size_t)kWaves * 16 * (16 * NT + 4) * 4 +
+static size_t i8_lds_bytes(int NT, int R) {
+  return (size_t)2 * kStageSteps * NT * 64 * 16 + (size_t)kWaves * R * 64 * 16 +
          (size_t)kMaxCols * kColRec * 8;
 }
 
@@ -627,7 +637,7 @@ int64_t i8_block_kmers(int R) { return (int64_t)kWaves * 16 * R; }
 
 hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const ShardDesc* sd,
                           int nshards, int64_t total_blocks, const ScanOut& o) {
-  const size_t shm = i8_lds_bytes(NT);
+  const size_t shm = i8_lds_bytes(NT, R);
   dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
   switch (NT * 8 + R) {
 #define KG_CASE(X)                                                                  \
@@ -685,10 +695,10 @@ template <int NT>
 static hipError_t set_attr_nt() {
   hipError_t e = hipFuncSetAttribute((const void*)scan_i8<NT, 4>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)i8_lds_bytes(NT));
+                                     (int)i8_lds_bytes(NT, 4));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)scan_i8<NT, 2>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)i8_lds_bytes(NT));
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)i8_lds_bytes(NT, 2));
   if (e == hipSuccess && NT <= 6)
     e = hipFuncSetAttribute((const void*)scan_f32<(NT <= 6 ? NT : 6), 4>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * NT * 64 * 4);
