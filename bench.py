@@ -155,6 +155,13 @@ def main():
         log(f"[bench] cpu: {cs / cel:.3e} k-mers/s on {threads} threads ({cel:.1f}s)")
 
     # ---- roofline of the bulk kernel ----
+    # HBM bytes per k-mer measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (copy-calibrated)
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic_scan_i8_n2048_p32.md")
+    if os.path.exists(tp):
+        for ln in open(tp):
+            if ln.startswith("traffic_bytes_per_kmer:"):
+                traffic = float(ln.split(":")[1]) * args.m
     bulk_s = np.mean(bulk_ms) / 1e3
     alg_bytes = args.m * (n_acc / 8.0)
     achieved = alg_bytes / bulk_s / 1e9
@@ -176,7 +183,7 @@ def main():
                                f"{args.limbs} int8 limbs", "n": n, "p": p, "m": args.m,
                    "corr_threshold": 0.2, "log10_thresh": 6.0},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "mfma": {"algorithmic_TOPs": alg_ops / bulk_s / 1e12,
                  "executed_TOPs": 2.0 * n_acc * (16 * int(np.ceil(args.limbs * (p) / 16.0)))
                  * args.m / bulk_s / 1e12, "peak_int8_TOPs": I8_PEAK_TOPS},

@@ -26,9 +26,9 @@ hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const ShardDes
                           int nshards, int64_t total_blocks, const ScanOut& o);
 int64_t i8_block_kmers(int R);
 hipError_t set_kernel_attributes();
-hipError_t launch_scan_f32(int NT, int grid, hipStream_t s, const uint64_t* bits, int64_t m,
-                           int64_t rw, int64_t row0, const float* wts, const uint64_t* mask,
-                           const EpiF32& ep, const ScanOut& o);
+hipError_t launch_scan_f32(int NT, int nch, int grid, hipStream_t s, const uint64_t* bits,
+                           int64_t m, int64_t rw, int64_t row0, const float* wts,
+                           const uint64_t* mask, const EpiF32& ep, const ScanOut& o);
 hipError_t launch_refine(hipStream_t s, const Cand* cand, const unsigned long long* count,
                          int64_t cap, const RefDesc* rd, double* res, int blocks);
 hipError_t launch_zero(hipStream_t s, unsigned long long* p, int n);
@@ -139,8 +139,8 @@ struct kg_ctx {
   int64_t cand_cap = 0;
   size_t counters_len = 0;
   kg_timing timing{};
-  int R = 2;          // k-mer tiles per wave (KG_TILES_PER_WAVE)
-  int grid_mult = 4;  // persistent workgroups per CU (KG_GRID_MULT)
+  int R = 4;          // k-mer tiles per wave (KG_TILES_PER_WAVE)
+  int grid_mult = 8;  // persistent workgroups per CU (KG_GRID_MULT)
   // captured graph of the last batched scan
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -922,7 +922,8 @@ static int scan_f32_shard(kg_ctx* ctx, const HShard& s, int sidx, const kg_scan_
     KG_CK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     KG_CK(ctx, launch_zero(ctx->stream, o.counters, 1 + kMaxTraits));
     KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    KG_CK(ctx, launch_scan_f32(M.NT, grid, ctx->stream, s.bits, s.m, s.rw, 0, M.wf32.as<float>(),
+    KG_CK(ctx, launch_scan_f32(M.NT, md.nch_f32, grid, ctx->stream, s.bits, s.m, s.rw, 0,
+                               M.wf32.as<float>(),
                                M.mask.as<uint64_t>(), ep, o));
     KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     KG_CK(ctx, hipMemcpyAsync(cnt, o.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));

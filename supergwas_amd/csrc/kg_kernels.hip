@@ -663,10 +663,10 @@ static hipError_t launch_f32_nt(dim3 grid, size_t shm, hipStream_t s, const uint
   return hipGetLastError();
 }
 
-hipError_t launch_scan_f32(int NT, int grid, hipStream_t s, const uint64_t* bits, int64_t m,
-                           int64_t rw, int64_t row0, const float* wts, const uint64_t* mask,
-                           const EpiF32& ep, const ScanOut& o) {
-  const int nch = (int)(rw / 4);
+// nch = the model's 256-sample chunks (the weights hold exactly that many).
+hipError_t launch_scan_f32(int NT, int nch, int grid, hipStream_t s, const uint64_t* bits,
+                           int64_t m, int64_t rw, int64_t row0, const float* wts,
+                           const uint64_t* mask, const EpiF32& ep, const ScanOut& o) {
   const int64_t per_block = (int64_t)kWaves * 16 * 4;
   const int64_t nblocks = (m + per_block - 1) / per_block;
   const size_t shm = (size_t)64 * NT * 64 * 4;
