@@ -312,7 +312,9 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
     if (!(k == 0 && md.rotated && md.drop_d)) zc.push_back(k);
   const int nz = (int)zc.size();
   const int nb = nz + t;
-  const int NU = ((L * nb + 15) / 16) * 16;
+  // the epilogue reads each lane's structure limbs as whole 16-byte chunks of
+  // 4 columns, so the row must also cover L * (nz rounded up to 4) columns
+  const int NU = std::max(((L * nb + 15) / 16) * 16, ((L * 4 * ((nz + 3) / 4) + 15) / 16) * 16);
   const int NT = NU / 16;
   if (NT > 8)
     return fail(ctx, KG_ERR_ARG,

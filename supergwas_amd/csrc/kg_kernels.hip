@@ -63,12 +63,11 @@ __device__ __forceinline__ void append_cand(bool want, const Cand& rec, const Sc
 // Returns candidate flags (0 = decided "not reported and not counted").
 // *pf_def is set when the pre-filter is decided "pass" (counted on device).
 __device__ uint32_t classify_i8(int c, double xxM, double bxx, double S, double bS,
-                                double xyM, double bxy, const EpiI8& ep, int tr,
-                                bool* pf_def) {
+                                double xyM, double bxy, const EpiI8& ep, double ycn,
+                                double yy, bool* pf_def) {
   *pf_def = false;
   const int n = ep.n;
   if (c <= 0 || c >= n) return 0;               // |x - mean x| = 0 (src/kgwas.cpp:465)
-  const double ycn = ep.ycn[tr];
   if (!(ycn >= 1e-12)) return 0;                // |y - mean y| < eps
   uint32_t f;
   if (ep.tau_trivial) {
@@ -83,7 +82,6 @@ __device__ uint32_t classify_i8(int c, double xxM, double bxx, double S, double 
   }
   *pf_def = (f == kPfPass);
   uint32_t s;
-  const double yy = ep.yyM[tr];
   if (ep.all_sig) {
     s = kSigDef;
   } else if (!(yy > 1e-12)) {
@@ -143,6 +141,12 @@ __device__ __forceinline__ void dma_bits(const uint64_t* const (&rowp)[R], int C
                                      0, 0);
 }
 
+// The accession-mask slice of lane group g for chunk C (each lane its own copy).
+__device__ __forceinline__ void dma_mask(const void* mask, int C, int g, v4i* dst) {
+  __builtin_amdgcn_global_load_lds((const void*)(reinterpret_cast<const uint4*>(mask) + 4 * C + g),
+                                   (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
 template <int R>
 __device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wave, int g, int col,
                                            const uint64_t* (&rowp)[R]) {
@@ -162,17 +166,20 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
   constexpr int NU = 16 * NT;
   constexpr int FRAG2 = kStageSteps * NT * 64;       // v4i per 256-sample stage
   constexpr int SWZ = (NT % 2 == 0) ? 7 : 3;         // epilogue row swizzle (16-byte chunks)
-  extern __shared__ v4i lds[];
-  v4i* const buf0 = lds;
-  v4i* const buf1 = lds + FRAG2;
-  v4i* const bitsbuf = lds + 2 * FRAG2;              // [wave][r][lane] next bits chunk
-  double* const colrec_lds = reinterpret_cast<double*>(bitsbuf + kWaves * R * 64);
+  // Separate LDS objects, so that reads of one buffer do not wait for the
+  // direct-to-LDS loads still filling another.
+  __shared__ v4i buf0[FRAG2];
+  __shared__ v4i buf1[FRAG2];
+  __shared__ v4i bitsbuf[kWaves * (R + 1) * 64];     // [wave][r | mask][lane] next chunk
+  __shared__ double colrec_lds[kMaxCols * kColRec];
+  __shared__ double bp_lds[kMaxTraits * kMaxCols];     // b' of traits 1.. (trait 0 is in colrec)
+  __shared__ double trc_lds[kMaxTraits * 4];           // per trait: mean r, |y - ybar|, yyM, b'_0
   int* const sc_all = reinterpret_cast<int*>(buf1);  // epilogue rows alias the spent stage
   __shared__ unsigned int wg_pass[kMaxTraits];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, col = lane & 15;
   const int kk = lane >> 2, q = lane & 3;
-  v4i* const mybits = bitsbuf + wave * R * 64;
+  v4i* const mybits = bitsbuf + wave * (R + 1) * 64;
   if (threadIdx.x < kMaxTraits) wg_pass[threadIdx.x] = 0;
 
   int64_t w = blockIdx.x;
@@ -183,6 +190,7 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
     const uint64_t* rp[R];
     block_rows<R>(sd[s], w, wave, g, col, rp);
     dma_bits<R>(rp, 0, mybits, lane);
+    dma_mask(sd[s].mask, 0, g, mybits + R * 64);
     dma_stage<NT>(reinterpret_cast<const v4i*>(sd[s].wts), buf0, wave, lane);
   }
 
@@ -190,7 +198,6 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
     const ShardDesc& D = sd[s];
     const EpiI8& ep = D.ep;
     const v4i* __restrict__ wts = reinterpret_cast<const v4i*>(D.wts);
-    const uint4* __restrict__ maskv = reinterpret_cast<const uint4*>(D.mask);
     const int nchunks = D.nchunks;
     const int NS = 2 * nchunks;
     const int64_t kbase = ((w - D.blk0) * kWaves + wave) * (16 * R);
@@ -215,7 +222,8 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
       const v4i* cur = h ? buf1 : buf0;
       if (S + 1 < NS) dma_stage<NT>(wts + (size_t)(S + 1) * FRAG2, h ? buf0 : buf1, wave, lane);
       if (h == 0) {
-        const uint4 mk = maskv[4 * C + g];
+        const v4i mkv = mybits[R * 64 + lane];
+        const uint4 mk = make_uint4((uint32_t)mkv.x, (uint32_t)mkv.y, (uint32_t)mkv.z, (uint32_t)mkv.w);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const v4i v = mybits[r * 64 + lane];
@@ -226,7 +234,10 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
           cnt[r] += __popc(raw[r].x) + __popc(raw[r].y) + __popc(raw[r].z) + __popc(raw[r].w);
         }
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's bits reads are done
-        if (C + 1 < nchunks) dma_bits<R>(rowp, C + 1, mybits, lane);
+        if (C + 1 < nchunks) {
+          dma_bits<R>(rowp, C + 1, mybits, lane);
+          dma_mask(D.mask, C + 1, g, mybits + R * 64);
+        }
       }
 #pragma unroll
       for (int jj = 0; jj < kStageSteps; ++jj) {
@@ -252,6 +263,22 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
      }
     }
 
+    // this shard's epilogue constants into LDS while no direct-to-LDS load is
+    // in flight (the epilogue then reads only LDS)
+    {
+      const int nbc0 = ep.nz + ep.t;
+      const double* __restrict__ gcr = ep.colrec;
+      for (int i = threadIdx.x; i < nbc0 * kColRec; i += kThreads) colrec_lds[i] = gcr[i];
+      for (int i = threadIdx.x; i < ep.t * ep.nz; i += kThreads) bp_lds[i] = ep.bp[i];
+      if (threadIdx.x < ep.t) {
+        trc_lds[4 * threadIdx.x + 0] = ep.rbar[threadIdx.x];
+        trc_lds[4 * threadIdx.x + 1] = ep.ycn[threadIdx.x];
+        trc_lds[4 * threadIdx.x + 2] = ep.yyM[threadIdx.x];
+        trc_lds[4 * threadIdx.x + 3] = ep.bp0[threadIdx.x];
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+    }
+
     // next block: locate its shard, prefetch its first stage (buf0 is free:
     // the last stage used buf1) and its first bits chunk
     const int64_t wn = w + gridDim.x;
@@ -261,14 +288,16 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
       const uint64_t* rp[R];
       block_rows<R>(sd[sn], wn, wave, g, col, rp);
       dma_bits<R>(rp, 0, mybits, lane);
+      dma_mask(sd[sn].mask, 0, g, mybits + R * 64);
       dma_stage<NT>(reinterpret_cast<const v4i*>(sd[sn].wts), buf0, wave, lane);
     }
 
     // ---- epilogue: per tile, spill D to LDS as [k-mer][weight column] ----
-    // Column records (limb weights, bound, b') are staged in LDS once per
-    // block; lane (kk, q) of a tile evaluates k-mer kk over columns q, q+4, ...
-    const int nz = ep.nz, L = ep.L, nbc = nz + ep.t;
-    for (int i = threadIdx.x; i < nbc * kColRec; i += kThreads) colrec_lds[i] = ep.colrec[i];
+    // Lane (kk, q) evaluates k-mer kk over the structure columns
+    // [q*cpl, (q+1)*cpl), cpl a multiple of 4 so a lane's limbs are whole
+    // 16-byte chunks of the row (rows XOR-swizzled by k-mer in 16-byte chunks).
+    const int nz = ep.nz, L = ep.L;
+    const int cpl = 4 * ((nz + 15) / 16);
     int* sc = sc_all + wave * 16 * NU;
     const double inv_n = 1.0 / (double)ep.n;
 #pragma unroll
@@ -289,18 +318,45 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
 #define KG_D(uc) drow[4 * (((uc) >> 2) ^ ksw) + ((uc) & 3)]
       const double cd = (double)c, cn = cd * inv_n;
 
-      // one pass over the structure columns: |z|^2, sum |z| h (bound),
-      // trait 0's z'b', and z_0 (intercept-remainder column)
+      // structure columns of this lane: |z|^2, sum |z| h (bound), trait 0's
+      // z'b', and z_0 (intercept-remainder column)
       double zz = 0.0, za = 0.0, S0 = 0.0, z0 = 0.0;
-      for (int b = q; b < nz; b += 4) {
-        const double* cr = colrec_lds + b * kColRec;
-        double z = cr[0] * (double)KG_D(L * b);
-        if (L > 1) z += cr[1] * (double)KG_D(L * b + 1);
-        if (L > 2) z += cr[2] * (double)KG_D(L * b + 2);
-        zz += z * z;
-        za += fabs(z) * cr[3];
-        S0 += z * cr[4];
-        if (b == 0) z0 = z;
+      const int b0 = q * cpl;
+      const int b1 = min(b0 + cpl, nz);
+      for (int bb = b0; bb < b1; bb += 4) {
+        // 4 columns = L*4 consecutive limbs = L 16-byte chunks
+        int lim[12];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (k < L) {
+            const int ch = ((L * bb) >> 2) + k;
+            const v4i v = *reinterpret_cast<const v4i*>(drow + 4 * (ch ^ ksw));
+            lim[4 * k + 0] = v.x;
+            lim[4 * k + 1] = v.y;
+            lim[4 * k + 2] = v.z;
+            lim[4 * k + 3] = v.w;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int b = bb + u;
+          if (b < b1) {
+            const double* cr = colrec_lds + b * kColRec;
+            double z;
+            if (L == 3) {  // limbs 1, 2 merged exactly in int32: a1*254 + a2
+              const int a12 = lim[3 * u + 1] * 254 + lim[3 * u + 2];
+              z = cr[0] * (double)lim[3 * u] + cr[2] * (double)a12;
+            } else if (L == 2) {
+              z = cr[0] * (double)lim[2 * u] + cr[1] * (double)lim[2 * u + 1];
+            } else {
+              z = cr[0] * (double)lim[u];
+            }
+            zz += z * z;
+            za += fabs(z) * cr[3];
+            S0 += z * cr[4];
+            if (b == 0) z0 = z;
+          }
+        }
       }
       zz = wave_sum4(zz);
       za = wave_sum4(za);
@@ -322,33 +378,36 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
 
       for (int tr = 0; tr < ep.t; ++tr) {
         double S = S0;
-        if (tr > 0) {
+        if (tr > 0) {  // further traits: z'b' again (rare path)
           S = 0.0;
-          for (int b = q; b < nz; b += 4) {
+          for (int b = b0; b < b1; ++b) {
             const double* cr = colrec_lds + b * kColRec;
             double z = cr[0] * (double)KG_D(L * b);
             if (L > 1) z += cr[1] * (double)KG_D(L * b + 1);
             if (L > 2) z += cr[2] * (double)KG_D(L * b + 2);
-            S += z * ep.bp[tr * nz + b];
+            S += z * bp_lds[tr * nz + b];
           }
           S = wave_sum4(S);
         }
         // residual column of this trait (every lane reads it: no reduction)
-        const double* crr = colrec_lds + (nz + tr) * kColRec;
         const int br = nz + tr;
+        const double* crr = colrec_lds + br * kColRec;
         double sv = crr[0] * (double)KG_D(L * br);
         if (L > 1) sv += crr[1] * (double)KG_D(L * br + 1);
         if (L > 2) sv += crr[2] * (double)KG_D(L * br + 2);
         const double es = cd * crr[3];
+        const double rbar = trc_lds[4 * tr + 0];
         S += sv;
         double bS = cd * ep.sum_hbp[tr] + es;
-        if (ep.dmax_drop > 0.0) bS += cd * ep.dmax_drop * fabs(ep.bp0[tr]);
+        if (ep.dmax_drop > 0.0) bS += cd * ep.dmax_drop * fabs(trc_lds[4 * tr + 3]);
         bS += 1e-12 * (fabs(S) + fabs(sv) + 1.0);
-        const double xyM = sv + cd * ep.rbar[tr];
-        const double bxy = es + 1e-12 * (fabs(sv) + cd * fabs(ep.rbar[tr]) + 1.0);
+        const double xyM = sv + cd * rbar;
+        const double bxy = es + 1e-12 * (fabs(sv) + cd * fabs(rbar) + 1.0);
         bool pf_def = false;
         uint32_t fl = 0;
-        if (q == 0 && valid) fl = classify_i8(c, xxM, bxx, S, bS, xyM, bxy, ep, tr, &pf_def);
+        if (q == 0 && valid)
+          fl = classify_i8(c, xxM, bxx, S, bS, xyM, bxy, ep, trc_lds[4 * tr + 1],
+                           trc_lds[4 * tr + 2], &pf_def);
         if (out.st_c != nullptr && q == 0 && valid) {
           if (tr == 0) {
             out.st_c[kidx] = c;
@@ -626,18 +685,12 @@ static hipError_t launch_i8_tr(dim3 grid, size_t shm, hipStream_t s, const Shard
   return hipGetLastError();
 }
 
-// Dynamic LDS of scan_i8<NT, *>: two weight stages + the epilogue rows.
-static size_t i8_lds_bytes(int NT, int R) {
-  return (size_t)2 * kStageSteps * NT * 64 * 16 + (size_t)kWaves * R * 64 * 16 +
-         (size_t)kMaxCols * kColRec * 8;
-}
-
 // k-mers per workgroup block for tiles-per-wave R
 int64_t i8_block_kmers(int R) { return (int64_t)kWaves * 16 * R; }
 
 hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const ShardDesc* sd,
                           int nshards, int64_t total_blocks, const ScanOut& o) {
-  const size_t shm = i8_lds_bytes(NT, R);
+  const size_t shm = 0;  // all LDS of scan_i8 is static
   dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
   switch (NT * 8 + R) {
 #define KG_CASE(X)                                                                  \
@@ -693,13 +746,8 @@ hipError_t launch_refine(hipStream_t s, const Cand* cand, const unsigned long lo
 // (outside any stream capture).
 template <int NT>
 static hipError_t set_attr_nt() {
-  hipError_t e = hipFuncSetAttribute((const void*)scan_i8<NT, 4>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)i8_lds_bytes(NT, 4));
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)scan_i8<NT, 2>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)i8_lds_bytes(NT, 2));
-  if (e == hipSuccess && NT <= 6)
+  hipError_t e = hipSuccess;
+  if (NT <= 6)
     e = hipFuncSetAttribute((const void*)scan_f32<(NT <= 6 ? NT : 6), 4>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * NT * 64 * 4);
   return e;
