@@ -22,9 +22,10 @@
 #include "kg_internal.h"
 
 namespace kg {
-hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const ShardDesc* sd,
+hipError_t launch_scan_i8(int NT, int L, int t, int grid, hipStream_t s, const ShardDesc* sd,
                           int nshards, int64_t total_blocks, const ScanOut& o);
-int64_t i8_block_kmers(int R);
+int64_t i8_block_kmers(int NT, int t);
+int i8_column_capacity(int NT, int L);
 hipError_t set_kernel_attributes();
 hipError_t launch_scan_f32(int NT, int nch, int grid, hipStream_t s, const uint64_t* bits,
                            int64_t m, int64_t rw, int64_t row0, const float* wts,
@@ -117,7 +118,7 @@ struct Model {
 struct ModelDev {
   Model md;
   int NT = 0, nz = 0, nb = 0;
-  DBuf wts, mask, colw, colh, bp, bp0, rbar, ycn, yyM, B64, bpfull, colrec;  // int8 path
+  DBuf wts, mask, rbar, B64, bpfull, colrec, trc;  // int8 path
   DBuf wf32, uyf;                                                     // FP32 path
   EpiI8 epi{};
   EpiF32 epf{};
@@ -139,7 +140,6 @@ struct kg_ctx {
   int64_t cand_cap = 0;
   size_t counters_len = 0;
   kg_timing timing{};
-  int R = 4;          // k-mer tiles per wave (KG_TILES_PER_WAVE)
   int grid_mult = 8;  // persistent workgroups per CU (KG_GRID_MULT)
   // captured graph of the last batched scan
   hipGraph_t graph = nullptr;
@@ -307,30 +307,43 @@ static void quantize(const double* col, int n, int L, std::vector<int8_t>& limbs
 static int build_i8(kg_ctx* ctx, ModelDev& M) {
   Model& md = M.md;
   const int n = md.n, p = md.p, t = md.t, L = ctx->limbs;
+  // 32385 n < 2^31 keeps every limb sum (|sum| <= 127 n) and the epilogue's
+  // int32 arithmetic exact; n is far below this in practice
+  if (n > 60000)
+    return fail(ctx, KG_ERR_ARG, "more than 60000 phenotyped samples for the int8 kernel");
   std::vector<int> zc;
   for (int k = 0; k < p; ++k)
     if (!(k == 0 && md.rotated && md.drop_d)) zc.push_back(k);
   const int nz = (int)zc.size();
-  const int nb = nz + t;
-  // the epilogue reads each lane's structure limbs as whole 16-byte chunks of
-  // 4 columns, so the row must also cover L * (nz rounded up to 4) columns
-  const int NU = std::max(((L * nb + 15) / 16) * 16, ((L * 4 * ((nz + 3) / 4) + 15) / 16) * 16);
-  const int NT = NU / 16;
-  if (NT > 8)
+  const int nb = nz + t;  // base columns: t residuals, then nz structure columns
+  int NT = 1;
+  while (NT <= 8 && i8_column_capacity(NT, L) < nb) ++NT;
+  if (NT > 8 || (L == 1 && NT > 3) || (L == 2 && NT > 6))
     return fail(ctx, KG_ERR_ARG,
-                "weight columns exceed the int8 kernel's 128-column tile budget (p + t too "
-                "large for this limb count)");
+                "p + t exceeds the int8 kernel's weight columns for this limb count (at most " +
+                    std::to_string(i8_column_capacity(L == 3 ? 8 : (L == 2 ? 6 : 3), L)) + ")");
+  const int NU = 16 * NT;
+  const int ncol = i8_column_capacity(NT, L);
   M.NT = NT;
   M.nz = nz;
   M.nb = nb;
+  // base column b -> source column, limb l -> MFMA output row u
+  auto src_col = [&](int b) -> const double* {
+    return (b < t) ? &md.rc[(size_t)b * n] : &md.Uc[(size_t)zc[b - t] * n];
+  };
+  auto urow = [&](int b, int l) {
+    const int sl = L * (b >> 2) + l;
+    return 16 * (sl >> 2) + 4 * (b & 3) + (sl & 3);
+  };
   std::vector<int8_t> lim;
   std::vector<double> colw((size_t)nb * L), colh(nb);
-  std::vector<int8_t> ucol((size_t)n * NU, 0);  // [sample][u-col]
+  std::vector<int8_t> ucol((size_t)n * NU, 0);  // [sample][u-row]
   for (int b = 0; b < nb; ++b) {
-    const double* src = (b < nz) ? &md.Uc[(size_t)zc[b] * n] : &md.rc[(size_t)(b - nz) * n];
-    quantize(src, n, L, lim, &colw[(size_t)b * L], &colh[b]);
-    for (int j = 0; j < n; ++j)
-      for (int l = 0; l < L; ++l) ucol[(size_t)j * NU + L * b + l] = lim[(size_t)j * L + l];
+    quantize(src_col(b), n, L, lim, &colw[(size_t)b * L], &colh[b]);
+    for (int l = 0; l < L; ++l) {
+      const int u = urow(b, l);
+      for (int j = 0; j < n; ++j) ucol[(size_t)j * NU + u] = lim[(size_t)j * L + l];
+    }
   }
   // accession -> sample
   std::vector<int> samp(md.n_acc, -1);
@@ -350,11 +363,6 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
             if (a < md.n_acc && samp[a] >= 0) dst[e] = ucol[(size_t)samp[a] * NU + uc];
           }
         }
-  std::vector<double> bp((size_t)t * (nz > 0 ? nz : 1), 0.0), bp0(t, 0.0);
-  for (int tr = 0; tr < t; ++tr) {
-    for (int b = 0; b < nz; ++b) bp[(size_t)tr * nz + b] = md.bprime[(size_t)tr * p + zc[b]];
-    if (md.rotated && md.drop_d) bp0[tr] = md.bprime[(size_t)tr * p + 0];
-  }
   // FP64 refinement columns: all p rotated-centred columns + t residuals, by accession
   const int nbf = p + t;
   std::vector<double> B((size_t)md.n_acc * nbf, 0.0);
@@ -363,6 +371,35 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
     for (int k = 0; k < p; ++k) B[(size_t)a * nbf + k] = md.Uc[(size_t)k * n + j];
     for (int tr = 0; tr < t; ++tr) B[(size_t)a * nbf + p + tr] = md.rc[(size_t)tr * n + j];
   }
+  // column records {w0, w1, w2, k_0..k_7, 0}; zero past nb
+  std::vector<double> rec((size_t)ncol * kColRec, 0.0);
+  double sum_h2 = 0.0;
+  std::vector<double> sum_hbp(kMaxTraits, 0.0);
+  for (int b = 0; b < nb; ++b) {
+    double* r = &rec[(size_t)b * kColRec];
+    for (int l = 0; l < L; ++l) r[l] = colw[(size_t)b * L + l];
+    if (b < t) {
+      r[3 + b] = 1.0;
+    } else {
+      const int k = zc[b - t];
+      sum_h2 += colh[b] * colh[b];
+      for (int tr = 0; tr < t; ++tr) {
+        const double bpv = md.bprime[(size_t)tr * p + k];
+        r[3 + tr] = bpv;
+        sum_hbp[tr] += colh[b] * std::fabs(bpv);
+      }
+    }
+  }
+  // per-trait records {mean r, |y - ybar|, yyM, b'_0 (dropped d column), h of r_c}
+  std::vector<double> trc((size_t)t * kTrcRec, 0.0);
+  for (int tr = 0; tr < t; ++tr) {
+    double* r = &trc[(size_t)tr * kTrcRec];
+    r[0] = md.rbar[tr];
+    r[1] = md.ycn[tr];
+    r[2] = md.yyM[tr];
+    r[3] = (md.rotated && md.drop_d) ? md.bprime[(size_t)tr * p + 0] : 0.0;
+    r[4] = colh[tr];
+  }
   auto up = [&](DBuf& d, const void* src, size_t bytes) -> hipError_t {
     hipError_t e = d.ensure(bytes);
     if (e != hipSuccess) return e;
@@ -370,50 +407,26 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
   };
   KG_CK(ctx, up(M.wts, W.data(), W.size()));
   KG_CK(ctx, up(M.mask, md.maskw.data(), md.maskw.size() * 8));
-  KG_CK(ctx, up(M.colw, colw.data(), colw.size() * 8));
-  KG_CK(ctx, up(M.colh, colh.data(), colh.size() * 8));
-  KG_CK(ctx, up(M.bp, bp.data(), bp.size() * 8));
-  KG_CK(ctx, up(M.bp0, bp0.data(), bp0.size() * 8));
   KG_CK(ctx, up(M.rbar, md.rbar.data(), t * 8));
-  KG_CK(ctx, up(M.ycn, md.ycn.data(), t * 8));
-  KG_CK(ctx, up(M.yyM, md.yyM.data(), t * 8));
   KG_CK(ctx, up(M.B64, B.data(), B.size() * 8));
   KG_CK(ctx, up(M.bpfull, md.bprime.data(), std::max<size_t>(md.bprime.size(), 1) * 8));
-  // per-column epilogue records {w0, w1, w2, h, b'_0, 0} and bound sums
-  if (nb > kMaxCols) return fail(ctx, KG_ERR_ARG, "too many weight columns for the int8 kernel");
-  std::vector<double> rec((size_t)nb * kColRec, 0.0);
-  double sum_h2 = 0.0;
-  std::vector<double> sum_hbp(kMaxTraits, 0.0);
-  for (int b = 0; b < nb; ++b) {
-    for (int l = 0; l < L; ++l) rec[(size_t)b * kColRec + l] = colw[(size_t)b * L + l];
-    rec[(size_t)b * kColRec + 3] = colh[b];
-    if (b < nz) {
-      rec[(size_t)b * kColRec + 4] = bp[b];  // trait 0
-      sum_h2 += colh[b] * colh[b];
-      for (int tr = 0; tr < t; ++tr) sum_hbp[tr] += colh[b] * std::fabs(bp[(size_t)tr * nz + b]);
-    }
-  }
   KG_CK(ctx, up(M.colrec, rec.data(), rec.size() * 8));
+  KG_CK(ctx, up(M.trc, trc.data(), trc.size() * 8));
   EpiI8& ep = M.epi;
-  ep.colrec = M.colrec.as<double>();
-  ep.sum_h2 = sum_h2 * (1.0 + 1e-12);
-  for (int tr = 0; tr < kMaxTraits; ++tr) ep.sum_hbp[tr] = sum_hbp[tr] * (1.0 + 1e-12);
   ep.n = n;
   ep.nz = nz;
   ep.t = t;
   ep.L = L;
-  ep.NU = NU;
+  ep.ncol = ncol;
   ep.has_d = (md.rotated && !md.drop_d) ? 1 : 0;
   ep.sigma = md.sigma;
   ep.base_coef = md.base_coef;
   ep.dmax_drop = (md.rotated && md.drop_d) ? md.dmax : 0.0;
-  ep.colw = M.colw.as<double>();
-  ep.colh = M.colh.as<double>();
-  ep.bp = M.bp.as<double>();
-  ep.bp0 = M.bp0.as<double>();
-  ep.rbar = M.rbar.as<double>();
-  ep.ycn = M.ycn.as<double>();
-  ep.yyM = M.yyM.as<double>();
+  ep.h_d = ep.has_d ? colh[t] : 0.0;  // structure column 0 is d when it is kept
+  ep.colrec = M.colrec.as<double>();
+  ep.trc = M.trc.as<double>();
+  ep.sum_h2 = sum_h2 * (1.0 + 1e-12);
+  for (int tr = 0; tr < kMaxTraits; ++tr) ep.sum_hbp[tr] = sum_hbp[tr] * (1.0 + 1e-12);
   return KG_OK;
 }
 
@@ -521,10 +534,6 @@ extern "C" int kg_create(kg_ctx** out, const kg_opts* opts) {
   KG_CK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   for (auto& e : ctx->ev) KG_CK(ctx, hipEventCreate(&e));
   KG_CK(ctx, set_kernel_attributes());
-  if (const char* v = std::getenv("KG_TILES_PER_WAVE")) {
-    const int r = atoi(v);
-    if (r == 2 || r == 4) ctx->R = r;
-  }
   if (const char* v = std::getenv("KG_GRID_MULT")) {
     const int g = atoi(v);
     if (g >= 1 && g <= 64) ctx->grid_mult = g;
@@ -670,19 +679,19 @@ static int check_shard(kg_ctx* ctx, const HShard& s) {
 }
 
 // Device part of a batched int8 scan: zero counters, bulk kernel(s), refine.
-static int enqueue_i8(kg_ctx* ctx, int nsh, int64_t total_blocks, bool per_launch,
+static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks, bool per_launch,
                       const std::vector<int64_t>& nblk, const ScanOut& o, int ref_blocks) {
-  const int NT = ctx->models[0]->NT;
+  const int L = ctx->limbs;
   const int grid = ctx->cus * ctx->grid_mult;
   KG_CK(ctx, launch_zero(ctx->stream, o.counters, (int)ctx->counters_len));
   KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (!per_launch) {
-    KG_CK(ctx, launch_scan_i8(NT, ctx->R, grid, ctx->stream, ctx->sdesc.as<ShardDesc>(), nsh,
+    KG_CK(ctx, launch_scan_i8(NT, L, t, grid, ctx->stream, ctx->sdesc.as<ShardDesc>(), nsh,
                               total_blocks, o));
   } else {
     for (int s = 0; s < nsh; ++s) {
       if (nblk[s] == 0) continue;
-      KG_CK(ctx, launch_scan_i8(NT, ctx->R, grid, ctx->stream, ctx->sdesc1.as<ShardDesc>() + s, 1,
+      KG_CK(ctx, launch_scan_i8(NT, L, t, grid, ctx->stream, ctx->sdesc1.as<ShardDesc>() + s, 1,
                                 nblk[s], o));
     }
   }
@@ -710,7 +719,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   }
   const double p_thr = std::pow(10.0, -prm.log10_thresh);
   const double Ts = t_star(p_thr);
-  const int64_t bk = i8_block_kmers(ctx->R);
+  const int64_t bk = i8_block_kmers(NT, t);
   std::vector<ShardDesc> sd(nsh), sd1(nsh);
   std::vector<RefDesc> rd(nsh);
   std::vector<int64_t> nblk(nsh);
@@ -801,7 +810,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       kp += sizeof(RefDesc) * nsh;
       memcpy(kp, &o, sizeof(o));
       kp += sizeof(o);
-      const int64_t extra[2] = {(int64_t)flags, (int64_t)ctx->R};
+      const int64_t extra[2] = {(int64_t)flags, (int64_t)ctx->grid_mult};
       memcpy(kp, extra, 16);
     }
     const bool replay = use_graph && ctx->gexec && key == ctx->graph_key;
@@ -816,7 +825,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       if (!replay) {
         drop_graph(ctx);
         KG_CK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_i8(ctx, nsh, total_blocks, per_launch, nblk, o, ref_blocks);
+        int rc = enqueue_i8(ctx, NT, t, nsh, total_blocks, per_launch, nblk, o, ref_blocks);
         hipGraph_t g = nullptr;
         hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
         if (rc != KG_OK) {
@@ -831,7 +840,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       KG_CK(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
       KG_CK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     } else {
-      int rc = enqueue_i8(ctx, nsh, total_blocks, per_launch, nblk, o, ref_blocks);
+      int rc = enqueue_i8(ctx, NT, t, nsh, total_blocks, per_launch, nblk, o, ref_blocks);
       if (rc != KG_OK) return rc;
     }
     KG_CK(ctx, hipMemcpyAsync(cnt.data(), dc, sizeof(unsigned long long) * cnt.size(),

@@ -36,12 +36,18 @@ struct ScanOut {
 };
 
 // Epilogue constants of the int8-limb kernel (device pointers for arrays).
+//
+// Weight columns ("base columns") b = 0..nb-1 are ordered residuals first
+// (b < t: r_c of trait b), then the projected structure columns (b = t + k:
+// z_k).  Column b, limb l sits in MFMA output row
+//   u = 16 * (s / 4) + 4 * (b % 4) + s % 4,   s = L * (b / 4) + l,
+// so every limb of a column lands in one lane of the 16x16 output tile.
 struct EpiI8 {
   int n;            // phenotyped samples
   int nz;           // projected structure columns kept (z columns)
-  int t;            // traits (r columns follow the z columns)
+  int t;            // traits
   int L;            // limbs per column
-  int NU;           // weight columns in the MFMA tiles (multiple of 16)
+  int ncol;         // column records (4 per lane slot; zero past nb)
   int tau_trivial;  // corr threshold <= 0: every polymorphic k-mer passes
   int all_sig;      // p threshold >= 1: every pre-filter pass is reported
   int has_d;        // z column 0 is the kept intercept-remainder column d
@@ -50,19 +56,18 @@ struct EpiI8 {
   double dmax_drop; // max|d| of the dropped intercept-remainder column (0 if kept)
   double tau;
   double r2star;    // 1D threshold in r^2
-  const double* colw;   // [nb][L] limb weights
-  const double* colh;   // [nb] per-set-bit quantisation bound
-  const double* bp;     // [t][nz] rotated coefficients b' = Q'U'y (S = s + z'b')
-  const double* bp0;    // [t] b'_0 of the dropped column (bound only)
-  const double* rbar;   // [t] mean of r = y - U U'y
-  const double* ycn;    // [t] |y - mean y|
-  const double* yyM;    // [t]
-  const double* colrec; // [nb][kColRec] {w0, w1, w2, h, b'_0 (trait 0), 0}: staged in LDS
-  double sum_h2;        // sum over z columns of h^2   (|z|^2 bound: 2c sum|z|h + c^2 sum h^2)
+  double h_d;       // quantisation bound per set bit of column d (if kept)
+  const double* colrec; // [ncol][kColRec] {w0, w1, w2, k_0..k_7, 0}: staged in LDS
+  const double* trc;    // [t][kTrcRec] {mean r, |y - ybar|, yyM, b'_0, h of r_c}
+  double sum_h2;        // sum over z columns of h^2   (|z|^2 bound: 2c |z| |h| + c^2 |h|^2)
   double sum_hbp[8];    // per trait: sum over z columns of h |b'|
 };
-constexpr int kColRec = 6;        // doubles per column record
-constexpr int kMaxCols = 48;      // base columns the int8 kernel supports (NT <= 8, L = 3)
+// Column record: limb weights w_l, then per trait k_tr = b'_tr of a structure
+// column, 1 for the trait's own residual column and 0 otherwise, so that
+// sum_b z_b k_tr = z'b' + x'r_c = S.
+constexpr int kColRec = 12;
+constexpr int kTrcRec = 8;
+constexpr int kMaxCols = 48;      // column records the int8 kernel stages
 
 // One shard of a batched int8 scan: its rows and its null model.  The
 // persistent kernel walks the concatenation of all shards' 256-k-mer blocks.

@@ -30,6 +30,18 @@ constexpr int kWaves = 4;        // waves per workgroup
 constexpr int kThreads = 64 * kWaves;
 constexpr int kI8Steps = 8;      // 64-sample MFMA steps per 512-sample chunk
 
+// Bare workgroup barrier for the stage loop.  Every LDS value a wave read in
+// the stage has been consumed (so those reads are complete) and each wave has
+// waited for its own direct-to-LDS loads with an explicit vmcnt; a fence here
+// would make the compiler wait for every load in flight, including the bits
+// prefetch meant to overlap the next stage.  The empty asm statements keep
+// the compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // 4 bits -> 4 bytes of 0/1: bit b of nib lands in byte b.
 __device__ __forceinline__ int spread4(uint32_t nib) {
   return (int)(__umul24(nib, 0x00204081u) & 0x01010101u);
@@ -62,9 +74,12 @@ __device__ __forceinline__ void append_cand(bool want, const Cand& rec, const Sc
 // Pre-filter + 1D threshold decision of the int8 path, with error bounds.
 // Returns candidate flags (0 = decided "not reported and not counted").
 // *pf_def is set when the pre-filter is decided "pass" (counted on device).
-__device__ uint32_t classify_i8(int c, double xxM, double bxx, double S, double bS,
-                                double xyM, double bxy, const EpiI8& ep, double ycn,
-                                double yy, bool* pf_def) {
+// Both tests compare squares (no division or square root); the relative
+// margins 3e-12 and 1e-9 dwarf the rounding of the squared forms.
+__device__ __forceinline__ uint32_t classify_i8(int c, double xxM, double bxx, double S,
+                                                double bS, double xyM, double bxy,
+                                                const EpiI8& ep, double ycn, double yy,
+                                                bool* pf_def) {
   *pf_def = false;
   const int n = ep.n;
   if (c <= 0 || c >= n) return 0;               // |x - mean x| = 0 (src/kgwas.cpp:465)
@@ -73,11 +88,12 @@ __device__ uint32_t classify_i8(int c, double xxM, double bxx, double S, double 
   if (ep.tau_trivial) {
     f = kPfPass;
   } else {
-    const double v = (double)c * (double)(n - c) / (double)n;
-    const double thr = ep.tau * sqrt(v) * ycn;  // |cor| >= tau  <=>  |S| >= thr
+    // |cor| >= tau  <=>  S^2 >= tau^2 (c (n - c) / n) |y - ybar|^2
+    const double thr2 = ep.tau * ep.tau * ((double)c * (double)(n - c) / (double)n) * ycn * ycn;
     const double aS = fabs(S);
-    if (aS - bS >= thr * (1.0 + 1e-12)) f = kPfPass;
-    else if (aS + bS < thr * (1.0 - 1e-12)) return 0;
+    const double lo = aS - bS, hi = aS + bS;
+    if (lo > 0.0 && lo * lo >= thr2 * (1.0 + 3e-12)) f = kPfPass;
+    else if (hi * hi < thr2 * (1.0 - 3e-12)) return 0;
     else f = kPfAmb;
   }
   *pf_def = (f == kPfPass);
@@ -91,13 +107,12 @@ __device__ uint32_t classify_i8(int c, double xxM, double bxx, double S, double 
   } else if (xxM - bxx <= 1e-12) {
     s = kSigAmb;
   } else {
+    // r^2 = xyM^2 / (xxM yyM) against r2star, cross-multiplied
     const double axy = fabs(xyM);
     const double hi = axy + bxy;
     const double lo = fmax(axy - bxy, 0.0);
-    const double r2hi = hi * hi / ((xxM - bxx) * yy);
-    const double r2lo = lo * lo / ((xxM + bxx) * yy);
-    if (r2lo >= ep.r2star * (1.0 + 1e-9)) s = kSigDef;
-    else if (r2hi < ep.r2star * (1.0 - 1e-9)) s = 0;
+    if (lo * lo >= ep.r2star * (1.0 + 1e-9) * ((xxM + bxx) * yy)) s = kSigDef;
+    else if (hi * hi < ep.r2star * (1.0 - 1e-9) * ((xxM - bxx) * yy)) s = 0;
     else s = kSigAmb;
   }
   if (s == 0 && f != kPfAmb) return 0;
@@ -159,26 +174,37 @@ __device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wa
   }
 }
 
-template <int NT, int R>
+// Sum over the four lane groups (lanes l, l^16, l^32, l^48).
+__device__ __forceinline__ double group_sum(double v) {
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+
+// Tiles per wave of an instantiation: 4 while the accumulators leave room,
+// 2 for wide tiles or the multi-trait epilogue.
+__host__ __device__ constexpr int i8_tiles(int NT, int TT) { return (TT == 1 && NT <= 6) ? 4 : 2; }
+
+// L = limbs per column; TT = traits the epilogue carries (1, or kMaxTraits).
+template <int NT, int L, int TT>
 __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
                                                         int nshards, int64_t total_blocks,
                                                         ScanOut out) {
-  constexpr int NU = 16 * NT;
+  constexpr int R = i8_tiles(NT, TT);
   constexpr int FRAG2 = kStageSteps * NT * 64;       // v4i per 256-sample stage
-  constexpr int SWZ = (NT % 2 == 0) ? 7 : 3;         // epilogue row swizzle (16-byte chunks)
+  constexpr int J = (4 * NT) / L;                    // base columns per lane group
+  constexpr int JSP = (TT + 4) / 4;                  // slots that may hold r_c or d columns
+  static_assert(4 * J <= kMaxCols, "column records exceed the LDS table");
   // Separate LDS objects, so that reads of one buffer do not wait for the
   // direct-to-LDS loads still filling another.
   __shared__ v4i buf0[FRAG2];
   __shared__ v4i buf1[FRAG2];
   __shared__ v4i bitsbuf[kWaves * (R + 1) * 64];     // [wave][r | mask][lane] next chunk
   __shared__ double colrec_lds[kMaxCols * kColRec];
-  __shared__ double bp_lds[kMaxTraits * kMaxCols];     // b' of traits 1.. (trait 0 is in colrec)
-  __shared__ double trc_lds[kMaxTraits * 4];           // per trait: mean r, |y - ybar|, yyM, b'_0
-  int* const sc_all = reinterpret_cast<int*>(buf1);  // epilogue rows alias the spent stage
+  __shared__ double trc_lds[kMaxTraits * kTrcRec];
   __shared__ unsigned int wg_pass[kMaxTraits];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, col = lane & 15;
-  const int kk = lane >> 2, q = lane & 3;
   v4i* const mybits = bitsbuf + wave * (R + 1) * 64;
   if (threadIdx.x < kMaxTraits) wg_pass[threadIdx.x] = 0;
 
@@ -258,24 +284,21 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
             acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bf[r], acc[r][t], 0, 0, 0);
         }
       }
-      __builtin_amdgcn_s_waitcnt(0);  // next stage's weights (and next chunk's bits) landed
-      __syncthreads();
+      // Next stage's weights landed.  Loads complete in issue order, so after
+      // an even stage the next chunk's bits (issued last) may stay in flight
+      // for one more stage: vmcnt(R + 1).
+      if (h == 0 && C + 1 < nchunks) __builtin_amdgcn_s_waitcnt(0x0F70 | (R + 1));
+      else __builtin_amdgcn_s_waitcnt(0);
+      lds_barrier();
      }
     }
 
     // this shard's epilogue constants into LDS while no direct-to-LDS load is
     // in flight (the epilogue then reads only LDS)
     {
-      const int nbc0 = ep.nz + ep.t;
       const double* __restrict__ gcr = ep.colrec;
-      for (int i = threadIdx.x; i < nbc0 * kColRec; i += kThreads) colrec_lds[i] = gcr[i];
-      for (int i = threadIdx.x; i < ep.t * ep.nz; i += kThreads) bp_lds[i] = ep.bp[i];
-      if (threadIdx.x < ep.t) {
-        trc_lds[4 * threadIdx.x + 0] = ep.rbar[threadIdx.x];
-        trc_lds[4 * threadIdx.x + 1] = ep.ycn[threadIdx.x];
-        trc_lds[4 * threadIdx.x + 2] = ep.yyM[threadIdx.x];
-        trc_lds[4 * threadIdx.x + 3] = ep.bp0[threadIdx.x];
-      }
+      for (int i = threadIdx.x; i < ep.ncol * kColRec; i += kThreads) colrec_lds[i] = gcr[i];
+      for (int i = threadIdx.x; i < ep.t * kTrcRec; i += kThreads) trc_lds[i] = ep.trc[i];
       __builtin_amdgcn_s_waitcnt(0);
     }
 
@@ -291,130 +314,109 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
       dma_mask(sd[sn].mask, 0, g, mybits + R * 64);
       dma_stage<NT>(reinterpret_cast<const v4i*>(sd[sn].wts), buf0, wave, lane);
     }
+    lds_barrier();  // column records staged by every wave
 
-    // ---- epilogue: per tile, spill D to LDS as [k-mer][weight column] ----
-    // Lane (kk, q) evaluates k-mer kk over the structure columns
-    // [q*cpl, (q+1)*cpl), cpl a multiple of 4 so a lane's limbs are whole
-    // 16-byte chunks of the row (rows XOR-swizzled by k-mer in 16-byte chunks).
-    const int nz = ep.nz, L = ep.L;
-    const int cpl = 4 * ((nz + 15) / 16);
-    int* sc = sc_all + wave * 16 * NU;
-    const double inv_n = 1.0 / (double)ep.n;
+    // ---- epilogue, from registers ----
+    // Lane (g, col) holds, for k-mer col of tile r, output rows
+    // u = 16 t + 4 g + i: the L limbs of base columns b = 4 j + g, j < J.  Each
+    // lane folds its columns into partial sums, the four lane groups are
+    // summed, and lane (g, col) keeps tile g % R: afterwards every lane of
+    // the wave owns one k-mer (R = 4) and decides it alone.
+    const int t = ep.t, n = ep.n;
+    const int rme = g % R;
+    const bool primary = g < R;
+    int c = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       uint32_t cc = cnt[r];
       cc += __shfl_xor(cc, 16);
       cc += __shfl_xor(cc, 32);
+      if (r == rme) c = (int)cc;
+    }
+    double kzz = 0.0, kz0 = 0.0, kS[TT], ksv[TT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
-        *reinterpret_cast<v4i*>(sc + col * NU + 4 * ((4 * t + g) ^ (col & SWZ))) = acc[r][t];
-      __syncthreads();
-      const int c = (int)__shfl(cc, kk);
-      const int64_t kidx = kbase + 16 * r + kk;
-      const bool valid = (kidx < D.m);
-      const int* drow = sc + kk * NU;
-      const int ksw = kk & SWZ;
-      // column uc of this k-mer's row (16-byte chunks XOR-swizzled by row)
-#define KG_D(uc) drow[4 * (((uc) >> 2) ^ ksw) + ((uc) & 3)]
-      const double cd = (double)c, cn = cd * inv_n;
+    for (int tr = 0; tr < TT; ++tr) kS[tr] = ksv[tr] = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double zz = 0.0, z0 = 0.0, S[TT], sv[TT];
+#pragma unroll
+      for (int tr = 0; tr < TT; ++tr) S[tr] = sv[tr] = 0.0;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const double* cr = colrec_lds + (4 * j + g) * kColRec;
+        double z = cr[0] * (double)acc[r][(L * j) >> 2][(L * j) & 3];
+        if (L > 1) z = fma(cr[1], (double)acc[r][(L * j + 1) >> 2][(L * j + 1) & 3], z);
+        if (L > 2) z = fma(cr[2], (double)acc[r][(L * j + 2) >> 2][(L * j + 2) & 3], z);
+        if (j < JSP) {  // may be a residual column (b < t) or column d (b == t)
+          const int b = 4 * j + g;
+          const double zs = (b < t) ? 0.0 : z;
+          zz = fma(zs, zs, zz);
+          if (b == t) z0 = z;
+#pragma unroll
+          for (int tr = 0; tr < TT; ++tr) sv[tr] += (b == tr) ? z : 0.0;
+        } else {
+          zz = fma(z, z, zz);
+        }
+#pragma unroll
+        for (int tr = 0; tr < TT; ++tr) S[tr] = fma(z, cr[3 + tr], S[tr]);
+      }
+      zz = group_sum(zz);
+      z0 = group_sum(z0);
+#pragma unroll
+      for (int tr = 0; tr < TT; ++tr) {
+        S[tr] = group_sum(S[tr]);
+        sv[tr] = group_sum(sv[tr]);
+      }
+      if (r == rme) {
+        kzz = zz;
+        kz0 = z0;
+#pragma unroll
+        for (int tr = 0; tr < TT; ++tr) {
+          kS[tr] = S[tr];
+          ksv[tr] = sv[tr];
+        }
+      }
+    }
 
-      // structure columns of this lane: |z|^2, sum |z| h (bound), trait 0's
-      // z'b', and z_0 (intercept-remainder column)
-      double zz = 0.0, za = 0.0, S0 = 0.0, z0 = 0.0;
-      const int b0 = q * cpl;
-      const int b1 = min(b0 + cpl, nz);
-      for (int bb = b0; bb < b1; bb += 4) {
-        // 4 columns = L*4 consecutive limbs = L 16-byte chunks
-        int lim[12];
+    const int64_t kidx = kbase + 16 * rme + col;
+    const bool valid = primary && (kidx < D.m);
+    const double cd = (double)c, cn = cd / (double)n;
+    // |dz_k| <= c h_k  =>  |d |z|^2| <= 2c sum|z_k| h_k + c^2 sum h_k^2
+    //                              <= 2c |z| |h| + c^2 |h|^2
+    const double bzz = 2.0 * cd * sqrt(kzz * ep.sum_h2) + cd * cd * ep.sum_h2;
+    double cross = 0.0, bcross = 0.0;
+    if (ep.has_d) {
+      cross = 2.0 * cn * ep.sigma * kz0;
+      bcross = 2.0 * cn * fabs(ep.sigma) * cd * ep.h_d;
+    }
+    const double xxM = cd - cd * cd * ep.base_coef - cross - kzz;
+    double bxx = bzz + bcross + 1e-11 * (cd + kzz + fabs(cross) + 1.0);
+    if (ep.dmax_drop > 0.0) {
+      const double e0 = cd * ep.dmax_drop;
+      bxx += 2.0 * cn * fabs(ep.sigma) * e0 + e0 * e0;
+    }
+    if (out.st_c != nullptr && valid) {
+      out.st_c[kidx] = c;
+      out.st_xx[kidx] = xxM;
+    }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          if (k < L) {
-            const int ch = ((L * bb) >> 2) + k;
-            const v4i v = *reinterpret_cast<const v4i*>(drow + 4 * (ch ^ ksw));
-            lim[4 * k + 0] = v.x;
-            lim[4 * k + 1] = v.y;
-            lim[4 * k + 2] = v.z;
-            lim[4 * k + 3] = v.w;
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int b = bb + u;
-          if (b < b1) {
-            const double* cr = colrec_lds + b * kColRec;
-            double z;
-            if (L == 3) {  // limbs 1, 2 merged exactly in int32: a1*254 + a2
-              const int a12 = lim[3 * u + 1] * 254 + lim[3 * u + 2];
-              z = cr[0] * (double)lim[3 * u] + cr[2] * (double)a12;
-            } else if (L == 2) {
-              z = cr[0] * (double)lim[2 * u] + cr[1] * (double)lim[2 * u + 1];
-            } else {
-              z = cr[0] * (double)lim[u];
-            }
-            zz += z * z;
-            za += fabs(z) * cr[3];
-            S0 += z * cr[4];
-            if (b == 0) z0 = z;
-          }
-        }
-      }
-      zz = wave_sum4(zz);
-      za = wave_sum4(za);
-      S0 = wave_sum4(S0);
-      z0 = wave_sum4(z0);
-      // |dz_k| <= c h_k  =>  |d |z|^2| <= 2c sum|z_k|h_k + c^2 sum h_k^2
-      const double bzz = 2.0 * cd * za + cd * cd * ep.sum_h2;
-      double cross = 0.0, bcross = 0.0;
-      if (ep.has_d) {
-        cross = 2.0 * cn * ep.sigma * z0;
-        bcross = 2.0 * cn * fabs(ep.sigma) * cd * colrec_lds[3];
-      }
-      const double xxM = cd - cd * cd * ep.base_coef - cross - zz;
-      double bxx = bzz + bcross + 1e-11 * (cd + zz + fabs(cross) + 1.0);
-      if (ep.dmax_drop > 0.0) {
-        const double e0 = cd * ep.dmax_drop;
-        bxx += 2.0 * cn * fabs(ep.sigma) * e0 + e0 * e0;
-      }
-
-      for (int tr = 0; tr < ep.t; ++tr) {
-        double S = S0;
-        if (tr > 0) {  // further traits: z'b' again (rare path)
-          S = 0.0;
-          for (int b = b0; b < b1; ++b) {
-            const double* cr = colrec_lds + b * kColRec;
-            double z = cr[0] * (double)KG_D(L * b);
-            if (L > 1) z += cr[1] * (double)KG_D(L * b + 1);
-            if (L > 2) z += cr[2] * (double)KG_D(L * b + 2);
-            S += z * bp_lds[tr * nz + b];
-          }
-          S = wave_sum4(S);
-        }
-        // residual column of this trait (every lane reads it: no reduction)
-        const int br = nz + tr;
-        const double* crr = colrec_lds + br * kColRec;
-        double sv = crr[0] * (double)KG_D(L * br);
-        if (L > 1) sv += crr[1] * (double)KG_D(L * br + 1);
-        if (L > 2) sv += crr[2] * (double)KG_D(L * br + 2);
-        const double es = cd * crr[3];
-        const double rbar = trc_lds[4 * tr + 0];
-        S += sv;
+    for (int tr = 0; tr < TT; ++tr) {
+      if (tr < t) {
+        const double* tc = trc_lds + tr * kTrcRec;
+        const double S = kS[tr], sv = ksv[tr];
+        const double rbar = tc[0];
+        const double es = cd * tc[4];
         double bS = cd * ep.sum_hbp[tr] + es;
-        if (ep.dmax_drop > 0.0) bS += cd * ep.dmax_drop * fabs(trc_lds[4 * tr + 3]);
+        if (ep.dmax_drop > 0.0) bS += cd * ep.dmax_drop * fabs(tc[3]);
         bS += 1e-12 * (fabs(S) + fabs(sv) + 1.0);
         const double xyM = sv + cd * rbar;
         const double bxy = es + 1e-12 * (fabs(sv) + cd * fabs(rbar) + 1.0);
         bool pf_def = false;
         uint32_t fl = 0;
-        if (q == 0 && valid)
-          fl = classify_i8(c, xxM, bxx, S, bS, xyM, bxy, ep, trc_lds[4 * tr + 1],
-                           trc_lds[4 * tr + 2], &pf_def);
-        if (out.st_c != nullptr && q == 0 && valid) {
-          if (tr == 0) {
-            out.st_c[kidx] = c;
-            out.st_xx[kidx] = xxM;
-          }
-          out.st_xy[kidx * ep.t + tr] = xyM;
-          out.st_S[kidx * ep.t + tr] = S;
+        if (valid) fl = classify_i8(c, xxM, bxx, S, bS, xyM, bxy, ep, tc[1], tc[2], &pf_def);
+        if (out.st_c != nullptr && valid) {
+          out.st_xy[kidx * t + tr] = xyM;
+          out.st_S[kidx * t + tr] = S;
         }
         const unsigned long long pb = __ballot(pf_def);
         if (lane == 0 && pb) atomicAdd(&wg_pass[tr], (unsigned int)__popcll(pb));
@@ -428,11 +430,10 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
         rec.pad = 0;
         append_cand(fl != 0, rec, out);
       }
-      __syncthreads();
     }
-#undef KG_D
+    __syncthreads();
     // flush this block's pre-filter passes to its shard
-    if (threadIdx.x < ep.t) {
+    if (threadIdx.x < t) {
       const unsigned int v = wg_pass[threadIdx.x];
       if (v) atomicAdd(&D.pass[threadIdx.x], (unsigned long long)v);
       wg_pass[threadIdx.x] = 0;
@@ -677,28 +678,35 @@ __global__ void synth_bits(uint64_t* __restrict__ out, uint64_t seed, int64_t km
 // ----------------------------------------------------------------------------
 // Launchers (host).  The tile count NT is a template parameter.
 
-template <int NT, int R>
-static hipError_t launch_i8_tr(dim3 grid, size_t shm, hipStream_t s, const ShardDesc* sd,
-                               int nshards, int64_t total_blocks, const ScanOut& o) {
-  auto kfn = scan_i8<NT, R>;
-  hipLaunchKernelGGL(kfn, grid, dim3(kThreads), shm, s, sd, nshards, total_blocks, o);
+template <int NT, int L, int TT>
+static hipError_t launch_i8_tr(dim3 grid, hipStream_t s, const ShardDesc* sd, int nshards,
+                               int64_t total_blocks, const ScanOut& o) {
+  auto kfn = scan_i8<NT, L, TT>;
+  hipLaunchKernelGGL(kfn, grid, dim3(kThreads), 0, s, sd, nshards, total_blocks, o);
   return hipGetLastError();
 }
 
-// k-mers per workgroup block for tiles-per-wave R
-int64_t i8_block_kmers(int R) { return (int64_t)kWaves * 16 * R; }
+// k-mers per workgroup block of the instantiation for (NT, t)
+int64_t i8_block_kmers(int NT, int t) { return (int64_t)kWaves * 16 * i8_tiles(NT, t > 1 ? kMaxTraits : 1); }
 
-hipError_t launch_scan_i8(int NT, int R, int grid, hipStream_t s, const ShardDesc* sd,
+// Column capacity of NT tiles at L limbs: 4 lane groups x floor(4 NT / L).
+int i8_column_capacity(int NT, int L) { return 4 * ((4 * NT) / L); }
+
+// NT ranges per limb count cover up to kMaxCols base columns.
+hipError_t launch_scan_i8(int NT, int L, int t, int grid, hipStream_t s, const ShardDesc* sd,
                           int nshards, int64_t total_blocks, const ScanOut& o) {
-  const size_t shm = 0;  // all LDS of scan_i8 is static
   dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
-  switch (NT * 8 + R) {
-#define KG_CASE(X)                                                                  \
-  case X * 8 + 4:                                                                   \
-    return launch_i8_tr<X, 4>(g, shm, s, sd, nshards, total_blocks, o);             \
-  case X * 8 + 2:                                                                   \
-    return launch_i8_tr<X, 2>(g, shm, s, sd, nshards, total_blocks, o);
-    KG_CASE(1) KG_CASE(2) KG_CASE(3) KG_CASE(4) KG_CASE(5) KG_CASE(6) KG_CASE(7) KG_CASE(8)
+  const int TT = t > 1 ? kMaxTraits : 1;
+  switch ((NT * 4 + L) * 16 + TT) {
+#define KG_CASE(X, Y)                                                                   \
+  case (X * 4 + Y) * 16 + 1:                                                            \
+    return launch_i8_tr<X, Y, 1>(g, s, sd, nshards, total_blocks, o);                   \
+  case (X * 4 + Y) * 16 + kMaxTraits:                                                   \
+    return launch_i8_tr<X, Y, kMaxTraits>(g, s, sd, nshards, total_blocks, o);
+    KG_CASE(1, 1) KG_CASE(2, 1) KG_CASE(3, 1)
+    KG_CASE(1, 2) KG_CASE(2, 2) KG_CASE(3, 2) KG_CASE(4, 2) KG_CASE(5, 2) KG_CASE(6, 2)
+    KG_CASE(1, 3) KG_CASE(2, 3) KG_CASE(3, 3) KG_CASE(4, 3) KG_CASE(5, 3) KG_CASE(6, 3)
+    KG_CASE(7, 3) KG_CASE(8, 3)
 #undef KG_CASE
     default:
       return hipErrorInvalidValue;
