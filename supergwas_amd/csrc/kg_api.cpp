@@ -140,7 +140,7 @@ struct kg_ctx {
   int64_t cand_cap = 0;
   size_t counters_len = 0;
   kg_timing timing{};
-  int grid_mult = 8;  // persistent workgroups per CU (KG_GRID_MULT)
+  int grid_mult = 1;  // persistent workgroups per CU (KG_GRID_MULT)
   // captured graph of the last batched scan
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -359,7 +359,9 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
           const int uc = 16 * tt + i;
           int8_t* dst = &W[(size_t)C * frag + (((size_t)jj * NT + tt) * 64 + lane) * 16];
           for (int e = 0; e < 16; ++e) {
-            const int a = 512 * C + 128 * g + 16 * jj + e;
+            // operand byte e of step jj: bit 8 (e % 4) + 4 (jj % 2) + e / 4 of
+            // the lane group's word jj / 2 of the chunk (kg_kernels.hip)
+            const int a = 512 * C + 128 * g + 32 * (jj >> 1) + 8 * (e & 3) + 4 * (jj & 1) + (e >> 2);
             if (a < md.n_acc && samp[a] >= 0) dst[e] = ucol[(size_t)samp[a] * NU + uc];
           }
         }
@@ -423,6 +425,7 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
   ep.base_coef = md.base_coef;
   ep.dmax_drop = (md.rotated && md.drop_d) ? md.dmax : 0.0;
   ep.h_d = ep.has_d ? colh[t] : 0.0;  // structure column 0 is d when it is kept
+  ep.inv_n = 1.0 / (double)n;
   ep.colrec = M.colrec.as<double>();
   ep.trc = M.trc.as<double>();
   ep.sum_h2 = sum_h2 * (1.0 + 1e-12);

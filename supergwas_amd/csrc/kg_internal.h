@@ -57,6 +57,7 @@ struct EpiI8 {
   double tau;
   double r2star;    // 1D threshold in r^2
   double h_d;       // quantisation bound per set bit of column d (if kept)
+  double inv_n;     // 1 / n
   const double* colrec; // [ncol][kColRec] {w0, w1, w2, k_0..k_7, 0}: staged in LDS
   const double* trc;    // [t][kTrcRec] {mean r, |y - ybar|, yyM, b'_0, h of r_c}
   double sum_h2;        // sum over z columns of h^2   (|z|^2 bound: 2c |z| |h| + c^2 |h|^2)

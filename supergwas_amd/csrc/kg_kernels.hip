@@ -10,8 +10,9 @@
 //
 // Sample (K) order.  Both bulk kernels contract over accessions in an order
 // chosen so that every lane streams contiguous bits:
-//   scan_i8 : accession 512*C + 128*g + 16*j + e  <->  chunk C, lane group
-//             g = lane/16, MFMA step j (0..7), operand byte e (0..15)
+//   scan_i8 : accession 512*C + 128*g + 32*(j/2) + 8*(e%4) + 4*(j%2) + e/4
+//             <->  chunk C, lane group g = lane/16, MFMA step j (0..7),
+//             operand byte e (0..15)
 //   scan_f32: accession 256*C + 64*g + e          <->  chunk C, lane group g,
 //             MFMA step e (0..63)
 // The host lays the weights out in the same order (kg_api.cpp), so the
@@ -26,8 +27,12 @@ namespace kg {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int kWaves = 4;        // waves per workgroup
+constexpr int kWaves = 4;        // waves per workgroup (FP32 scan, refinement)
 constexpr int kThreads = 64 * kWaves;
+// The int8 scan runs one 8-wave workgroup per CU (two waves per SIMD): two
+// 4-wave workgroups per CU measured ~40% slower on the MFMA pipe.
+constexpr int kI8Waves = 8;
+constexpr int kI8Threads = 64 * kI8Waves;
 constexpr int kI8Steps = 8;      // 64-sample MFMA steps per 512-sample chunk
 
 // Bare workgroup barrier for the stage loop.  Every LDS value a wave read in
@@ -42,9 +47,28 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
-// 4 bits -> 4 bytes of 0/1: bit b of nib lands in byte b.
-__device__ __forceinline__ int spread4(uint32_t nib) {
-  return (int)(__umul24(nib, 0x00204081u) & 0x01010101u);
+template <int R>
+__device__ __forceinline__ void unpack_tile(int jj, int r, const uint32_t (&lo)[R],
+                                            const uint32_t (&hi)[R], v4i (&b)[R]) {
+  const int sb = 4 * (jj & 1);
+  const uint32_t w = (jj < 2) ? lo[r] : hi[r];
+  b[r] = v4i{(int)((w >> sb) & 0x01010101u), (int)((w >> (sb + 1)) & 0x01010101u),
+             (int)((w >> (sb + 2)) & 0x01010101u), (int)((w >> (sb + 3)) & 0x01010101u)};
+}
+
+// B operand of MFMA step jj (0..3) of a 256-sample stage: operand byte e =
+// 4 d + q of a lane takes bit 8 q + 4 (jj & 1) + d of its stage word
+// (jj < 2 ? lo : hi), so each dword is one shift and one mask.
+template <int R>
+__device__ __forceinline__ void unpack_step(int jj, const uint32_t (&lo)[R],
+                                            const uint32_t (&hi)[R], v4i (&b)[R]) {
+  const int sb = 4 * (jj & 1);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t w = (jj < 2) ? lo[r] : hi[r];
+    b[r] = v4i{(int)((w >> sb) & 0x01010101u), (int)((w >> (sb + 1)) & 0x01010101u),
+               (int)((w >> (sb + 2)) & 0x01010101u), (int)((w >> (sb + 3)) & 0x01010101u)};
+  }
 }
 
 __device__ __forceinline__ double wave_sum4(double v) {  // over lanes l^1, l^2
@@ -136,12 +160,14 @@ constexpr int kStageSteps = 4;
 template <int NT>
 __device__ __forceinline__ void dma_stage(const v4i* __restrict__ src, v4i* dst, int wave,
                                           int lane) {
+  // the stage's 4 NT blocks of 64 v4i, dealt round-robin to the waves
 #pragma unroll
-  for (int i = 0; i < NT; ++i) {
-    const int qd = i * kWaves + wave;  // 64-v4i block of the stage handled by this wave
-    __builtin_amdgcn_global_load_lds((const void*)(src + qd * 64 + lane),
-                                     (__attribute__((address_space(3))) void*)(dst + qd * 64),
-                                     16, 0, 0);
+  for (int i = 0; i < (4 * NT + kI8Waves - 1) / kI8Waves; ++i) {
+    const int qd = i * kI8Waves + wave;
+    if (qd < 4 * NT)
+      __builtin_amdgcn_global_load_lds((const void*)(src + qd * 64 + lane),
+                                       (__attribute__((address_space(3))) void*)(dst + qd * 64),
+                                       16, 0, 0);
   }
 }
 
@@ -165,7 +191,7 @@ __device__ __forceinline__ void dma_mask(const void* mask, int C, int g, v4i* ds
 template <int R>
 __device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wave, int g, int col,
                                            const uint64_t* (&rowp)[R]) {
-  const int64_t kbase = ((w - D.blk0) * kWaves + wave) * (16 * R);
+  const int64_t kbase = ((w - D.blk0) * kI8Waves + wave) * (16 * R);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     int64_t k = kbase + 16 * r + col;
@@ -174,11 +200,48 @@ __device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wa
   }
 }
 
-// Sum over the four lane groups (lanes l, l^16, l^32, l^48).
-__device__ __forceinline__ double group_sum(double v) {
-  v += __shfl_xor(v, 16);
-  v += __shfl_xor(v, 32);
-  return v;
+// Lane-swap sums.  swap16(x, y): rows of 16 lanes pair up (0,1), (2,3); the
+// even row gets x summed over the pair, the odd row y summed over the pair.
+// swap32(x, y): halves of 32 lanes pair up; the low half gets x summed over
+// both halves, the high half y.  (v_permlane16_swap / v_permlane32_swap.)
+__device__ __forceinline__ uint32_t swap16(uint32_t x, uint32_t y) {
+  const auto v = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+  return v[0] + v[1];
+}
+__device__ __forceinline__ uint32_t swap32(uint32_t x, uint32_t y) {
+  const auto v = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+  return v[0] + v[1];
+}
+__device__ __forceinline__ double swap16(double x, double y) {
+  const uint64_t xb = (uint64_t)__double_as_longlong(x), yb = (uint64_t)__double_as_longlong(y);
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)xb, (uint32_t)yb, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(xb >> 32), (uint32_t)(yb >> 32),
+                                                   false, false);
+  return __longlong_as_double((long long)(((uint64_t)hi[0] << 32) | lo[0])) +
+         __longlong_as_double((long long)(((uint64_t)hi[1] << 32) | lo[1]));
+}
+__device__ __forceinline__ double swap32(double x, double y) {
+  const uint64_t xb = (uint64_t)__double_as_longlong(x), yb = (uint64_t)__double_as_longlong(y);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)xb, (uint32_t)yb, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(xb >> 32), (uint32_t)(yb >> 32),
+                                                   false, false);
+  return __longlong_as_double((long long)(((uint64_t)hi[0] << 32) | lo[0])) +
+         __longlong_as_double((long long)(((uint64_t)hi[1] << 32) | lo[1]));
+}
+
+// Reduce-scatter over the four lane groups g = lane / 16: v[r] holds lane
+// group g's part of tile r; returns the total of tile g % R.
+template <int R, typename T>
+__device__ __forceinline__ T group_scatter(const T (&v)[R]) {
+  if constexpr (R == 4) {
+    const T n0 = swap32(v[0], v[2]);  // halves: tiles 0 | 2
+    const T n1 = swap32(v[1], v[3]);  //         tiles 1 | 3
+    return swap16(n0, n1);            // rows:   even n0, odd n1
+  } else {
+    static_assert(R == 2, "tiles per wave");
+    const T n = swap16(v[0], v[1]);   // even rows tile 0, odd rows tile 1
+    return swap32(n, n);
+  }
 }
 
 // Tiles per wave of an instantiation: 4 while the accumulators leave room,
@@ -187,7 +250,7 @@ __host__ __device__ constexpr int i8_tiles(int NT, int TT) { return (TT == 1 && 
 
 // L = limbs per column; TT = traits the epilogue carries (1, or kMaxTraits).
 template <int NT, int L, int TT>
-__global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
+__global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
                                                         int nshards, int64_t total_blocks,
                                                         ScanOut out) {
   constexpr int R = i8_tiles(NT, TT);
@@ -199,7 +262,7 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
   // direct-to-LDS loads still filling another.
   __shared__ v4i buf0[FRAG2];
   __shared__ v4i buf1[FRAG2];
-  __shared__ v4i bitsbuf[kWaves * (R + 1) * 64];     // [wave][r | mask][lane] next chunk
+  __shared__ v4i bitsbuf[kI8Waves * (R + 1) * 64];   // [wave][r | mask][lane] next chunk
   __shared__ double colrec_lds[kMaxCols * kColRec];
   __shared__ double trc_lds[kMaxTraits * kTrcRec];
   __shared__ unsigned int wg_pass[kMaxTraits];
@@ -226,7 +289,7 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
     const v4i* __restrict__ wts = reinterpret_cast<const v4i*>(D.wts);
     const int nchunks = D.nchunks;
     const int NS = 2 * nchunks;
-    const int64_t kbase = ((w - D.blk0) * kWaves + wave) * (16 * R);
+    const int64_t kbase = ((w - D.blk0) * kI8Waves + wave) * (16 * R);
     const uint64_t* rowp[R];
     block_rows<R>(D, w, wave, g, col, rowp);
 
@@ -265,24 +328,39 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
           dma_mask(D.mask, C + 1, g, mybits + R * 64);
         }
       }
+      // This stage's 64 bits per tile: lo = steps 0-1, hi = steps 2-3.
+      uint32_t lo[R], hi[R];
 #pragma unroll
-      for (int jj = 0; jj < kStageSteps; ++jj) {
-        v4i bf[R];
+      for (int r = 0; r < R; ++r) {
+        lo[r] = raw[r].x ^ ((raw[r].x ^ raw[r].z) & (0u - (uint32_t)h));
+        hi[r] = raw[r].y ^ ((raw[r].y ^ raw[r].w) & (0u - (uint32_t)h));
+      }
+      // Software pipeline over the stage's 4 NT weight fragments q = NT jj + t:
+      // fragment q+2 is read from LDS while q's MFMAs run, and step jj+1's
+      // bit operands are unpacked beside step jj's MFMAs.  The scheduling
+      // barrier after each fragment keeps the compiler from regrouping the
+      // LDS reads next to their first use.
+      constexpr int NQ = kStageSteps * NT;
+      v4i bf[R], bn[R], af[3];
+      unpack_step<R>(0, lo, hi, bf);
+      af[0] = cur[lane];
+      if (NQ > 1) af[1] = cur[64 + lane];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const uint32_t lo = raw[r].x ^ ((raw[r].x ^ raw[r].z) & (0u - (uint32_t)h));
-          const uint32_t hi = raw[r].y ^ ((raw[r].y ^ raw[r].w) & (0u - (uint32_t)h));
-          const uint32_t w16 = ((jj < 2) ? lo : hi) >> (16 * (jj & 1));
-          bf[r] = v4i{spread4(w16 & 15u), spread4((w16 >> 4) & 15u), spread4((w16 >> 8) & 15u),
-                      spread4((w16 >> 12) & 15u)};
+      for (int q = 0; q < NQ; ++q) {
+        const int jj = q / NT, t = q % NT;
+        if (jj + 1 < kStageSteps) {  // tile r's next operands beside fragment t = r mod NT
+#pragma unroll
+          for (int r = t; r < R; r += NT) unpack_tile<R>(jj + 1, r, lo, hi, bn);
         }
+        if (q + 2 < NQ) af[(q + 2) % 3] = cur[(q + 2) * 64 + lane];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const v4i a = cur[(jj * NT + t) * 64 + lane];
+        for (int r = 0; r < R; ++r)
+          acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[q % 3], bf[r], acc[r][t], 0, 0, 0);
+        if (t == NT - 1 && jj + 1 < kStageSteps) {
 #pragma unroll
-          for (int r = 0; r < R; ++r)
-            acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bf[r], acc[r][t], 0, 0, 0);
+          for (int r = 0; r < R; ++r) bf[r] = bn[r];
         }
+        __builtin_amdgcn_sched_barrier(0);
       }
       // Next stage's weights landed.  Loads complete in issue order, so after
       // an even stage the next chunk's bits (issued last) may stay in flight
@@ -297,8 +375,8 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
     // in flight (the epilogue then reads only LDS)
     {
       const double* __restrict__ gcr = ep.colrec;
-      for (int i = threadIdx.x; i < ep.ncol * kColRec; i += kThreads) colrec_lds[i] = gcr[i];
-      for (int i = threadIdx.x; i < ep.t * kTrcRec; i += kThreads) trc_lds[i] = ep.trc[i];
+      for (int i = threadIdx.x; i < ep.ncol * kColRec; i += kI8Threads) colrec_lds[i] = gcr[i];
+      for (int i = threadIdx.x; i < ep.t * kTrcRec; i += kI8Threads) trc_lds[i] = ep.trc[i];
       __builtin_amdgcn_s_waitcnt(0);
     }
 
@@ -319,68 +397,76 @@ __global__ __launch_bounds__(kThreads, 2) void scan_i8(const ShardDesc* __restri
     // ---- epilogue, from registers ----
     // Lane (g, col) holds, for k-mer col of tile r, output rows
     // u = 16 t + 4 g + i: the L limbs of base columns b = 4 j + g, j < J.  Each
-    // lane folds its columns into partial sums, the four lane groups are
-    // summed, and lane (g, col) keeps tile g % R: afterwards every lane of
-    // the wave owns one k-mer (R = 4) and decides it alone.
+    // lane folds its columns into partial sums per tile, then a reduce-scatter
+    // over the four lane groups (lane-swap instructions, no LDS) leaves lane
+    // (g, col) with the totals of tile g % R: every lane of the wave then owns
+    // one k-mer (R = 4) and decides it alone.
     const int t = ep.t, n = ep.n;
     const int rme = g % R;
     const bool primary = g < R;
-    int c = 0;
+    double pzz[R], pz0[R], pS[R][TT], psv[R][TT];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      uint32_t cc = cnt[r];
-      cc += __shfl_xor(cc, 16);
-      cc += __shfl_xor(cc, 32);
-      if (r == rme) c = (int)cc;
+      pzz[r] = pz0[r] = 0.0;
+#pragma unroll
+      for (int tr = 0; tr < TT; ++tr) pS[r][tr] = psv[r][tr] = 0.0;
     }
-    double kzz = 0.0, kz0 = 0.0, kS[TT], ksv[TT];
 #pragma unroll
-    for (int tr = 0; tr < TT; ++tr) kS[tr] = ksv[tr] = 0.0;
+    for (int j = 0; j < J; ++j) {
+      const double* cr = colrec_lds + (4 * j + g) * kColRec;
+      const double w0 = cr[0], w1 = cr[1], w2 = cr[2];
+      double kc[TT];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      double zz = 0.0, z0 = 0.0, S[TT], sv[TT];
+      for (int tr = 0; tr < TT; ++tr) kc[tr] = cr[3 + tr];
+      const int b = 4 * j + g;
 #pragma unroll
-      for (int tr = 0; tr < TT; ++tr) S[tr] = sv[tr] = 0.0;
-#pragma unroll
-      for (int j = 0; j < J; ++j) {
-        const double* cr = colrec_lds + (4 * j + g) * kColRec;
-        double z = cr[0] * (double)acc[r][(L * j) >> 2][(L * j) & 3];
-        if (L > 1) z = fma(cr[1], (double)acc[r][(L * j + 1) >> 2][(L * j + 1) & 3], z);
-        if (L > 2) z = fma(cr[2], (double)acc[r][(L * j + 2) >> 2][(L * j + 2) & 3], z);
-        if (j < JSP) {  // may be a residual column (b < t) or column d (b == t)
-          const int b = 4 * j + g;
-          const double zs = (b < t) ? 0.0 : z;
-          zz = fma(zs, zs, zz);
-          if (b == t) z0 = z;
-#pragma unroll
-          for (int tr = 0; tr < TT; ++tr) sv[tr] += (b == tr) ? z : 0.0;
+      for (int r = 0; r < R; ++r) {
+        const int sl = L * j;
+        const int a0 = acc[r][sl >> 2][sl & 3];
+        double z;
+        if (L == 3) {
+          // limbs 1 and 2 merged exactly: |a| <= 127 n < 2^23 for n <= 60000, so
+          // the 24-bit multiply-add is exact
+          const int a12 = __mul24(acc[r][(sl + 1) >> 2][(sl + 1) & 3], 254) +
+                          acc[r][(sl + 2) >> 2][(sl + 2) & 3];
+          z = fma(w2, (double)a12, w0 * (double)a0);
+        } else if (L == 2) {
+          z = fma(w1, (double)acc[r][(sl + 1) >> 2][(sl + 1) & 3], w0 * (double)a0);
         } else {
-          zz = fma(z, z, zz);
+          z = w0 * (double)a0;
+        }
+        if (j < JSP) {  // may be a residual column (b < t) or column d (b == t)
+          const double zs = (b < t) ? 0.0 : z;
+          pzz[r] = fma(zs, zs, pzz[r]);
+          if (b == t) pz0[r] = z;
+#pragma unroll
+          for (int tr = 0; tr < TT; ++tr) psv[r][tr] += (b == tr) ? z : 0.0;
+        } else {
+          pzz[r] = fma(z, z, pzz[r]);
         }
 #pragma unroll
-        for (int tr = 0; tr < TT; ++tr) S[tr] = fma(z, cr[3 + tr], S[tr]);
+        for (int tr = 0; tr < TT; ++tr) pS[r][tr] = fma(z, kc[tr], pS[r][tr]);
       }
-      zz = group_sum(zz);
-      z0 = group_sum(z0);
+    }
+    const int c = (int)group_scatter<R>(cnt);
+    const double kzz = group_scatter<R>(pzz);
+    const double kz0 = group_scatter<R>(pz0);
+    double kS[TT], ksv[TT];
 #pragma unroll
-      for (int tr = 0; tr < TT; ++tr) {
-        S[tr] = group_sum(S[tr]);
-        sv[tr] = group_sum(sv[tr]);
-      }
-      if (r == rme) {
-        kzz = zz;
-        kz0 = z0;
+    for (int tr = 0; tr < TT; ++tr) {
+      double vS[R], vsv[R];
 #pragma unroll
-        for (int tr = 0; tr < TT; ++tr) {
-          kS[tr] = S[tr];
-          ksv[tr] = sv[tr];
-        }
+      for (int r = 0; r < R; ++r) {
+        vS[r] = pS[r][tr];
+        vsv[r] = psv[r][tr];
       }
+      kS[tr] = group_scatter<R>(vS);
+      ksv[tr] = group_scatter<R>(vsv);
     }
 
     const int64_t kidx = kbase + 16 * rme + col;
     const bool valid = primary && (kidx < D.m);
-    const double cd = (double)c, cn = cd / (double)n;
+    const double cd = (double)c, cn = cd * ep.inv_n;
     // |dz_k| <= c h_k  =>  |d |z|^2| <= 2c sum|z_k| h_k + c^2 sum h_k^2
     //                              <= 2c |z| |h| + c^2 |h|^2
     const double bzz = 2.0 * cd * sqrt(kzz * ep.sum_h2) + cd * cd * ep.sum_h2;
@@ -682,12 +768,14 @@ template <int NT, int L, int TT>
 static hipError_t launch_i8_tr(dim3 grid, hipStream_t s, const ShardDesc* sd, int nshards,
                                int64_t total_blocks, const ScanOut& o) {
   auto kfn = scan_i8<NT, L, TT>;
-  hipLaunchKernelGGL(kfn, grid, dim3(kThreads), 0, s, sd, nshards, total_blocks, o);
+  hipLaunchKernelGGL(kfn, grid, dim3(kI8Threads), 0, s, sd, nshards, total_blocks, o);
   return hipGetLastError();
 }
 
 // k-mers per workgroup block of the instantiation for (NT, t)
-int64_t i8_block_kmers(int NT, int t) { return (int64_t)kWaves * 16 * i8_tiles(NT, t > 1 ? kMaxTraits : 1); }
+int64_t i8_block_kmers(int NT, int t) {
+  return (int64_t)kI8Waves * 16 * i8_tiles(NT, t > 1 ? kMaxTraits : 1);
+}
 
 // Column capacity of NT tiles at L limbs: 4 lane groups x floor(4 NT / L).
 int i8_column_capacity(int NT, int L) { return 4 * ((4 * NT) / L); }
