@@ -33,7 +33,6 @@ constexpr int kThreads = 64 * kWaves;
 // 4-wave workgroups per CU measured ~40% slower on the MFMA pipe.
 constexpr int kI8Waves = 8;
 constexpr int kI8Threads = 64 * kI8Waves;
-constexpr int kI8Steps = 8;      // 64-sample MFMA steps per 512-sample chunk
 
 // Bare workgroup barrier for the stage loop.  Every LDS value a wave read in
 // the stage has been consumed (so those reads are complete) and each wave has
@@ -47,34 +46,14 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
-template <int R>
-__device__ __forceinline__ void unpack_tile(int jj, int r, const uint32_t (&lo)[R],
-                                            const uint32_t (&hi)[R], v4i (&b)[R]) {
+// B operand of MFMA step jj (0..7) of a 512-sample chunk from the lane's four
+// chunk words: operand byte e = 4 d + q takes bit 8 q + 4 (jj & 1) + d of
+// word jj / 2, so each dword is one shift and one mask.
+__device__ __forceinline__ v4i unpack_tile(int jj, const uint32_t (&wd)[4]) {
   const int sb = 4 * (jj & 1);
-  const uint32_t w = (jj < 2) ? lo[r] : hi[r];
-  b[r] = v4i{(int)((w >> sb) & 0x01010101u), (int)((w >> (sb + 1)) & 0x01010101u),
+  const uint32_t w = wd[jj >> 1];
+  return v4i{(int)((w >> sb) & 0x01010101u), (int)((w >> (sb + 1)) & 0x01010101u),
              (int)((w >> (sb + 2)) & 0x01010101u), (int)((w >> (sb + 3)) & 0x01010101u)};
-}
-
-// B operand of MFMA step jj (0..3) of a 256-sample stage: operand byte e =
-// 4 d + q of a lane takes bit 8 q + 4 (jj & 1) + d of its stage word
-// (jj < 2 ? lo : hi), so each dword is one shift and one mask.
-template <int R>
-__device__ __forceinline__ void unpack_step(int jj, const uint32_t (&lo)[R],
-                                            const uint32_t (&hi)[R], v4i (&b)[R]) {
-  const int sb = 4 * (jj & 1);
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const uint32_t w = (jj < 2) ? lo[r] : hi[r];
-    b[r] = v4i{(int)((w >> sb) & 0x01010101u), (int)((w >> (sb + 1)) & 0x01010101u),
-               (int)((w >> (sb + 2)) & 0x01010101u), (int)((w >> (sb + 3)) & 0x01010101u)};
-  }
-}
-
-__device__ __forceinline__ double wave_sum4(double v) {  // over lanes l^1, l^2
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
-  return v;
 }
 
 // Wave-aggregated append of candidate records.  Must be reached by every lane
@@ -144,27 +123,28 @@ __device__ __forceinline__ uint32_t classify_i8(int c, double xxM, double bxx, d
 }
 
 // ----------------------------------------------------------------------------
-// Bulk int8-limb scan over a table of shards.  Workgroup = 4 waves; global
-// block `w` belongs to the shard s with sd[s].blk0 <= w < sd[s+1].blk0, and
-// wave v of that block owns the shard's k-mers [(b*4 + v) * 16R, +16R)
-// (b = w - blk0) as R tiles of 16.  Persistent: blocks are strided over the
-// grid, so one launch covers every shard.
+// Bulk int8-limb scan over a table of shards.  Workgroup = 8 waves (one per
+// CU); global block `w` belongs to the shard s with sd[s].blk0 <= w <
+// sd[s+1].blk0, and wave v of that block owns the shard's k-mers
+// [(b*8 + v) * 16R, +16R) (b = w - blk0) as R tiles of 16.  Persistent:
+// blocks are strided over the grid, so one launch covers every shard.
 //
-// Pipeline per block: weights stream through two LDS stage buffers of 256
-// samples (4 MFMA steps) each, filled by direct global->LDS loads for stage
-// S+1 while stage S computes; each lane's bits for the next 512-sample chunk
-// are prefetched into registers one chunk ahead; the next block's first
-// stage and first chunk are prefetched before the epilogue runs.
-constexpr int kStageSteps = 4;
+// Pipeline per block: weights stream through two LDS stage buffers of one
+// 512-sample chunk (8 MFMA steps) each, filled by direct global->LDS loads
+// for chunk C+1 while chunk C computes; each lane's bits and accession mask
+// for chunk C+1 are delivered the same way into its wave's LDS slot; the next
+// block's first chunk is prefetched before the epilogue runs.
+constexpr int kStageSteps = 8;   // MFMA steps per LDS stage (one 512-sample chunk)
 
 template <int NT>
 __device__ __forceinline__ void dma_stage(const v4i* __restrict__ src, v4i* dst, int wave,
                                           int lane) {
-  // the stage's 4 NT blocks of 64 v4i, dealt round-robin to the waves
+  // the stage's kStageSteps NT blocks of 64 v4i, dealt round-robin to the waves
+  constexpr int NB = kStageSteps * NT;
 #pragma unroll
-  for (int i = 0; i < (4 * NT + kI8Waves - 1) / kI8Waves; ++i) {
+  for (int i = 0; i < (NB + kI8Waves - 1) / kI8Waves; ++i) {
     const int qd = i * kI8Waves + wave;
-    if (qd < 4 * NT)
+    if (qd < NB)
       __builtin_amdgcn_global_load_lds((const void*)(src + qd * 64 + lane),
                                        (__attribute__((address_space(3))) void*)(dst + qd * 64),
                                        16, 0, 0);
@@ -254,7 +234,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
                                                         int nshards, int64_t total_blocks,
                                                         ScanOut out) {
   constexpr int R = i8_tiles(NT, TT);
-  constexpr int FRAG2 = kStageSteps * NT * 64;       // v4i per 256-sample stage
+  constexpr int FRAG2 = kStageSteps * NT * 64;       // v4i per 512-sample chunk
   constexpr int J = (4 * NT) / L;                    // base columns per lane group
   constexpr int JSP = (TT + 4) / 4;                  // slots that may hold r_c or d columns
   static_assert(4 * J <= kMaxCols, "column records exceed the LDS table");
@@ -288,7 +268,6 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     const EpiI8& ep = D.ep;
     const v4i* __restrict__ wts = reinterpret_cast<const v4i*>(D.wts);
     const int nchunks = D.nchunks;
-    const int NS = 2 * nchunks;
     const int64_t kbase = ((w - D.blk0) * kI8Waves + wave) * (16 * R);
     const uint64_t* rowp[R];
     block_rows<R>(D, w, wave, g, col, rowp);
@@ -301,26 +280,25 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     uint32_t cnt[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) cnt[r] = 0;
-    uint4 raw[R];
 
-    __builtin_amdgcn_s_waitcnt(0);  // stage 0 weights and chunk 0 bits have landed
+    __builtin_amdgcn_s_waitcnt(0);  // chunk 0 weights and bits have landed
     __syncthreads();
-    for (int S = 0; S < NS; ++S) {
-     {
-      const int h = S & 1, C = S >> 1;  // stage parity, 512-sample chunk
+    for (int C = 0; C < nchunks; ++C) {
+      const int h = C & 1;
       const v4i* cur = h ? buf1 : buf0;
-      if (S + 1 < NS) dma_stage<NT>(wts + (size_t)(S + 1) * FRAG2, h ? buf0 : buf1, wave, lane);
-      if (h == 0) {
-        const v4i mkv = mybits[R * 64 + lane];
-        const uint4 mk = make_uint4((uint32_t)mkv.x, (uint32_t)mkv.y, (uint32_t)mkv.z, (uint32_t)mkv.w);
+      if (C + 1 < nchunks) dma_stage<NT>(wts + (size_t)(C + 1) * FRAG2, h ? buf0 : buf1, wave, lane);
+      // this chunk's 128 bits per tile (masked) and their popcount
+      uint32_t wd[R][4];
+      {
+        const v4i mk = mybits[R * 64 + lane];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const v4i v = mybits[r * 64 + lane];
-          raw[r].x = (uint32_t)v.x & mk.x;
-          raw[r].y = (uint32_t)v.y & mk.y;
-          raw[r].z = (uint32_t)v.z & mk.z;
-          raw[r].w = (uint32_t)v.w & mk.w;
-          cnt[r] += __popc(raw[r].x) + __popc(raw[r].y) + __popc(raw[r].z) + __popc(raw[r].w);
+          wd[r][0] = (uint32_t)(v.x & mk.x);
+          wd[r][1] = (uint32_t)(v.y & mk.y);
+          wd[r][2] = (uint32_t)(v.z & mk.z);
+          wd[r][3] = (uint32_t)(v.w & mk.w);
+          cnt[r] += __popc(wd[r][0]) + __popc(wd[r][1]) + __popc(wd[r][2]) + __popc(wd[r][3]);
         }
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's bits reads are done
         if (C + 1 < nchunks) {
@@ -328,29 +306,23 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
           dma_mask(D.mask, C + 1, g, mybits + R * 64);
         }
       }
-      // This stage's 64 bits per tile: lo = steps 0-1, hi = steps 2-3.
-      uint32_t lo[R], hi[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        lo[r] = raw[r].x ^ ((raw[r].x ^ raw[r].z) & (0u - (uint32_t)h));
-        hi[r] = raw[r].y ^ ((raw[r].y ^ raw[r].w) & (0u - (uint32_t)h));
-      }
-      // Software pipeline over the stage's 4 NT weight fragments q = NT jj + t:
+      // Software pipeline over the chunk's 8 NT weight fragments q = NT jj + t:
       // fragment q+2 is read from LDS while q's MFMAs run, and step jj+1's
       // bit operands are unpacked beside step jj's MFMAs.  The scheduling
       // barrier after each fragment keeps the compiler from regrouping the
       // LDS reads next to their first use.
       constexpr int NQ = kStageSteps * NT;
       v4i bf[R], bn[R], af[3];
-      unpack_step<R>(0, lo, hi, bf);
+#pragma unroll
+      for (int r = 0; r < R; ++r) bf[r] = unpack_tile(0, wd[r]);
       af[0] = cur[lane];
-      if (NQ > 1) af[1] = cur[64 + lane];
+      af[1] = cur[64 + lane];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int jj = q / NT, t = q % NT;
         if (jj + 1 < kStageSteps) {  // tile r's next operands beside fragment t = r mod NT
 #pragma unroll
-          for (int r = t; r < R; r += NT) unpack_tile<R>(jj + 1, r, lo, hi, bn);
+          for (int r = t; r < R; r += NT) bn[r] = unpack_tile(jj + 1, wd[r]);
         }
         if (q + 2 < NQ) af[(q + 2) % 3] = cur[(q + 2) * 64 + lane];
 #pragma unroll
@@ -362,13 +334,9 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      // Next stage's weights landed.  Loads complete in issue order, so after
-      // an even stage the next chunk's bits (issued last) may stay in flight
-      // for one more stage: vmcnt(R + 1).
-      if (h == 0 && C + 1 < nchunks) __builtin_amdgcn_s_waitcnt(0x0F70 | (R + 1));
-      else __builtin_amdgcn_s_waitcnt(0);
+      // next chunk's weights and bits landed in every wave's share
+      __builtin_amdgcn_s_waitcnt(0);
       lds_barrier();
-     }
     }
 
     // this shard's epilogue constants into LDS while no direct-to-LDS load is
