@@ -140,7 +140,8 @@ struct kg_ctx {
   int64_t cand_cap = 0;
   size_t counters_len = 0;
   kg_timing timing{};
-  int grid_mult = 1;  // persistent workgroups per CU (KG_GRID_MULT)
+  int grid_mult = 1;  // int8 scan: persistent workgroups per CU (KG_GRID_MULT)
+  int grid_mult_f32 = 8;  // FP32 scan: workgroups per CU
   // captured graph of the last batched scan
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -928,7 +929,7 @@ static int scan_f32_shard(kg_ctx* ctx, const HShard& s, int sidx, const kg_scan_
   ep.all_sig = p_thr >= 1.0 ? 1 : 0;
   ep.Tstar_lo = (Ts > Tmax) ? INFINITY : Ts * (1.0 - 1e-9);
   unsigned long long cnt[1 + kMaxTraits];
-  const int grid = ctx->cus * ctx->grid_mult;
+  const int grid = ctx->cus * ctx->grid_mult_f32;
   for (int attempt = 0; attempt < 3; ++attempt) {
     o.cand = ctx->cand.as<Cand>();
     o.counters = ctx->counters.as<unsigned long long>();

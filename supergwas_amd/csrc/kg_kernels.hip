@@ -307,16 +307,17 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         }
       }
       // Software pipeline over the chunk's 8 NT weight fragments q = NT jj + t:
-      // fragment q+2 is read from LDS while q's MFMAs run, and step jj+1's
+      // fragment q+PD is read from LDS while q's MFMAs run, and step jj+1's
       // bit operands are unpacked beside step jj's MFMAs.  The scheduling
       // barrier after each fragment keeps the compiler from regrouping the
       // LDS reads next to their first use.
       constexpr int NQ = kStageSteps * NT;
-      v4i bf[R], bn[R], af[3];
+      constexpr int PD = 4;  // fragments read ahead
+      v4i bf[R], bn[R], af[PD + 1];
 #pragma unroll
       for (int r = 0; r < R; ++r) bf[r] = unpack_tile(0, wd[r]);
-      af[0] = cur[lane];
-      af[1] = cur[64 + lane];
+#pragma unroll
+      for (int i = 0; i < PD; ++i) af[i] = cur[i * 64 + lane];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int jj = q / NT, t = q % NT;
@@ -324,10 +325,11 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 #pragma unroll
           for (int r = t; r < R; r += NT) bn[r] = unpack_tile(jj + 1, wd[r]);
         }
-        if (q + 2 < NQ) af[(q + 2) % 3] = cur[(q + 2) * 64 + lane];
+        if (q + PD < NQ) af[(q + PD) % (PD + 1)] = cur[(q + PD) * 64 + lane];
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[q % 3], bf[r], acc[r][t], 0, 0, 0);
+          acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[q % (PD + 1)], bf[r], acc[r][t], 0,
+                                                            0, 0);
         if (t == NT - 1 && jj + 1 < kStageSteps) {
 #pragma unroll
           for (int r = 0; r < R; ++r) bf[r] = bn[r];
