@@ -46,6 +46,11 @@ def main():
                     help="only the mixed scans (for rocprofv3 counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="copy this many bytes device-to-device and exit (counter calibration)")
+    ap.add_argument("--legs", default="c2,c4,c5",
+                    help="further workloads beside the headline: c2 (FP32 kernel, n=1024 p=20, "
+                         "50 batches of 1e6), c4 (400 uneven shards), c5 (n=4096 p=64 8 traits, "
+                         "streamed from page-locked host memory); '' for none")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -81,6 +86,12 @@ def main():
     t1 = time.time()
     mixed.synth_bits(dev.ptr, ds, 0, args.m, rw)
     log(f"[bench] synth {args.m} rows on device ({time.time() - t1:.1f}s)")
+
+    tune = None
+    if not args.no_autotune:
+        t_at = time.time()
+        tune = mixed.autotune(2_000_000)
+        log(f"[bench] autotune: {tune[0]} tiles/wave, {tune[1]} workgroups/CU ({time.time() - t_at:.1f}s)")
 
     # ---- mixed precision: warmup + timed steps ----
     for _ in range(args.warmup):
@@ -198,7 +209,174 @@ def main():
                           "flips": flips, "counts_bit_exact": counts_exact},
         "cpu_baseline": cpu,
     }
+    if tune is not None:
+        line["config"]["autotune"] = {"tiles_per_wave": tune[0], "workgroups_per_cu": tune[1]}
+    dev.free()
+    legs = [x for x in args.legs.split(",") if x]
+    if legs:
+        line["legs"] = {}
+    for name in legs:
+        t_leg = time.time()
+        line["legs"][name] = LEGS[name](args, synth, ds)
+        log(f"[bench] leg {name} done ({time.time() - t_leg:.1f}s)")
     print(json.dumps(line), flush=True)
+
+
+def leg_c2(args, synth, ds_c3):
+    """Plain FP32 kernel at scale: n=1024, p=20, m=5e7 in 50 batches of 1e6, hits at p < 1e-5
+    and a full-statistic checksum (sum of T in FP64, sum of allele counts exactly)."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    from tests.helpers import T_from
+    import oracle
+    cfg = dict(synth.CONFIGS["fp32_scale"])
+    ds = synth.make_dataset(seed=2, **cfg)
+    rw = synth.row_words_for(ds.n_acc)
+    nb, bm = 50, 1_000_000
+    prm = ScanParams(0.2, 5.0, 0.0)
+    out = {"workload": "FP32 GPU kernel: n=1024, p=20, m=5e7 as 50 batches of 1e6"}
+    yy = oracle.yyM(ds.y, ds.U)
+    for prec in ("fp32", "mixed"):
+        k = KgwasHip(precision=prec)
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        dev = k.alloc(nb * bm * rw * 8)
+        k.synth_bits(dev.ptr, ds, 0, nb * bm, rw)
+        k.scan_device(dev.ptr, bm, rw, prm)  # warm-up
+        t0 = time.perf_counter()
+        bulk, hits = 0.0, 0
+        for b in range(nb):
+            h, _ = k.scan_device(dev.ptr + b * bm * rw * 8, bm, rw, prm)
+            bulk += k.timing()["bulk_ms"]
+            hits += len(h)
+        el = time.perf_counter() - t0
+        sT, sc = 0.0, 0
+        for b in range(nb):
+            st = k.stats_device(dev.ptr + b * bm * rw * 8, bm, rw)
+            sT += float(T_from(st["xxM"], st["xyM"][:, 0], yy, ds.n).sum())
+            sc += int(st["c"].astype(np.int64).sum())
+        out[prec] = {"kmers_per_s": nb * bm / el, "kernel_kmers_per_s": nb * bm / (bulk / 1e3),
+                     "hits_p_lt_1e-5": hits, "checksum_sum_T": sT, "checksum_sum_c": sc}
+        log(f"[bench] c2 {prec}: {nb * bm / el:.3e} k-mers/s ({nb * bm / (bulk / 1e3):.3e} kernel),"
+            f" hits {hits}, sum T {sT:.6e}, sum c {sc}")
+        dev.free()
+        k.close()
+    out["checksum_rel_diff_T"] = abs(out["fp32"]["checksum_sum_T"] - out["mixed"]["checksum_sum_T"]) / \
+        max(abs(out["mixed"]["checksum_sum_T"]), 1.0)
+    out["checksum_c_equal"] = out["fp32"]["checksum_sum_c"] == out["mixed"]["checksum_sum_c"]
+    return out
+
+
+def leg_c4(args, synth, ds):
+    """400 shards, sizes log-uniform in [1e3, 2e6], each with its own 0-15% missing-phenotype
+    mask (own null model), n=2048, p=32: one persistent launch (and its HIP-graph replay)
+    against one launch per shard."""
+    from supergwas_amd.api import SHARDS_GRAPH, SHARDS_PER_LAUNCH, KgwasHip, ScanParams
+    rng = np.random.default_rng(4)
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(2e6), 400)).astype(np.int64)
+    total = int(sizes.sum())
+    rw = synth.row_words_for(ds.n_acc)
+    k = KgwasHip(precision="mixed")
+    mids = []
+    t0 = time.time()
+    for s in range(len(sizes)):
+        keep = np.ones(ds.n, bool)
+        drop = rng.choice(ds.n, size=int(rng.uniform(0, 0.15) * ds.n), replace=False)
+        keep[drop] = False
+        idxs = ds.idxs[keep]
+        U, _, _ = np.linalg.svd(ds.U[keep], full_matrices=False)
+        y = ds.y[keep]
+        if s == 0:
+            k.set_model(ds.n_acc, idxs, U, y)
+            mids.append(0)
+        else:
+            mids.append(k.add_model(ds.n_acc, idxs, U, y))
+    log(f"[bench] c4: {len(sizes)} shard models in {time.time() - t0:.1f}s, {total} k-mers")
+    dev = k.alloc(total * rw * 8)
+    k.synth_bits(dev.ptr, ds, 0, total, rw)
+    tune = k.autotune(2_000_000)
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    shards = [(dev.ptr + int(o) * rw * 8, int(m), rw, mid) for o, m, mid in zip(off, sizes, mids)]
+    prm = ScanParams(0.2, 6.0, 0.0)
+    out = {"workload": f"{len(sizes)} shards log-uniform in [1e3, 2e6], total {total} k-mers, "
+                       "n=2048, p=32, per-shard masks 0-15% missing",
+           "autotune": {"tiles_per_wave": tune[0], "workgroups_per_cu": tune[1]}}
+    ref = None
+    for name, flags in (("persistent", 0), ("graph", SHARDS_GRAPH), ("per_shard", SHARDS_PER_LAUNCH)):
+        k.scan_shards(shards, prm, flags)  # warm-up (and graph capture)
+        ts, ks = [], []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            hits, npass = k.scan_shards(shards, prm, flags)
+            ts.append(time.perf_counter() - t1)
+            ks.append(k.timing()["total_ms"])
+        same = True
+        if ref is None:
+            ref = (hits.copy(), npass.copy())
+        else:
+            same = bool(np.array_equal(hits["idx"], ref[0]["idx"]) and
+                        np.array_equal(hits["shard"], ref[0]["shard"]) and
+                        np.array_equal(npass, ref[1]))
+        out[name] = {"kmers_per_s": total / min(ts), "device_ms": min(ks), "wall_ms": 1e3 * min(ts),
+                     "hits": int(len(hits)), "matches_persistent": same}
+        log(f"[bench] c4 {name}: {total / min(ts):.3e} k-mers/s (device {min(ks):.1f} ms,"
+            f" wall {1e3 * min(ts):.1f} ms), hits {len(hits)}, same {same}")
+    out["speedup_persistent_vs_per_shard"] = out["per_shard"]["wall_ms"] / out["persistent"]["wall_ms"]
+    dev.free()
+    k.close()
+    return out
+
+
+def leg_c5(args, synth, ds_c3):
+    """n=4096, p=64, 8 traits (correlation 0.2, 100 causal each) in one pass: kernel-only rate on
+    HBM-resident rows and end-to-end rate streaming from page-locked host memory (64 MiB chunks
+    on a copy stream)."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    cfg = dict(synth.CONFIGS["multitrait"])
+    cfg["m"] = 10_000_000
+    ds = synth.make_dataset(seed=5, n_struct=1024, **cfg)
+    rw = synth.row_words_for(ds.n_acc)
+    k = KgwasHip(precision="mixed")
+    k.set_model(ds.n_acc, ds.idxs, ds.U, ds.Y)
+    prm = ScanParams(0.2, 6.0, 0.0)
+    mk, mh = 10_000_000, 8_000_000
+    dev = k.alloc(mk * rw * 8)
+    k.synth_bits(dev.ptr, ds, 0, mk, rw)
+    tune = k.autotune(1_000_000)
+    k.scan_device(dev.ptr, mk, rw, prm)
+    bulk = []
+    for _ in range(3):
+        k.scan_device(dev.ptr, mk, rw, prm)
+        bulk.append(k.timing()["bulk_ms"])
+    kb = min(bulk) / 1e3
+    host = k.host_alloc((mh, rw))
+    from supergwas_amd.api import lib
+    k._check(lib().kg_memcpy(k._h, host.ctypes.data, dev.ptr, mh * rw * 8, 1))
+    dev.free()
+    walls, copies, kern = [], [], []
+    for _ in range(3):
+        hits, npass = k.scan(host, prm)
+        tm = k.timing()
+        walls.append(tm["wall_ms"])
+        copies.append(tm["copy_ms"])
+        kern.append(tm["bulk_ms"])
+    k.host_free(host)
+    w = min(walls) / 1e3
+    out = {"workload": "n=4096, p=64, t=8 (corr 0.2, 100 causal each), 3 int8 limbs",
+           "autotune": {"tiles_per_wave": tune[0], "workgroups_per_cu": tune[1]},
+           "kernel_only": {"m": mk, "kmers_per_s": mk / kb, "GBps": mk * rw * 8 / kb / 1e9,
+                           "trait_tests_per_s": 8 * mk / kb},
+           "streamed": {"m_per_pass": mh, "kmers_per_s": mh / w, "GBps": mh * rw * 8 / w / 1e9,
+                        "copy_ms": min(copies), "kernel_ms": min(kern), "wall_ms": 1e3 * w,
+                        "chunk_MiB": 64, "host_memory": "page-locked (kg_host_alloc)",
+                        "hits": int(len(hits)), "passed_per_trait": [int(x) for x in npass],
+                        "projected_1e9_kmers_s": 1e9 / (mh / w)}}
+    log(f"[bench] c5 kernel {mk / kb:.3e} k-mers/s ({mk * rw * 8 / kb / 1e9:.0f} GB/s); streamed "
+        f"{mh / w:.3e} k-mers/s ({mh * rw * 8 / w / 1e9:.1f} GB/s, copy {min(copies):.0f} ms, "
+        f"kernel {min(kern):.0f} ms, wall {1e3 * w:.0f} ms)")
+    k.close()
+    return out
+
+
+LEGS = {"c2": leg_c2, "c4": leg_c4, "c5": leg_c5}
 
 
 if __name__ == "__main__":

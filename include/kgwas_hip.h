@@ -73,6 +73,8 @@ typedef struct kg_timing {
   double total_ms;      /* first to last device operation of the scan */
   int64_t candidates;   /* k-mers sent to refinement / host decision */
   int64_t kmers;        /* k-mers scanned */
+  double copy_ms;       /* kg_scan: host-to-device copies (copy stream, summed) */
+  double wall_ms;       /* kg_scan: host wall time of the whole call */
 } kg_timing;
 
 /* Device selection, stream creation.  Replaces the OpenBLAS/OpenMP thread
@@ -98,8 +100,9 @@ int kg_set_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int32_t n, con
  * n_passed[t] receives the pre-filter pass count per trait (the `.count`
  * value, src/kgwas.cpp:292-304).  If more than `cap` hits exist, returns
  * KG_ERR_CAPACITY with *n_out = required count.
- * kg_scan: host bits (copied in slices); kg_scan_device: bits already on the
- * context's device. */
+ * kg_scan: host bits, streamed to the device in 64 MiB chunks on a copy
+ * stream, each chunk's copy overlapping the previous chunk's scan;
+ * kg_scan_device: bits already on the context's device. */
 int kg_scan(kg_ctx* ctx, const uint64_t* bits, int64_t m, int64_t row_words,
             const kg_scan_params* prm, kg_hit* out, int64_t cap, int64_t* n_out,
             int64_t* n_passed);
@@ -143,12 +146,27 @@ int kg_stats_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t row_
 
 int kg_last_timing(const kg_ctx* ctx, kg_timing* out);
 
+/* Launch-shape autotuner (no reference counterpart; the reference has one
+ * code path): times the int8 bulk scan of `sample` random rows with model 0
+ * for each candidate k-mer tiles per wave (k-mers per workgroup) and
+ * persistent workgroups per CU, keeps the fastest for later scans on ctx and
+ * returns it.  A no-op for KG_PREC_FP32. */
+int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t* grid_mult);
+
 /* Device memory on the context's device for callers without their own
  * allocator (tests, benchmarks, ctypes hosts).  kg_memcpy direction:
  * 0 = host to device, 1 = device to host, 2 = device to device. */
 int kg_dev_alloc(kg_ctx* ctx, size_t bytes, void** d_ptr);
 int kg_dev_free(kg_ctx* ctx, void* d_ptr);
 int kg_memcpy(kg_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t direction);
+
+/* Page-locked host memory for callers that stream rows with kg_scan: rows in
+ * such memory are copied in 64 MiB chunks on a copy stream while the previous
+ * chunk is scanned (pageable rows go through the context's own page-locked
+ * bounce buffers).  No reference counterpart (the reference reads its input
+ * from files into host memory, src/kgwas_utils.hpp:225-282). */
+int kg_host_alloc(kg_ctx* ctx, size_t bytes, void** h_ptr);
+int kg_host_free(kg_ctx* ctx, void* h_ptr);
 
 /* Seeded synthetic bits on the device (benchmarks and tests; no reference
  * counterpart): rows kmer0 .. kmer0+m-1 of the model in synth_core.h. */

@@ -60,7 +60,8 @@ SHARDS_GRAPH = 2
 class _Timing(ctypes.Structure):
     _fields_ = [("bulk_ms", ctypes.c_double), ("refine_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("candidates", ctypes.c_int64),
-                ("kmers", ctypes.c_int64)]
+                ("kmers", ctypes.c_int64), ("copy_ms", ctypes.c_double),
+                ("wall_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -86,6 +87,9 @@ SIGNATURES = {
     "kg_last_timing": ([_vp, ctypes.POINTER(_Timing)], ctypes.c_int),
     "kg_dev_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
     "kg_dev_free": ([_vp, _vp], ctypes.c_int),
+    "kg_host_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
+    "kg_autotune": ([_vp, _i64, ctypes.POINTER(_i32), ctypes.POINTER(_i32)], ctypes.c_int),
+    "kg_host_free": ([_vp, _vp], ctypes.c_int),
     "kg_memcpy": ([_vp, _vp, _vp, ctypes.c_size_t, _i32], ctypes.c_int),
     "kg_synth_bits": ([_vp, _vp, ctypes.c_uint64, _i64, _i64, _i32, _i64, _vp, _vp, _i32, _i32],
                       ctypes.c_int),
@@ -159,9 +163,13 @@ class KgwasHip:
         self.precision = precision
         self.t = 0
         self.n = 0
+        self._pinned = {}
 
     def close(self):
         if self._h and self._h.value:
+            for ptr in list(getattr(self, "_pinned", {}).values()):
+                lib().kg_host_free(self._h, ptr)
+            self._pinned = {}
             lib().kg_destroy(self._h)
             self._h = _vp()
 
@@ -284,4 +292,27 @@ class KgwasHip:
         t = _Timing()
         self._check(lib().kg_last_timing(self._h, ctypes.byref(t)))
         return dict(bulk_ms=t.bulk_ms, refine_ms=t.refine_ms, total_ms=t.total_ms,
-                    candidates=t.candidates, kmers=t.kmers)
+                    candidates=t.candidates, kmers=t.kmers, copy_ms=t.copy_ms, wall_ms=t.wall_ms)
+
+    def autotune(self, sample=2_000_000):
+        """Pick k-mer tiles per wave and workgroups per CU by timing `sample` random
+        rows with model 0; returns (tiles, grid_mult)."""
+        tl, gm = _i32(), _i32()
+        self._check(lib().kg_autotune(self._h, int(sample), ctypes.byref(tl), ctypes.byref(gm)))
+        return tl.value, gm.value
+
+    def host_alloc(self, shape, dtype=np.uint64):
+        """Page-locked host array (numpy view); kg_scan streams such rows without a
+        bounce copy.  Free with host_free(arr) before the context closes."""
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = _vp()
+        self._check(lib().kg_host_alloc(self._h, nbytes, ctypes.byref(p)))
+        buf = (ctypes.c_uint8 * max(nbytes, 1)).from_address(p.value)
+        arr = np.frombuffer(buf, dtype=np.uint8, count=nbytes).view(dtype).reshape(shape)
+        self._pinned[arr.ctypes.data] = p.value
+        return arr
+
+    def host_free(self, arr):
+        ptr = self._pinned.pop(arr.ctypes.data, None)
+        if ptr:
+            self._check(lib().kg_host_free(self._h, ptr))

@@ -13,6 +13,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <functional>
 #include <cstdio>
 #include <memory>
 #include <string>
@@ -22,9 +24,11 @@
 #include "kg_internal.h"
 
 namespace kg {
-hipError_t launch_scan_i8(int NT, int L, int t, int grid, hipStream_t s, const ShardDesc* sd,
-                          int nshards, int64_t total_blocks, const ScanOut& o);
-int64_t i8_block_kmers(int NT, int t);
+hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t s,
+                          const ShardDesc* sd, int nshards, int64_t total_blocks,
+                          const ScanOut& o);
+int64_t i8_block_kmers(int NT, int L, int t, int R_req);
+int i8_tiles_for(int NT, int L, int t, int R_req);
 int i8_column_capacity(int NT, int L);
 hipError_t set_kernel_attributes();
 hipError_t launch_scan_f32(int NT, int nch, int grid, hipStream_t s, const uint64_t* bits,
@@ -64,6 +68,27 @@ struct DBuf {
   template <class T>
   T* as() const {
     return reinterpret_cast<T*>(p);
+  }
+};
+
+// Page-locked host buffer (bounce buffer for pageable rows).
+struct HBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  HBuf() = default;
+  HBuf(const HBuf&) = delete;
+  HBuf& operator=(const HBuf&) = delete;
+  ~HBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&p, b ? b : 16, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = b;
+    return e;
   }
 };
 
@@ -133,14 +158,21 @@ struct kg_ctx {
   int cus = 256;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // kg_scan streaming: copy stream, copy-done events and copy timing events
+  // per staging buffer, page-locked bounce buffers for pageable rows
+  hipStream_t cstream = nullptr;
+  hipEvent_t cev[2] = {nullptr, nullptr};
+  hipEvent_t ctev[4] = {nullptr, nullptr, nullptr, nullptr};
+  HBuf bounce[2];
   std::string err;
   std::vector<std::unique_ptr<ModelDev>> models;
   // scratch
-  DBuf counters, cand, res, stage, st_c, st_xx, st_xy, st_S, sdesc, sdesc1, rdesc;
+  DBuf counters, cand, res, stage, stage2, st_c, st_xx, st_xy, st_S, sdesc, sdesc1, rdesc;
   int64_t cand_cap = 0;
   size_t counters_len = 0;
   kg_timing timing{};
   int grid_mult = 1;  // int8 scan: persistent workgroups per CU (KG_GRID_MULT)
+  int tiles = 0;      // int8 scan: k-mer tiles per wave (0 = the kernel's default)
   int grid_mult_f32 = 8;  // FP32 scan: workgroups per CU
   // captured graph of the last batched scan
   hipGraph_t graph = nullptr;
@@ -317,12 +349,13 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
     if (!(k == 0 && md.rotated && md.drop_d)) zc.push_back(k);
   const int nz = (int)zc.size();
   const int nb = nz + t;  // base columns: t residuals, then nz structure columns
+  const int nt_max = (L == 3) ? 16 : (L == 2 ? 6 : 3);
   int NT = 1;
-  while (NT <= 8 && i8_column_capacity(NT, L) < nb) ++NT;
-  if (NT > 8 || (L == 1 && NT > 3) || (L == 2 && NT > 6))
+  while (NT <= nt_max && i8_column_capacity(NT, L) < nb) ++NT;
+  if (NT > nt_max)
     return fail(ctx, KG_ERR_ARG,
                 "p + t exceeds the int8 kernel's weight columns for this limb count (at most " +
-                    std::to_string(i8_column_capacity(L == 3 ? 8 : (L == 2 ? 6 : 3), L)) + ")");
+                    std::to_string(i8_column_capacity(nt_max, L)) + " structure + trait columns)");
   const int NU = 16 * NT;
   const int ncol = i8_column_capacity(NT, L);
   M.NT = NT;
@@ -536,7 +569,10 @@ extern "C" int kg_create(kg_ctx** out, const kg_opts* opts) {
   KG_CK(ctx, hipGetDeviceProperties(&prop, ctx->device));
   ctx->cus = prop.multiProcessorCount;
   KG_CK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  KG_CK(ctx, hipStreamCreateWithFlags(&ctx->cstream, hipStreamNonBlocking));
   for (auto& e : ctx->ev) KG_CK(ctx, hipEventCreate(&e));
+  for (auto& e : ctx->cev) KG_CK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (auto& e : ctx->ctev) KG_CK(ctx, hipEventCreate(&e));
   KG_CK(ctx, set_kernel_attributes());
   if (const char* v = std::getenv("KG_GRID_MULT")) {
     const int g = atoi(v);
@@ -561,8 +597,14 @@ extern "C" void kg_destroy(kg_ctx* ctx) {
   }
   drop_graph(ctx);
   ctx->models.clear();
+  if (ctx->cstream) (void)hipStreamSynchronize(ctx->cstream);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->cev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->ctev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->cstream) (void)hipStreamDestroy(ctx->cstream);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -690,13 +732,13 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
   KG_CK(ctx, launch_zero(ctx->stream, o.counters, (int)ctx->counters_len));
   KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (!per_launch) {
-    KG_CK(ctx, launch_scan_i8(NT, L, t, grid, ctx->stream, ctx->sdesc.as<ShardDesc>(), nsh,
-                              total_blocks, o));
+    KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream,
+                              ctx->sdesc.as<ShardDesc>(), nsh, total_blocks, o));
   } else {
     for (int s = 0; s < nsh; ++s) {
       if (nblk[s] == 0) continue;
-      KG_CK(ctx, launch_scan_i8(NT, L, t, grid, ctx->stream, ctx->sdesc1.as<ShardDesc>() + s, 1,
-                                nblk[s], o));
+      KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream,
+                                ctx->sdesc1.as<ShardDesc>() + s, 1, nblk[s], o));
     }
   }
   KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
@@ -710,9 +752,14 @@ enum { kPerLaunch = 1, kGraph = 2 };
 
 // Scans every shard (int8 path), appending hits (idx offset by shard idx0,
 // shard index set) and adding pre-filter passes to n_passed[s * t + tr].
+// `overlap`, when given, runs once on the host after the first attempt's device
+// work is enqueued and before the host waits for it (kg_scan issues the next
+// chunk's copy there).
+using Overlap = std::function<int()>;
+
 static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_scan_params& prm,
                           unsigned flags, std::vector<kg_hit>& hits, int64_t* n_passed,
-                          bool stats) {
+                          bool stats, const Overlap* overlap = nullptr) {
   const int nsh = (int)sh.size();
   const int t = ctx->models[sh[0].model]->md.t;
   const int NT = ctx->models[sh[0].model]->NT;
@@ -723,7 +770,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   }
   const double p_thr = std::pow(10.0, -prm.log10_thresh);
   const double Ts = t_star(p_thr);
-  const int64_t bk = i8_block_kmers(NT, t);
+  const int64_t bk = i8_block_kmers(NT, ctx->limbs, t, ctx->tiles);
   std::vector<ShardDesc> sd(nsh), sd1(nsh);
   std::vector<RefDesc> rd(nsh);
   std::vector<int64_t> nblk(nsh);
@@ -814,7 +861,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       kp += sizeof(RefDesc) * nsh;
       memcpy(kp, &o, sizeof(o));
       kp += sizeof(o);
-      const int64_t extra[2] = {(int64_t)flags, (int64_t)ctx->grid_mult};
+      const int64_t extra[2] = {(int64_t)flags, (int64_t)ctx->grid_mult * 64 + ctx->tiles};
       memcpy(kp, extra, 16);
     }
     const bool replay = use_graph && ctx->gexec && key == ctx->graph_key;
@@ -849,6 +896,13 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     }
     KG_CK(ctx, hipMemcpyAsync(cnt.data(), dc, sizeof(unsigned long long) * cnt.size(),
                               hipMemcpyDeviceToHost, ctx->stream));
+    if (overlap && attempt == 0) {
+      const int orc = (*overlap)();
+      if (orc != KG_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return orc;
+      }
+    }
     KG_CK(ctx, hipStreamSynchronize(ctx->stream));
     if ((int64_t)cnt[0] <= ctx->cand_cap) break;
     ctx->cand_cap = (int64_t)cnt[0] + (int64_t)cnt[0] / 4 + 1024;
@@ -900,7 +954,8 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
 
 // Plain FP32 path, one shard at a time.
 static int scan_f32_shard(kg_ctx* ctx, const HShard& s, int sidx, const kg_scan_params& prm,
-                          std::vector<kg_hit>& hits, int64_t* n_passed, bool stats) {
+                          std::vector<kg_hit>& hits, int64_t* n_passed, bool stats,
+                          const Overlap* overlap = nullptr) {
   const ModelDev& M = *ctx->models[s.model];
   const Model& md = M.md;
   const int t = md.t;
@@ -942,6 +997,13 @@ static int scan_f32_shard(kg_ctx* ctx, const HShard& s, int sidx, const kg_scan_
                                M.mask.as<uint64_t>(), ep, o));
     KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     KG_CK(ctx, hipMemcpyAsync(cnt, o.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    if (overlap && attempt == 0) {
+      const int orc = (*overlap)();
+      if (orc != KG_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return orc;
+      }
+    }
     KG_CK(ctx, hipStreamSynchronize(ctx->stream));
     if ((int64_t)cnt[0] <= ctx->cand_cap) break;
     ctx->cand_cap = (int64_t)cnt[0] + (int64_t)cnt[0] / 4 + 1024;
@@ -987,10 +1049,12 @@ static int scan_f32_shard(kg_ctx* ctx, const HShard& s, int sidx, const kg_scan_
 
 static int scan_dispatch(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_scan_params& prm,
                          unsigned flags, std::vector<kg_hit>& hits, int64_t* n_passed,
-                         bool stats) {
-  if (ctx->precision == KG_PREC_MIXED) return scan_i8_shards(ctx, sh, prm, flags, hits, n_passed, stats);
+                         bool stats, const Overlap* overlap = nullptr) {
+  if (ctx->precision == KG_PREC_MIXED)
+    return scan_i8_shards(ctx, sh, prm, flags, hits, n_passed, stats, overlap);
   for (size_t i = 0; i < sh.size(); ++i) {
-    int rc = scan_f32_shard(ctx, sh[i], (int)i, prm, hits, n_passed, stats);
+    int rc = scan_f32_shard(ctx, sh[i], (int)i, prm, hits, n_passed, stats,
+                            i + 1 == sh.size() ? overlap : nullptr);
     if (rc) return rc;
   }
   return KG_OK;
@@ -1045,6 +1109,7 @@ extern "C" int kg_scan_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, in
 extern "C" int kg_scan(kg_ctx* ctx, const uint64_t* bits, int64_t m, int64_t row_words,
                        const kg_scan_params* prm, kg_hit* out, int64_t cap, int64_t* n_out,
                        int64_t* n_passed) {
+  const auto wall0 = std::chrono::steady_clock::now();
   int rc = check_common(ctx, prm);
   if (rc) return rc;
   if (m > 0 && !bits) return fail(ctx, KG_ERR_ARG, "null bits");
@@ -1053,23 +1118,104 @@ extern "C" int kg_scan(kg_ctx* ctx, const uint64_t* bits, int64_t m, int64_t row
     return fail(ctx, KG_ERR_ARG, "row_words must be a multiple of 8 and cover all accessions");
   KG_CK(ctx, hipSetDevice(ctx->device));
   const int t = md.t;
-  // slices of at most 512 MiB of bits
-  const int64_t slice = std::max<int64_t>(1, (int64_t)(512ll << 20) / (row_words * 8));
-  const int64_t rows = std::min<int64_t>(slice, std::max<int64_t>(m, 1));
-  KG_CK(ctx, ctx->stage.ensure((size_t)rows * row_words * 8));
+  // 64 MiB chunks, double-buffered on the device; chunk i+1 is copied on the
+  // copy stream while chunk i is scanned on the compute stream
+  const size_t rowb = (size_t)row_words * 8;
+  const int64_t rows = std::max<int64_t>(1, (int64_t)((64ull << 20) / rowb));
+  const int64_t nchunk = (m + rows - 1) / rows;
+  const size_t cb = (size_t)std::min<int64_t>(rows, std::max<int64_t>(m, 1)) * rowb;
+  KG_CK(ctx, ctx->stage.ensure(cb));
+  KG_CK(ctx, ctx->stage2.ensure(cb));
+  uint64_t* dbuf[2] = {ctx->stage.as<uint64_t>(), ctx->stage2.as<uint64_t>()};
+  // rows in page-locked memory are copied directly, pageable rows through
+  // the bounce buffers
+  bool pinned = false;
+  if (m > 0) {
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, bits) == hipSuccess && attr.type == hipMemoryTypeHost)
+      pinned = true;
+    (void)hipGetLastError();  // a pageable pointer may leave an error behind
+  }
+  if (!pinned && m > 0) {
+    KG_CK(ctx, ctx->bounce[0].ensure(cb));
+    KG_CK(ctx, ctx->bounce[1].ensure(cb));
+  }
+  double copy_ms = 0.0;
+  int64_t ncopied = 0;
+  auto issue = [&](int64_t i) -> int {
+    const int b = (int)(i & 1);
+    const int64_t r0 = i * rows, mm = std::min<int64_t>(rows, m - r0);
+    const size_t bytes = (size_t)mm * rowb;
+    const void* src = bits + r0 * row_words;
+    if (!pinned) {
+      // the bounce buffer's previous copy (chunk i-2) must be done
+      KG_CK(ctx, hipEventSynchronize(ctx->cev[b]));
+      memcpy(ctx->bounce[b].p, src, bytes);
+      src = ctx->bounce[b].p;
+    }
+    KG_CK(ctx, hipEventRecord(ctx->ctev[2 * b], ctx->cstream));
+    KG_CK(ctx, hipMemcpyAsync(dbuf[b], src, bytes, hipMemcpyHostToDevice, ctx->cstream));
+    KG_CK(ctx, hipEventRecord(ctx->ctev[2 * b + 1], ctx->cstream));
+    KG_CK(ctx, hipEventRecord(ctx->cev[b], ctx->cstream));
+    ++ncopied;
+    return KG_OK;
+  };
+  auto copy_time = [&](int b) -> int {
+    float ms = 0.f;
+    KG_CK(ctx, hipEventSynchronize(ctx->ctev[2 * b + 1]));
+    KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ctev[2 * b], ctx->ctev[2 * b + 1]));
+    copy_ms += ms;
+    return KG_OK;
+  };
   std::vector<kg_hit> hits;
   std::vector<int64_t> np(t, 0);
-  for (int64_t s0 = 0; s0 < m; s0 += rows) {
-    const int64_t mm = std::min<int64_t>(rows, m - s0);
-    KG_CK(ctx, hipMemcpyAsync(ctx->stage.p, bits + s0 * row_words, (size_t)mm * row_words * 8,
-                              hipMemcpyHostToDevice, ctx->stream));
-    HShard s{ctx->stage.as<uint64_t>(), mm, row_words, s0, 0};
-    rc = scan_dispatch(ctx, {s}, *prm, 0, hits, np.data(), false);
+  kg_timing acc{};
+  if (nchunk > 0) {
+    rc = issue(0);
     if (rc) return rc;
   }
+  for (int64_t i = 0; i < nchunk; ++i) {
+    const int b = (int)(i & 1);
+    const int64_t r0 = i * rows, mm = std::min<int64_t>(rows, m - r0);
+    KG_CK(ctx, hipStreamWaitEvent(ctx->stream, ctx->cev[b], 0));
+    const Overlap next = [&]() -> int { return (i + 1 < nchunk) ? issue(i + 1) : KG_OK; };
+    HShard s{dbuf[b], mm, row_words, r0, 0};
+    rc = scan_dispatch(ctx, {s}, *prm, 0, hits, np.data(), false, &next);
+    if (rc) {
+      (void)hipStreamSynchronize(ctx->cstream);
+      return rc;
+    }
+    rc = copy_time(b);
+    if (rc) return rc;
+    acc.bulk_ms += ctx->timing.bulk_ms;
+    acc.refine_ms += ctx->timing.refine_ms;
+    acc.total_ms += ctx->timing.total_ms;
+    acc.candidates += ctx->timing.candidates;
+  }
+  KG_CK(ctx, hipStreamSynchronize(ctx->cstream));
+  acc.kmers = m;
+  acc.copy_ms = copy_ms;
+  acc.wall_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+  ctx->timing = acc;
   if (n_passed)
     for (int tr = 0; tr < t; ++tr) n_passed[tr] = np[tr];
   return finish_hits(ctx, hits, out, cap, n_out);
+}
+
+extern "C" int kg_host_alloc(kg_ctx* ctx, size_t bytes, void** h_ptr) {
+  if (!ctx || !h_ptr) return KG_ERR_ARG;
+  *h_ptr = nullptr;
+  KG_CK(ctx, hipSetDevice(ctx->device));
+  KG_CK(ctx, hipHostMalloc(h_ptr, bytes ? bytes : 16, hipHostMallocDefault));
+  return KG_OK;
+}
+
+extern "C" int kg_host_free(kg_ctx* ctx, void* h_ptr) {
+  if (!ctx) return KG_ERR_ARG;
+  if (!h_ptr) return KG_OK;
+  KG_CK(ctx, hipHostFree(h_ptr));
+  return KG_OK;
 }
 
 extern "C" int kg_scan_shards(kg_ctx* ctx, const kg_shard* shards, int32_t nshards,
@@ -1095,6 +1241,68 @@ extern "C" int kg_scan_shards(kg_ctx* ctx, const kg_shard* shards, int32_t nshar
   if (rc) return rc;
   if (n_passed) memcpy(n_passed, np.data(), sizeof(int64_t) * np.size());
   return finish_hits(ctx, hits, out, cap, n_out);
+}
+
+// Launch-shape autotuner: times the bulk scan of `sample` synthetic rows
+// (random bits) with model 0 for each candidate (k-mer tiles per wave x
+// persistent workgroups per CU) and keeps the fastest for this context.
+extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t* grid_mult) {
+  if (!ctx) return KG_ERR_ARG;
+  if (ctx->models.empty()) return fail(ctx, KG_ERR_STATE, "autotune before kg_set_model");
+  if (sample <= 0) return fail(ctx, KG_ERR_ARG, "autotune sample must be positive");
+  KG_CK(ctx, hipSetDevice(ctx->device));
+  const ModelDev& M = *ctx->models[0];
+  const Model& md = M.md;
+  if (ctx->precision == KG_PREC_MIXED) {
+    const int64_t rw = 8 * (int64_t)md.nch_i8;
+    DBuf rows, pop, thr;
+    KG_CK(ctx, rows.ensure((size_t)sample * rw * 8));
+    KG_CK(ctx, pop.ensure((size_t)md.n_acc));
+    KG_CK(ctx, thr.ensure(sizeof(uint32_t)));
+    KG_CK(ctx, hipMemset(pop.p, 0, (size_t)md.n_acc));
+    const uint32_t half = 32768u;  // every bit set with probability 1/2 (16-bit threshold)
+    KG_CK(ctx, hipMemcpy(thr.p, &half, sizeof(half), hipMemcpyHostToDevice));
+    KG_CK(ctx, launch_synth(ctx->stream, rows.as<uint64_t>(), 12345, 0, sample, (uint32_t)md.n_acc,
+                            rw, pop.as<uint8_t>(), thr.as<uint32_t>(), 1, 1));
+    KG_CK(ctx, hipStreamSynchronize(ctx->stream));
+    const kg_scan_params prm{0.2, 6.0, 0.0};
+    const int R_def = i8_tiles_for(M.NT, ctx->limbs, md.t, 0);
+    std::vector<int> Rs{0};
+    if (i8_tiles_for(M.NT, ctx->limbs, md.t, 2) != R_def) Rs.push_back(2);
+    const int gms[3] = {1, 2, 4};
+    const int keep_tiles = ctx->tiles, keep_gm = ctx->grid_mult;
+    double best = 1e300;
+    int best_R = keep_tiles, best_gm = keep_gm;
+    for (int R : Rs)
+      for (int gm : gms) {
+        ctx->tiles = R;
+        ctx->grid_mult = gm;
+        double tmin = 1e300;
+        for (int rep = 0; rep < 4; ++rep) {
+          std::vector<kg_hit> hits;
+          std::vector<int64_t> np(md.t, 0);
+          HShard s{rows.as<uint64_t>(), sample, rw, 0, 0};
+          const int rc = scan_dispatch(ctx, {s}, prm, 0, hits, np.data(), false);
+          if (rc) {
+            ctx->tiles = keep_tiles;
+            ctx->grid_mult = keep_gm;
+            return rc;
+          }
+          if (rep > 0) tmin = std::min(tmin, ctx->timing.bulk_ms);
+        }
+        if (tmin < best) {
+          best = tmin;
+          best_R = R;
+          best_gm = gm;
+        }
+      }
+    ctx->tiles = best_R;
+    ctx->grid_mult = best_gm;
+    drop_graph(ctx);
+  }
+  if (tiles) *tiles = i8_tiles_for(M.NT, ctx->limbs, md.t, ctx->tiles);
+  if (grid_mult) *grid_mult = ctx->grid_mult;
+  return KG_OK;
 }
 
 extern "C" int kg_stats_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m,

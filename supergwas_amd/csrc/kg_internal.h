@@ -68,7 +68,7 @@ struct EpiI8 {
 // sum_b z_b k_tr = z'b' + x'r_c = S.
 constexpr int kColRec = 12;
 constexpr int kTrcRec = 8;
-constexpr int kMaxCols = 48;      // column records the int8 kernel stages
+constexpr int kMaxCols = 88;      // column records the int8 kernel stages (NT <= 16, L = 3)
 
 // One shard of a batched int8 scan: its rows and its null model.  The
 // persistent kernel walks the concatenation of all shards' 256-k-mer blocks.

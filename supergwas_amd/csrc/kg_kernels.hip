@@ -46,14 +46,16 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
-// B operand of MFMA step jj (0..7) of a 512-sample chunk from the lane's four
-// chunk words: operand byte e = 4 d + q takes bit 8 q + 4 (jj & 1) + d of
-// word jj / 2, so each dword is one shift and one mask.
-__device__ __forceinline__ v4i unpack_tile(int jj, const uint32_t (&wd)[4]) {
-  const int sb = 4 * (jj & 1);
-  const uint32_t w = wd[jj >> 1];
+// B operand of one MFMA step from a lane's chunk word w: operand byte
+// e = 4 d + q takes bit 8 q + sb + d (sb = 0 or 4), so each dword is one shift
+// and one mask.  Step js (0..7) of a 512-sample chunk uses word js / 2 and
+// sb = 4 (js % 2).
+__device__ __forceinline__ v4i unpack_word(uint32_t w, int sb) {
   return v4i{(int)((w >> sb) & 0x01010101u), (int)((w >> (sb + 1)) & 0x01010101u),
              (int)((w >> (sb + 2)) & 0x01010101u), (int)((w >> (sb + 3)) & 0x01010101u)};
+}
+__device__ __forceinline__ uint32_t pick_word(const uint32_t (&wd)[4], int i) {
+  return i == 0 ? wd[0] : (i == 1 ? wd[1] : (i == 2 ? wd[2] : wd[3]));
 }
 
 // Wave-aggregated append of candidate records.  Must be reached by every lane
@@ -134,13 +136,15 @@ __device__ __forceinline__ uint32_t classify_i8(int c, double xxM, double bxx, d
 // for chunk C+1 while chunk C computes; each lane's bits and accession mask
 // for chunk C+1 are delivered the same way into its wave's LDS slot; the next
 // block's first chunk is prefetched before the epilogue runs.
-constexpr int kStageSteps = 8;   // MFMA steps per LDS stage (one 512-sample chunk)
+// MFMA steps per LDS stage: a whole 512-sample chunk while two stages of
+// weights fit in LDS, a half or a quarter chunk for the widest tiles.
+__host__ __device__ constexpr int i8_stage_steps(int NT) { return NT <= 8 ? 8 : (NT <= 15 ? 4 : 2); }
 
 template <int NT>
 __device__ __forceinline__ void dma_stage(const v4i* __restrict__ src, v4i* dst, int wave,
                                           int lane) {
-  // the stage's kStageSteps NT blocks of 64 v4i, dealt round-robin to the waves
-  constexpr int NB = kStageSteps * NT;
+  // the stage's SS NT blocks of 64 v4i, dealt round-robin to the waves
+  constexpr int NB = i8_stage_steps(NT) * NT;
 #pragma unroll
   for (int i = 0; i < (NB + kI8Waves - 1) / kI8Waves; ++i) {
     const int qd = i * kI8Waves + wave;
@@ -224,17 +228,19 @@ __device__ __forceinline__ T group_scatter(const T (&v)[R]) {
   }
 }
 
-// Tiles per wave of an instantiation: 4 while the accumulators leave room,
-// 2 for wide tiles or the multi-trait epilogue.
+// Default tiles per wave of an instantiation: 4 while the accumulators leave
+// room, 2 for wide tiles or the multi-trait epilogue.
 __host__ __device__ constexpr int i8_tiles(int NT, int TT) { return (TT == 1 && NT <= 6) ? 4 : 2; }
 
-// L = limbs per column; TT = traits the epilogue carries (1, or kMaxTraits).
-template <int NT, int L, int TT>
+// L = limbs per column; TT = traits the epilogue carries (1, or kMaxTraits);
+// R = k-mer tiles of 16 per wave.
+template <int NT, int L, int TT, int R>
 __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
                                                         int nshards, int64_t total_blocks,
                                                         ScanOut out) {
-  constexpr int R = i8_tiles(NT, TT);
-  constexpr int FRAG2 = kStageSteps * NT * 64;       // v4i per 512-sample chunk
+  constexpr int SS = i8_stage_steps(NT);             // MFMA steps per stage
+  constexpr int SPC = 8 / SS;                        // stages per 512-sample chunk
+  constexpr int FRAG2 = SS * NT * 64;                // v4i per stage
   constexpr int J = (4 * NT) / L;                    // base columns per lane group
   constexpr int JSP = (TT + 4) / 4;                  // slots that may hold r_c or d columns
   static_assert(4 * J <= kMaxCols, "column records exceed the LDS table");
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
   __shared__ v4i buf0[FRAG2];
   __shared__ v4i buf1[FRAG2];
   __shared__ v4i bitsbuf[kI8Waves * (R + 1) * 64];   // [wave][r | mask][lane] next chunk
-  __shared__ double colrec_lds[kMaxCols * kColRec];
+  __shared__ double colrec_lds[4 * J * kColRec];
   __shared__ double trc_lds[kMaxTraits * kTrcRec];
   __shared__ unsigned int wg_pass[kMaxTraits];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -281,15 +287,15 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 #pragma unroll
     for (int r = 0; r < R; ++r) cnt[r] = 0;
 
-    __builtin_amdgcn_s_waitcnt(0);  // chunk 0 weights and bits have landed
+    __builtin_amdgcn_s_waitcnt(0);  // stage 0 weights and chunk 0 bits have landed
     __syncthreads();
-    for (int C = 0; C < nchunks; ++C) {
-      const int h = C & 1;
-      const v4i* cur = h ? buf1 : buf0;
-      if (C + 1 < nchunks) dma_stage<NT>(wts + (size_t)(C + 1) * FRAG2, h ? buf0 : buf1, wave, lane);
-      // this chunk's 128 bits per tile (masked) and their popcount
-      uint32_t wd[R][4];
-      {
+    uint32_t wd[R][4];  // the current chunk's 128 bits per tile (masked)
+    for (int S = 0; S < nchunks * SPC; ++S) {
+      const int C = S / SPC, h = S % SPC;  // chunk, stage within the chunk
+      const v4i* cur = (S & 1) ? buf1 : buf0;
+      if (S + 1 < nchunks * SPC)
+        dma_stage<NT>(wts + (size_t)(S + 1) * FRAG2, (S & 1) ? buf0 : buf1, wave, lane);
+      if (h == 0) {
         const v4i mk = mybits[R * 64 + lane];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -306,37 +312,38 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
           dma_mask(D.mask, C + 1, g, mybits + R * 64);
         }
       }
-      // Software pipeline over the chunk's 8 NT weight fragments q = NT jj + t:
+      // Software pipeline over the stage's SS NT weight fragments q = NT jj + t:
       // fragment q+PD is read from LDS while q's MFMAs run, and step jj+1's
       // bit operands are unpacked beside step jj's MFMAs.  The scheduling
       // barrier after each fragment keeps the compiler from regrouping the
       // LDS reads next to their first use.
-      constexpr int NQ = kStageSteps * NT;
+      constexpr int NQ = SS * NT;
       constexpr int PD = 4;  // fragments read ahead
       v4i bf[R], bn[R], af[PD + 1];
 #pragma unroll
-      for (int r = 0; r < R; ++r) bf[r] = unpack_tile(0, wd[r]);
+      for (int r = 0; r < R; ++r) bf[r] = unpack_word(pick_word(wd[r], (h * SS) >> 1), 0);
 #pragma unroll
       for (int i = 0; i < PD; ++i) af[i] = cur[i * 64 + lane];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int jj = q / NT, t = q % NT;
-        if (jj + 1 < kStageSteps) {  // tile r's next operands beside fragment t = r mod NT
+        if (jj + 1 < SS) {  // tile r's next operands beside fragment t = r mod NT
 #pragma unroll
-          for (int r = t; r < R; r += NT) bn[r] = unpack_tile(jj + 1, wd[r]);
+          for (int r = t; r < R; r += NT)
+            bn[r] = unpack_word(pick_word(wd[r], (h * SS + jj + 1) >> 1), 4 * ((jj + 1) & 1));
         }
         if (q + PD < NQ) af[(q + PD) % (PD + 1)] = cur[(q + PD) * 64 + lane];
 #pragma unroll
         for (int r = 0; r < R; ++r)
           acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[q % (PD + 1)], bf[r], acc[r][t], 0,
                                                             0, 0);
-        if (t == NT - 1 && jj + 1 < kStageSteps) {
+        if (t == NT - 1 && jj + 1 < SS) {
 #pragma unroll
           for (int r = 0; r < R; ++r) bf[r] = bn[r];
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      // next chunk's weights and bits landed in every wave's share
+      // next stage's weights (and next chunk's bits) landed in every wave's share
       __builtin_amdgcn_s_waitcnt(0);
       lds_barrier();
     }
@@ -734,38 +741,56 @@ __global__ void synth_bits(uint64_t* __restrict__ out, uint64_t seed, int64_t km
 // ----------------------------------------------------------------------------
 // Launchers (host).  The tile count NT is a template parameter.
 
-template <int NT, int L, int TT>
+template <int NT, int L, int TT, int R>
 static hipError_t launch_i8_tr(dim3 grid, hipStream_t s, const ShardDesc* sd, int nshards,
                                int64_t total_blocks, const ScanOut& o) {
-  auto kfn = scan_i8<NT, L, TT>;
+  auto kfn = scan_i8<NT, L, TT, R>;
   hipLaunchKernelGGL(kfn, grid, dim3(kI8Threads), 0, s, sd, nshards, total_blocks, o);
   return hipGetLastError();
 }
 
-// k-mers per workgroup block of the instantiation for (NT, t)
-int64_t i8_block_kmers(int NT, int t) {
-  return (int64_t)kI8Waves * 16 * i8_tiles(NT, t > 1 ? kMaxTraits : 1);
+// Tiles per wave actually used for (NT, t) and a requested count (0 = default).
+// Besides the default, single-trait tiles of at most 96 columns at 3 limbs
+// can run 2 tiles per wave.
+int i8_tiles_for(int NT, int L, int t, int R_req) {
+  const int def = i8_tiles(NT, t > 1 ? kMaxTraits : 1);
+  if (R_req == 2 && def == 4 && L == 3) return 2;
+  return def;
+}
+
+// k-mers per workgroup block
+int64_t i8_block_kmers(int NT, int L, int t, int R_req) {
+  return (int64_t)kI8Waves * 16 * i8_tiles_for(NT, L, t, R_req);
 }
 
 // Column capacity of NT tiles at L limbs: 4 lane groups x floor(4 NT / L).
 int i8_column_capacity(int NT, int L) { return 4 * ((4 * NT) / L); }
 
-// NT ranges per limb count cover up to kMaxCols base columns.
-hipError_t launch_scan_i8(int NT, int L, int t, int grid, hipStream_t s, const ShardDesc* sd,
-                          int nshards, int64_t total_blocks, const ScanOut& o) {
+// NT ranges per limb count: 1-3 (L = 1), 1-6 (L = 2), 1-16 (L = 3).
+hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t s,
+                          const ShardDesc* sd, int nshards, int64_t total_blocks,
+                          const ScanOut& o) {
   dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
   const int TT = t > 1 ? kMaxTraits : 1;
-  switch ((NT * 4 + L) * 16 + TT) {
-#define KG_CASE(X, Y)                                                                   \
-  case (X * 4 + Y) * 16 + 1:                                                            \
-    return launch_i8_tr<X, Y, 1>(g, s, sd, nshards, total_blocks, o);                   \
-  case (X * 4 + Y) * 16 + kMaxTraits:                                                   \
-    return launch_i8_tr<X, Y, kMaxTraits>(g, s, sd, nshards, total_blocks, o);
+  const int R = i8_tiles_for(NT, L, t, R_req);
+  switch (((NT * 4 + L) * 16 + TT) * 8 + R) {
+#define KG_CASE(X, Y)                                                                       \
+  case ((X * 4 + Y) * 16 + 1) * 8 + i8_tiles(X, 1):                                         \
+    return launch_i8_tr<X, Y, 1, i8_tiles(X, 1)>(g, s, sd, nshards, total_blocks, o);       \
+  case ((X * 4 + Y) * 16 + kMaxTraits) * 8 + i8_tiles(X, kMaxTraits):                       \
+    return launch_i8_tr<X, Y, kMaxTraits, i8_tiles(X, kMaxTraits)>(g, s, sd, nshards,       \
+                                                                    total_blocks, o);
+#define KG_CASE_R2(X)                                                                       \
+  case ((X * 4 + 3) * 16 + 1) * 8 + 2:                                                      \
+    return launch_i8_tr<X, 3, 1, 2>(g, s, sd, nshards, total_blocks, o);
     KG_CASE(1, 1) KG_CASE(2, 1) KG_CASE(3, 1)
     KG_CASE(1, 2) KG_CASE(2, 2) KG_CASE(3, 2) KG_CASE(4, 2) KG_CASE(5, 2) KG_CASE(6, 2)
     KG_CASE(1, 3) KG_CASE(2, 3) KG_CASE(3, 3) KG_CASE(4, 3) KG_CASE(5, 3) KG_CASE(6, 3)
-    KG_CASE(7, 3) KG_CASE(8, 3)
+    KG_CASE(7, 3) KG_CASE(8, 3) KG_CASE(9, 3) KG_CASE(10, 3) KG_CASE(11, 3) KG_CASE(12, 3)
+    KG_CASE(13, 3) KG_CASE(14, 3) KG_CASE(15, 3) KG_CASE(16, 3)
+    KG_CASE_R2(1) KG_CASE_R2(2) KG_CASE_R2(3) KG_CASE_R2(4) KG_CASE_R2(5) KG_CASE_R2(6)
 #undef KG_CASE
+#undef KG_CASE_R2
     default:
       return hipErrorInvalidValue;
   }

@@ -195,6 +195,71 @@ def test_multi_trait_equals_single_trait_runs(built):
         assert np.allclose(got["T"], ref["T"][exp_idx], rtol=1e-9, atol=0)
 
 
+@pytest.mark.parametrize("p,t,n_acc", [(40, 1, 1100), (56, 1, 777), (64, 8, 1100), (80, 4, 1300)])
+def test_wide_tiles_match_oracle(built, p, t, n_acc):
+    """Weight tiles past 128 columns (half- and quarter-chunk LDS stages) and the
+    multi-trait epilogue: p + t up to 84 base columns at 3 limbs."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    m = 3000
+    ds = synth.make_dataset(seed=40 + p, n_acc=n_acc, n_missing=37, npop=5, p=p, m=m,
+                            n_causal=10, effect_sd=0.5, traits=t, trait_corr=0.2)
+    bits = ds.fill_bits(0, m)
+    prm = ScanParams(0.1, 2.0, 4.0)
+    with KgwasHip() as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.Y)
+        hits, npass = k.scan(bits, prm)
+        dev = k.to_device(bits)
+        st = k.stats_device(dev.ptr, m, bits.shape[1])
+        dev.free()
+    print(f"[wide] p={p} t={t} n={ds.n} hits={len(hits)} passed={list(npass)}", flush=True)
+    for tr in range(t):
+        ref = oracle.scan_f64(bits, ds.idxs, ds.Y[:, tr], ds.U, 0.1)
+        exp_idx, _ = expected_hits(ref, 2.0, 4.0)
+        got = hits[hits["trait"] == tr]
+        assert npass[tr] == ref["n_passed"], tr
+        assert np.array_equal(got["idx"], exp_idx), tr
+        assert np.allclose(got["T"], ref["T"][exp_idx], rtol=1e-9, atol=0)
+        assert np.array_equal(st["c"], ref["c"])
+        # the bulk value's 3-limb quantisation error grows with the column count
+        # and shrinks with n (1.4e-5 seen at p=56, n=740); decisions above are
+        # certified regardless and reported values are FP64-refined
+        Tb = T_from(st["xxM"], st["xyM"][:, tr], oracle.yyM(ds.Y[:, tr], ds.U), ds.n)
+        assert rel_err_T(Tb, ref["T"]).max() <= 1e-4, tr
+
+
+def test_streamed_scan_matches_device_scan(built):
+    """kg_scan streams host rows in 64 MiB chunks (copy of chunk i+1 overlapping
+    the scan of chunk i), from pageable memory through bounce buffers and from
+    page-locked memory directly; both equal the one-shot device scan."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    m = 600_000  # 2048 accessions: 256 B rows, 3 chunks
+    ds = synth.make_dataset(seed=11, n_acc=2048, n_missing=30, npop=4, p=16, m=m, n_causal=20,
+                            effect_sd=0.4)
+    bits = ds.fill_bits(0, m)
+    prm = ScanParams(0.2, 5.0, 9.0)
+    with KgwasHip() as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        dev = k.to_device(bits)
+        h_dev, n_dev = k.scan_device(dev.ptr, m, bits.shape[1], prm)
+        dev.free()
+        h_pg, n_pg = k.scan(bits, prm)
+        tm_pg = k.timing()
+        pin = k.host_alloc(bits.shape)
+        pin[:] = bits
+        h_pin, n_pin = k.scan(pin, prm)
+        tm_pin = k.timing()
+        k.host_free(pin)
+    print(f"[stream] hits={len(h_dev)} passed={int(n_dev[0])} pageable copy {tm_pg['copy_ms']:.1f} ms"
+          f" wall {tm_pg['wall_ms']:.1f} ms; pinned copy {tm_pin['copy_ms']:.1f} ms"
+          f" wall {tm_pin['wall_ms']:.1f} ms", flush=True)
+    assert len(h_dev) > 0
+    for h, n in ((h_pg, n_pg), (h_pin, n_pin)):
+        assert np.array_equal(n, n_dev)
+        assert np.array_equal(h["idx"], h_dev["idx"])
+        assert np.array_equal(h["T"], h_dev["T"])
+    assert tm_pin["kmers"] == m and tm_pin["copy_ms"] > 0 and tm_pin["wall_ms"] > 0
+
+
 def _shard_setup(k, nsh=5, seed=30):
     """Shards of uneven size, each with its own phenotyped-sample mask (own model)."""
     sizes = [1, 300, 0, 5003, 2500][:nsh]
