@@ -161,6 +161,16 @@ struct kg_ctx {
   // kg_scan streaming: copy stream, copy-done events and copy timing events
   // per staging buffer, page-locked bounce buffers for pageable rows
   hipStream_t cstream = nullptr;
+  // descriptor tables of recent non-graph int8 scans (a streamed scan
+  // alternates between two), so repeated shapes skip the upload
+  struct DescSlot {
+    std::vector<char> key;
+    DBuf sd, sd1, rd;
+    uint64_t used = 0;
+  };
+  DescSlot dslot[4];
+  uint64_t dclock = 0;
+  HBuf hcnt;  // page-locked counter read-back
   hipEvent_t cev[2] = {nullptr, nullptr};
   hipEvent_t ctev[4] = {nullptr, nullptr, nullptr, nullptr};
   HBuf bounce[2];
@@ -726,23 +736,24 @@ static int check_shard(kg_ctx* ctx, const HShard& s) {
 
 // Device part of a batched int8 scan: zero counters, bulk kernel(s), refine.
 static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks, bool per_launch,
-                      const std::vector<int64_t>& nblk, const ScanOut& o, int ref_blocks) {
+                      const std::vector<int64_t>& nblk, const ScanOut& o, int ref_blocks,
+                      const ShardDesc* d_sd, const ShardDesc* d_sd1, const RefDesc* d_rd) {
   const int L = ctx->limbs;
   const int grid = ctx->cus * ctx->grid_mult;
   KG_CK(ctx, launch_zero(ctx->stream, o.counters, (int)ctx->counters_len));
   KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (!per_launch) {
-    KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream,
-                              ctx->sdesc.as<ShardDesc>(), nsh, total_blocks, o));
+    KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream, d_sd, nsh, total_blocks,
+                              o));
   } else {
     for (int s = 0; s < nsh; ++s) {
       if (nblk[s] == 0) continue;
-      KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream,
-                                ctx->sdesc1.as<ShardDesc>() + s, 1, nblk[s], o));
+      KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream, d_sd1 + s, 1, nblk[s],
+                                o));
     }
   }
   KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-  KG_CK(ctx, launch_refine(ctx->stream, o.cand, o.counters, o.cap, ctx->rdesc.as<RefDesc>(),
+  KG_CK(ctx, launch_refine(ctx->stream, o.cand, o.counters, o.cap, d_rd,
                            ctx->res.as<double>(), ref_blocks));
   KG_CK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   return KG_OK;
@@ -843,6 +854,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   const bool use_graph = (flags & kGraph) != 0 && !stats;
   const int ref_blocks = ctx->cus * 4;
   std::vector<unsigned long long> cnt(ctx->counters_len);
+  KG_CK(ctx, ctx->hcnt.ensure(sizeof(unsigned long long) * cnt.size()));
   for (int attempt = 0; attempt < 3; ++attempt) {
     o.cand = ctx->cand.as<Cand>();
     o.counters = dc;
@@ -865,18 +877,52 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       memcpy(kp, extra, 16);
     }
     const bool replay = use_graph && ctx->gexec && key == ctx->graph_key;
-    if (!replay) {
-      KG_CK(ctx, hipMemcpy(ctx->sdesc.p, sd.data(), sizeof(ShardDesc) * nsh, hipMemcpyHostToDevice));
-      KG_CK(ctx, hipMemcpy(ctx->sdesc1.p, sd1.data(), sizeof(ShardDesc) * nsh,
-                           hipMemcpyHostToDevice));
-      KG_CK(ctx, hipMemcpy(ctx->rdesc.p, rd.data(), sizeof(RefDesc) * nsh, hipMemcpyHostToDevice));
+    const ShardDesc* d_sd = ctx->sdesc.as<ShardDesc>();
+    const ShardDesc* d_sd1 = ctx->sdesc1.as<ShardDesc>();
+    const RefDesc* d_rd = ctx->rdesc.as<RefDesc>();
+    if (use_graph) {
+      if (!replay) {
+        KG_CK(ctx, hipMemcpy(ctx->sdesc.p, sd.data(), sizeof(ShardDesc) * nsh, hipMemcpyHostToDevice));
+        KG_CK(ctx, hipMemcpy(ctx->sdesc1.p, sd1.data(), sizeof(ShardDesc) * nsh,
+                             hipMemcpyHostToDevice));
+        KG_CK(ctx, hipMemcpy(ctx->rdesc.p, rd.data(), sizeof(RefDesc) * nsh, hipMemcpyHostToDevice));
+      }
+    } else {
+      // descriptor tables: reuse an identical recent upload (every earlier scan
+      // has completed, so no kernel still reads a slot that gets replaced)
+      const size_t b0 = sizeof(ShardDesc) * nsh, b2 = sizeof(RefDesc) * nsh;
+      std::vector<char> dk(2 * b0 + b2);
+      memcpy(dk.data(), sd.data(), b0);
+      memcpy(dk.data() + b0, sd1.data(), b0);
+      memcpy(dk.data() + 2 * b0, rd.data(), b2);
+      kg_ctx::DescSlot* slot = nullptr;
+      for (auto& e : ctx->dslot)
+        if (e.key == dk) slot = &e;
+      if (!slot) {
+        slot = &ctx->dslot[0];
+        for (auto& e : ctx->dslot)
+          if (e.used < slot->used) slot = &e;
+        slot->key.clear();
+        KG_CK(ctx, slot->sd.ensure(b0));
+        KG_CK(ctx, slot->sd1.ensure(b0));
+        KG_CK(ctx, slot->rd.ensure(b2));
+        KG_CK(ctx, hipMemcpy(slot->sd.p, sd.data(), b0, hipMemcpyHostToDevice));
+        KG_CK(ctx, hipMemcpy(slot->sd1.p, sd1.data(), b0, hipMemcpyHostToDevice));
+        KG_CK(ctx, hipMemcpy(slot->rd.p, rd.data(), b2, hipMemcpyHostToDevice));
+        slot->key = std::move(dk);
+      }
+      slot->used = ++ctx->dclock;
+      d_sd = slot->sd.as<ShardDesc>();
+      d_sd1 = slot->sd1.as<ShardDesc>();
+      d_rd = slot->rd.as<RefDesc>();
     }
     KG_CK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (use_graph) {
       if (!replay) {
         drop_graph(ctx);
         KG_CK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_i8(ctx, NT, t, nsh, total_blocks, per_launch, nblk, o, ref_blocks);
+        int rc = enqueue_i8(ctx, NT, t, nsh, total_blocks, per_launch, nblk, o, ref_blocks, d_sd,
+                            d_sd1, d_rd);
         hipGraph_t g = nullptr;
         hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
         if (rc != KG_OK) {
@@ -891,10 +937,11 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       KG_CK(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
       KG_CK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     } else {
-      int rc = enqueue_i8(ctx, NT, t, nsh, total_blocks, per_launch, nblk, o, ref_blocks);
+      int rc = enqueue_i8(ctx, NT, t, nsh, total_blocks, per_launch, nblk, o, ref_blocks, d_sd,
+                          d_sd1, d_rd);
       if (rc != KG_OK) return rc;
     }
-    KG_CK(ctx, hipMemcpyAsync(cnt.data(), dc, sizeof(unsigned long long) * cnt.size(),
+    KG_CK(ctx, hipMemcpyAsync(ctx->hcnt.p, dc, sizeof(unsigned long long) * cnt.size(),
                               hipMemcpyDeviceToHost, ctx->stream));
     if (overlap && attempt == 0) {
       const int orc = (*overlap)();
@@ -904,6 +951,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       }
     }
     KG_CK(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(cnt.data(), ctx->hcnt.p, sizeof(unsigned long long) * cnt.size());
     if ((int64_t)cnt[0] <= ctx->cand_cap) break;
     ctx->cand_cap = (int64_t)cnt[0] + (int64_t)cnt[0] / 4 + 1024;
     KG_CK(ctx, ctx->cand.ensure(sizeof(Cand) * ctx->cand_cap));
