@@ -54,9 +54,6 @@ __device__ __forceinline__ v4i unpack_word(uint32_t w, int sb) {
   return v4i{(int)((w >> sb) & 0x01010101u), (int)((w >> (sb + 1)) & 0x01010101u),
              (int)((w >> (sb + 2)) & 0x01010101u), (int)((w >> (sb + 3)) & 0x01010101u)};
 }
-__device__ __forceinline__ uint32_t pick_word(const uint32_t (&wd)[4], int i) {
-  return i == 0 ? wd[0] : (i == 1 ? wd[1] : (i == 2 ? wd[2] : wd[3]));
-}
 
 // Wave-aggregated append of candidate records.  Must be reached by every lane
 // of the wave (uniform control flow).
@@ -290,62 +287,66 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     __builtin_amdgcn_s_waitcnt(0);  // stage 0 weights and chunk 0 bits have landed
     __syncthreads();
     uint32_t wd[R][4];  // the current chunk's 128 bits per tile (masked)
-    for (int S = 0; S < nchunks * SPC; ++S) {
-      const int C = S / SPC, h = S % SPC;  // chunk, stage within the chunk
-      const v4i* cur = (S & 1) ? buf1 : buf0;
-      if (S + 1 < nchunks * SPC)
-        dma_stage<NT>(wts + (size_t)(S + 1) * FRAG2, (S & 1) ? buf0 : buf1, wave, lane);
-      if (h == 0) {
-        const v4i mk = mybits[R * 64 + lane];
+    for (int C = 0; C < nchunks; ++C) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const v4i v = mybits[r * 64 + lane];
-          wd[r][0] = (uint32_t)(v.x & mk.x);
-          wd[r][1] = (uint32_t)(v.y & mk.y);
-          wd[r][2] = (uint32_t)(v.z & mk.z);
-          wd[r][3] = (uint32_t)(v.w & mk.w);
-          cnt[r] += __popc(wd[r][0]) + __popc(wd[r][1]) + __popc(wd[r][2]) + __popc(wd[r][3]);
+      for (int hc = 0; hc < SPC; ++hc) {  // stages of the chunk
+        const int S = C * SPC + hc;
+        const bool odd = (SPC == 1) ? (C & 1) != 0 : (hc & 1) != 0;
+        const v4i* cur = odd ? buf1 : buf0;
+        if (S + 1 < nchunks * SPC)
+          dma_stage<NT>(wts + (size_t)(S + 1) * FRAG2, odd ? buf0 : buf1, wave, lane);
+        if (hc == 0) {
+          const v4i mk = mybits[R * 64 + lane];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const v4i v = mybits[r * 64 + lane];
+            wd[r][0] = (uint32_t)(v.x & mk.x);
+            wd[r][1] = (uint32_t)(v.y & mk.y);
+            wd[r][2] = (uint32_t)(v.z & mk.z);
+            wd[r][3] = (uint32_t)(v.w & mk.w);
+            cnt[r] += __popc(wd[r][0]) + __popc(wd[r][1]) + __popc(wd[r][2]) + __popc(wd[r][3]);
+          }
+          __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's bits reads are done
+          if (C + 1 < nchunks) {
+            dma_bits<R>(rowp, C + 1, mybits, lane);
+            dma_mask(D.mask, C + 1, g, mybits + R * 64);
+          }
         }
-        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's bits reads are done
-        if (C + 1 < nchunks) {
-          dma_bits<R>(rowp, C + 1, mybits, lane);
-          dma_mask(D.mask, C + 1, g, mybits + R * 64);
+        // Software pipeline over the stage's SS NT weight fragments q = NT jj + t:
+        // fragment q+PD is read from LDS while q's MFMAs run, and step jj+1's
+        // bit operands are unpacked beside step jj's MFMAs.  The scheduling
+        // barrier after each fragment keeps the compiler from regrouping the
+        // LDS reads next to their first use.
+        constexpr int NQ = SS * NT;
+        constexpr int PD = 4;  // fragments read ahead
+        v4i bf[R], bn[R], af[PD + 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) bf[r] = unpack_word(wd[r][(hc * SS) >> 1], 0);
+#pragma unroll
+        for (int i = 0; i < PD; ++i) af[i] = cur[i * 64 + lane];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int jj = q / NT, t = q % NT;
+          if (jj + 1 < SS) {  // tile r's next operands beside fragment t = r mod NT
+#pragma unroll
+            for (int r = t; r < R; r += NT)
+              bn[r] = unpack_word(wd[r][(hc * SS + jj + 1) >> 1], 4 * ((jj + 1) & 1));
+          }
+          if (q + PD < NQ) af[(q + PD) % (PD + 1)] = cur[(q + PD) * 64 + lane];
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[q % (PD + 1)], bf[r], acc[r][t],
+                                                              0, 0, 0);
+          if (t == NT - 1 && jj + 1 < SS) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) bf[r] = bn[r];
+          }
+          __builtin_amdgcn_sched_barrier(0);
         }
+        // next stage's weights (and next chunk's bits) landed in every wave's share
+        __builtin_amdgcn_s_waitcnt(0);
+        lds_barrier();
       }
-      // Software pipeline over the stage's SS NT weight fragments q = NT jj + t:
-      // fragment q+PD is read from LDS while q's MFMAs run, and step jj+1's
-      // bit operands are unpacked beside step jj's MFMAs.  The scheduling
-      // barrier after each fragment keeps the compiler from regrouping the
-      // LDS reads next to their first use.
-      constexpr int NQ = SS * NT;
-      constexpr int PD = 4;  // fragments read ahead
-      v4i bf[R], bn[R], af[PD + 1];
-#pragma unroll
-      for (int r = 0; r < R; ++r) bf[r] = unpack_word(pick_word(wd[r], (h * SS) >> 1), 0);
-#pragma unroll
-      for (int i = 0; i < PD; ++i) af[i] = cur[i * 64 + lane];
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int jj = q / NT, t = q % NT;
-        if (jj + 1 < SS) {  // tile r's next operands beside fragment t = r mod NT
-#pragma unroll
-          for (int r = t; r < R; r += NT)
-            bn[r] = unpack_word(pick_word(wd[r], (h * SS + jj + 1) >> 1), 4 * ((jj + 1) & 1));
-        }
-        if (q + PD < NQ) af[(q + PD) % (PD + 1)] = cur[(q + PD) * 64 + lane];
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-          acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[q % (PD + 1)], bf[r], acc[r][t], 0,
-                                                            0, 0);
-        if (t == NT - 1 && jj + 1 < SS) {
-#pragma unroll
-          for (int r = 0; r < R; ++r) bf[r] = bn[r];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-      }
-      // next stage's weights (and next chunk's bits) landed in every wave's share
-      __builtin_amdgcn_s_waitcnt(0);
-      lds_barrier();
     }
 
     // this shard's epilogue constants into LDS while no direct-to-LDS load is
