@@ -260,6 +260,24 @@ def test_streamed_scan_matches_device_scan(built):
     assert tm_pin["kmers"] == m and tm_pin["copy_ms"] > 0 and tm_pin["wall_ms"] > 0
 
 
+def test_autotune_keeps_results(built):
+    """kg_autotune only changes the launch shape: results are identical before and after."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    kw, m, corr, l10, l2d = CASES["n2048_p32"]
+    ds = dataset("n2048_p32", **kw)
+    bits = ds.fill_bits(0, m)
+    prm = ScanParams(corr, l10, l2d)
+    with KgwasHip() as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        h0, n0 = k.scan(bits, prm)
+        tiles, gm = k.autotune(300_000)
+        h1, n1 = k.scan(bits, prm)
+    print(f"[autotune] tiles/wave {tiles}, workgroups/CU {gm}", flush=True)
+    assert tiles in (2, 4) and gm in (1, 2, 4)
+    assert np.array_equal(n0, n1)
+    assert np.array_equal(h0["idx"], h1["idx"]) and np.array_equal(h0["T"], h1["T"])
+
+
 def _shard_setup(k, nsh=5, seed=30):
     """Shards of uneven size, each with its own phenotyped-sample mask (own model)."""
     sizes = [1, 300, 0, 5003, 2500][:nsh]
