@@ -36,13 +36,25 @@ extern "C" {
 
 #define KG_HIT_2D 1u       /* p <= 10^-log10_thresh_2d (src/kgwas.cpp:251-254) */
 
+/* Pre-filter screen of KG_PREC_MIXED scans (src/kgwas.cpp:183-218 filters
+ * before the projections; the screen does the same on the device): a cheap
+ * bulk pass drops the k-mers whose pre-filter fails for certain, and the full
+ * kernel runs on the rest.  Counts and hits are identical either way.
+ * AUTO screens whenever corr_threshold > 0 and keeps screening while at most
+ * half of the k-mers survive; OFF never screens; ON always screens where the
+ * model allows (n_acc <= 8192 accessions). */
+#define KG_SCREEN_AUTO 0
+#define KG_SCREEN_OFF 1
+#define KG_SCREEN_ON 2
+
 typedef struct kg_ctx kg_ctx;
 
 typedef struct kg_opts {
   int32_t device;      /* HIP device ordinal */
   int32_t precision;   /* KG_PREC_MIXED or KG_PREC_FP32 */
   int32_t limbs;       /* int8 limbs per weight column for KG_PREC_MIXED (2 or 3; 0 = 3) */
-  int32_t reserved[5];
+  int32_t screen;      /* KG_SCREEN_AUTO (0), KG_SCREEN_OFF or KG_SCREEN_ON */
+  int32_t reserved[4];
 } kg_opts;
 
 /* Thresholds of one scan (src/kgwas.cpp:28-30, 101-112, 230-232). */
@@ -68,13 +80,15 @@ typedef struct kg_hit {
 
 /* Timings of the last scan on the context's stream (HIP events), ms. */
 typedef struct kg_timing {
-  double bulk_ms;       /* bulk scan kernel */
+  double bulk_ms;       /* bulk scan kernel (the full kernel; after the screen when screened) */
   double refine_ms;     /* FP64 refinement kernel (0 for KG_PREC_FP32) */
   double total_ms;      /* first to last device operation of the scan */
   int64_t candidates;   /* k-mers sent to refinement / host decision */
   int64_t kmers;        /* k-mers scanned */
   double copy_ms;       /* kg_scan: host-to-device copies (copy stream, summed) */
   double wall_ms;       /* kg_scan: host wall time of the whole call */
+  double screen_ms;     /* pre-filter screen kernel + its descriptor fix-up (0 if unscreened) */
+  int64_t survivors;    /* k-mers the screen passed to the full kernel (= kmers if unscreened) */
 } kg_timing;
 
 /* Device selection, stream creation.  Replaces the OpenBLAS/OpenMP thread

@@ -19,6 +19,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libkgwas_hip.so")
 
 KG_OK, KG_ERR_ARG, KG_ERR_DEVICE, KG_ERR_STATE, KG_ERR_CAPACITY = 0, 1, 2, 3, 4
 PRECISIONS = {"mixed": 0, "fp32": 1}
+# pre-filter screen of mixed scans (KG_SCREEN_*)
+SCREEN = {"auto": 0, "off": 1, "on": 2}
 HIT_2D = 1
 
 HIT_DTYPE = np.dtype([("idx", np.int64), ("trait", np.int32), ("c", np.int32),
@@ -35,7 +37,8 @@ class KgError(RuntimeError):
 
 class _Opts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("limbs", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("limbs", ctypes.c_int32), ("screen", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class ScanParams(ctypes.Structure):
@@ -61,7 +64,8 @@ class _Timing(ctypes.Structure):
     _fields_ = [("bulk_ms", ctypes.c_double), ("refine_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("candidates", ctypes.c_int64),
                 ("kmers", ctypes.c_int64), ("copy_ms", ctypes.c_double),
-                ("wall_ms", ctypes.c_double)]
+                ("wall_ms", ctypes.c_double), ("screen_ms", ctypes.c_double),
+                ("survivors", ctypes.c_int64)]
 
 
 _lib = None
@@ -148,9 +152,9 @@ class DeviceArray:
 class KgwasHip:
     """One device context (kg_create) with one null model (kg_set_model)."""
 
-    def __init__(self, device=0, precision="mixed", limbs=3):
+    def __init__(self, device=0, precision="mixed", limbs=3, screen="auto"):
         L = lib()
-        opts = _Opts(device, PRECISIONS[precision], limbs)
+        opts = _Opts(device, PRECISIONS[precision], limbs, SCREEN[screen])
         h = _vp()
         rc = L.kg_create(ctypes.byref(h), ctypes.byref(opts))
         self._h = h
@@ -292,7 +296,8 @@ class KgwasHip:
         t = _Timing()
         self._check(lib().kg_last_timing(self._h, ctypes.byref(t)))
         return dict(bulk_ms=t.bulk_ms, refine_ms=t.refine_ms, total_ms=t.total_ms,
-                    candidates=t.candidates, kmers=t.kmers, copy_ms=t.copy_ms, wall_ms=t.wall_ms)
+                    candidates=t.candidates, kmers=t.kmers, copy_ms=t.copy_ms, wall_ms=t.wall_ms,
+                    screen_ms=t.screen_ms, survivors=t.survivors)
 
     def autotune(self, sample=2_000_000):
         """Pick k-mer tiles per wave and workgroups per CU by timing `sample` random

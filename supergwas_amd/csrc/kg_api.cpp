@@ -26,7 +26,14 @@
 namespace kg {
 hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t s,
                           const ShardDesc* sd, int nshards, int64_t total_blocks,
-                          const ScanOut& o);
+                          const ScanOut& o, const int64_t* total_dev);
+int64_t screen_block_kmers();
+int screen_max_chunks();
+int screen_wgs_per_cu(int t, int nch);
+hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
+                         int nshards, int64_t total_blocks);
+hipError_t launch_screen_fixup(hipStream_t s, const ScreenDesc* ss, ShardDesc* sd2, int nsh,
+                               int64_t bk, int64_t* total);
 int64_t i8_block_kmers(int NT, int L, int t, int R_req);
 int i8_tiles_for(int NT, int L, int t, int R_req);
 int i8_column_capacity(int NT, int L);
@@ -145,6 +152,11 @@ struct ModelDev {
   int NT = 0, nz = 0, nb = 0;
   DBuf wts, mask, rbar, B64, bpfull, colrec, trc;  // int8 path
   DBuf wf32, uyf;                                                     // FP32 path
+  // pre-filter screen: two int8 limbs of y - ybar per trait (absent when the
+  // model is too wide for the screen's LDS weight table)
+  DBuf swts;
+  bool screen_ok = false;
+  double sw[kMaxTraits] = {}, sh[kMaxTraits] = {};
   EpiI8 epi{};
   EpiF32 epf{};
 };
@@ -165,7 +177,7 @@ struct kg_ctx {
   // alternates between two), so repeated shapes skip the upload
   struct DescSlot {
     std::vector<char> key;
-    DBuf sd, sd1, rd;
+    DBuf sd, sd1, rd, ss, sd2;
     uint64_t used = 0;
   };
   DescSlot dslot[4];
@@ -184,6 +196,13 @@ struct kg_ctx {
   int grid_mult = 1;  // int8 scan: persistent workgroups per CU (KG_GRID_MULT)
   int tiles = 0;      // int8 scan: k-mer tiles per wave (0 = the kernel's default)
   int grid_mult_f32 = 8;  // FP32 scan: workgroups per CU
+  // pre-filter screen (int8 path): mode, and in auto mode the unscreened
+  // scans left before the screen is tried again
+  int screen_mode = KG_SCREEN_AUTO;
+  int screen_skip = 0;
+  DBuf surv, ssdesc, sdesc2;  // survivor rows; graph-path screen descriptor tables
+  hipEvent_t evs = nullptr;   // end of screen + fixup
+  int scr_occ_key = -1, scr_occ = 1;  // cached screen workgroups per CU
   // captured graph of the last batched scan
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -477,6 +496,60 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
   return KG_OK;
 }
 
+// Pre-filter screen weights: y - ybar of every trait as two balanced int8
+// limbs, v_j = w (254 q_0 + q_1), in the screen kernel's operand layout
+// (ScreenDesc, kg_internal.h).  h bounds |(y_j - exact mean) - v_j|: the
+// measured limb residual plus the FP64 rounding of the mean and of y - ybar.
+static int build_screen(kg_ctx* ctx, ModelDev& M) {
+  const Model& md = M.md;
+  const int n = md.n, t = md.t, nch = md.nch_i8;
+  M.screen_ok = false;
+  if (t > kMaxTraits || nch > screen_max_chunks()) return KG_OK;
+  const int Q = t <= 2 ? 4 : (t <= 4 ? 8 : 16), KG = 16 / Q;
+  std::vector<std::vector<int8_t>> lim(t);
+  std::vector<double> yc(n);
+  for (int tr = 0; tr < t; ++tr) {
+    const double* y = &md.Y[(size_t)tr * n];
+    const double ym = md.ymean[tr];
+    double ymax = 0.0;
+    for (int j = 0; j < n; ++j) {
+      yc[j] = y[j] - ym;
+      ymax = std::max(ymax, std::fabs(y[j]));
+    }
+    double w2[2], hq;
+    quantize(yc.data(), n, 2, lim[tr], w2, &hq);
+    double h = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const double v = w2[1] * (254.0 * lim[tr][2 * j] + lim[tr][2 * j + 1]);
+      h = std::max(h, std::fabs(yc[j] - v));
+    }
+    M.sw[tr] = w2[1];
+    M.sh[tr] = h * (1.0 + 1e-9) + 2.3e-16 * ((double)n + 8.0) * (ymax + std::fabs(ym));
+  }
+  std::vector<int> samp(md.n_acc, -1);
+  for (int j = 0; j < n; ++j) samp[md.idxs[j]] = j;
+  const size_t per = (size_t)8 * 64 * 16;  // bytes per 512-sample chunk
+  std::vector<int8_t> W((size_t)nch * per, 0);
+  for (int C = 0; C < nch; ++C)
+    for (int k = 0; k < 8; ++k)
+      for (int lane = 0; lane < 64; ++lane) {
+        const int g = lane >> 4, row = lane & 15;
+        if (row / Q != k % KG) continue;  // rows of another shift group
+        const int qn = row % Q, tr = qn >> 1, l = qn & 1;
+        if (tr >= t) continue;
+        int8_t* dst = &W[(size_t)C * per + ((size_t)k * 64 + lane) * 16];
+        for (int e = 0; e < 16; ++e) {
+          // operand byte e = 4 i + q: bit 8 q + k of the lane group's dword i
+          const int a = 512 * C + 128 * g + 32 * (e >> 2) + 8 * (e & 3) + k;
+          if (a < md.n_acc && samp[a] >= 0) dst[e] = lim[tr][(size_t)samp[a] * 2 + l];
+        }
+      }
+  KG_CK(ctx, M.swts.ensure(W.size()));
+  KG_CK(ctx, hipMemcpy(M.swts.p, W.data(), W.size(), hipMemcpyHostToDevice));
+  M.screen_ok = true;
+  return KG_OK;
+}
+
 static int build_f32(kg_ctx* ctx, ModelDev& M) {
   Model& md = M.md;
   const int n = md.n, p = md.p, t = md.t;
@@ -557,6 +630,11 @@ extern "C" int kg_create(kg_ctx** out, const kg_opts* opts) {
     ctx->device = opts->device;
     ctx->precision = opts->precision;
     if (opts->limbs) ctx->limbs = opts->limbs;
+    ctx->screen_mode = opts->screen;
+  }
+  if (ctx->screen_mode < KG_SCREEN_AUTO || ctx->screen_mode > KG_SCREEN_ON) {
+    delete ctx;
+    return KG_ERR_ARG;
   }
   if (ctx->precision != KG_PREC_MIXED && ctx->precision != KG_PREC_FP32) {
     delete ctx;
@@ -581,6 +659,7 @@ extern "C" int kg_create(kg_ctx** out, const kg_opts* opts) {
   KG_CK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   KG_CK(ctx, hipStreamCreateWithFlags(&ctx->cstream, hipStreamNonBlocking));
   for (auto& e : ctx->ev) KG_CK(ctx, hipEventCreate(&e));
+  KG_CK(ctx, hipEventCreate(&ctx->evs));
   for (auto& e : ctx->cev) KG_CK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (auto& e : ctx->ctev) KG_CK(ctx, hipEventCreate(&e));
   KG_CK(ctx, set_kernel_attributes());
@@ -610,6 +689,7 @@ extern "C" void kg_destroy(kg_ctx* ctx) {
   if (ctx->cstream) (void)hipStreamSynchronize(ctx->cstream);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  if (ctx->evs) (void)hipEventDestroy(ctx->evs);
   for (auto& e : ctx->cev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->ctev)
@@ -650,7 +730,8 @@ static int add_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int32_t n,
   md.maskw.assign((size_t)8 * md.nch_i8, 0ull);
   for (int j = 0; j < n; ++j) md.maskw[idxs[j] >> 6] |= 1ull << (idxs[j] & 63);
   prepare_model(md);
-  const int rc = (ctx->precision == KG_PREC_MIXED) ? build_i8(ctx, *M) : build_f32(ctx, *M);
+  int rc = (ctx->precision == KG_PREC_MIXED) ? build_i8(ctx, *M) : build_f32(ctx, *M);
+  if (rc == KG_OK && ctx->precision == KG_PREC_MIXED) rc = build_screen(ctx, *M);
   if (rc != KG_OK) return rc;
   drop_graph(ctx);
   ctx->models.push_back(std::move(M));
@@ -734,22 +815,42 @@ static int check_shard(kg_ctx* ctx, const HShard& s) {
   return KG_OK;
 }
 
-// Device part of a batched int8 scan: zero counters, bulk kernel(s), refine.
+// Device tables and launch shape of a screened scan.
+struct ScreenRun {
+  bool on = false;
+  const ScreenDesc* d_ss = nullptr;
+  ShardDesc* d_sd2 = nullptr;   // full-kernel table over the survivor lists
+  int64_t* d_total = nullptr;   // full-kernel block count, written by the fix-up
+  int64_t sblocks = 0;          // screen blocks
+  int sgrid = 0;
+  int nch_max = 0;
+};
+
+// Device part of a batched int8 scan: zero counters, [screen, fix-up,] bulk
+// kernel(s), refine.
 static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks, bool per_launch,
                       const std::vector<int64_t>& nblk, const ScanOut& o, int ref_blocks,
-                      const ShardDesc* d_sd, const ShardDesc* d_sd1, const RefDesc* d_rd) {
+                      const ShardDesc* d_sd, const ShardDesc* d_sd1, const RefDesc* d_rd,
+                      const ScreenRun& sr) {
   const int L = ctx->limbs;
   const int grid = ctx->cus * ctx->grid_mult;
   KG_CK(ctx, launch_zero(ctx->stream, o.counters, (int)ctx->counters_len));
   KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-  if (!per_launch) {
+  if (sr.on) {
+    KG_CK(ctx, launch_screen(t, sr.nch_max, sr.sgrid, ctx->stream, sr.d_ss, nsh, sr.sblocks));
+    KG_CK(ctx, launch_screen_fixup(ctx->stream, sr.d_ss, sr.d_sd2, nsh,
+                                   i8_block_kmers(NT, L, t, ctx->tiles), sr.d_total));
+    KG_CK(ctx, hipEventRecord(ctx->evs, ctx->stream));
+    KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream, sr.d_sd2, nsh,
+                              total_blocks, o, sr.d_total));
+  } else if (!per_launch) {
     KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream, d_sd, nsh, total_blocks,
-                              o));
+                              o, nullptr));
   } else {
     for (int s = 0; s < nsh; ++s) {
       if (nblk[s] == 0) continue;
       KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream, d_sd1 + s, 1, nblk[s],
-                                o));
+                                o, nullptr));
     }
   }
   KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
@@ -786,7 +887,25 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   std::vector<RefDesc> rd(nsh);
   std::vector<int64_t> nblk(nsh);
   int64_t total_blocks = 0;
-  ctx->counters_len = 1 + (size_t)nsh * t;
+  const bool per_launch = (flags & kPerLaunch) != 0;
+  // pre-filter screen: only where it can drop k-mers (tau > 0) and every
+  // shard's model has screen weights; row lists are 32-bit
+  bool screen = !stats && !per_launch && prm.corr_threshold > 0.0 &&
+                ctx->screen_mode != KG_SCREEN_OFF;
+  int nch_max = 0;
+  int64_t mtot = 0;
+  for (const auto& s : sh) {
+    const ModelDev& M = *ctx->models[s.model];
+    if (!M.screen_ok || s.m > (int64_t)UINT32_MAX) screen = false;
+    nch_max = std::max(nch_max, M.md.nch_i8);
+    mtot += s.m;
+  }
+  if (screen && ctx->screen_mode == KG_SCREEN_AUTO && ctx->screen_skip > 0) {
+    --ctx->screen_skip;
+    screen = false;
+  }
+  const size_t base_surv = 1 + (size_t)nsh * t;  // counters: [cand][passes][survivors][total]
+  ctx->counters_len = base_surv + (screen ? (size_t)nsh + 1 : 0);
   KG_CK(ctx, ctx->counters.ensure(sizeof(unsigned long long) * ctx->counters_len));
   unsigned long long* dc = ctx->counters.as<unsigned long long>();
   for (int s = 0; s < nsh; ++s) {
@@ -828,6 +947,57 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     R.t = md.t;
     R.n = md.n;
   }
+  // screen tables: survivors of shard s go to surv[off_s, off_s + m_s)
+  std::vector<ScreenDesc> ss;
+  std::vector<ShardDesc> sd2;
+  ScreenRun sr;
+  if (screen) {
+    KG_CK(ctx, ctx->surv.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(mtot, 1)));
+    uint32_t* sv = ctx->surv.as<uint32_t>();
+    const int64_t sbk = screen_block_kmers();
+    ss.resize(nsh);
+    sd2 = sd;
+    int64_t off = 0;
+    for (int s = 0; s < nsh; ++s) {
+      const ModelDev& M = *ctx->models[sh[s].model];
+      const Model& md = M.md;
+      ScreenDesc& S = ss[s];
+      S.bits = sh[s].bits;
+      S.m = sh[s].m;
+      S.rw = sh[s].rw;
+      S.blk0 = sr.sblocks;
+      sr.sblocks += (sh[s].m + sbk - 1) / sbk;
+      S.wts = M.swts.p;
+      S.mask = M.mask.p;
+      S.list = sv + off;
+      S.nsurv = dc + base_surv + s;
+      S.nchunks = md.nch_i8;
+      S.t = md.t;
+      S.n = md.n;
+      S.pad = 0;
+      for (int tr = 0; tr < kMaxTraits; ++tr) {
+        S.w[tr] = tr < md.t ? M.sw[tr] : 0.0;
+        S.h[tr] = tr < md.t ? M.sh[tr] : 0.0;
+        S.thr[tr] = (tr < md.t && md.ycn[tr] >= 1e-12)
+                        ? prm.corr_threshold * prm.corr_threshold * md.ycn[tr] * md.ycn[tr] /
+                              (double)md.n
+                        : -1.0;
+      }
+      sd2[s].rows = S.list;
+      sd2[s].m = 0;     // written on the device by the fix-up
+      sd2[s].blk0 = 0;
+      off += sh[s].m;
+    }
+    sr.on = true;
+    sr.nch_max = nch_max;
+    sr.d_total = reinterpret_cast<int64_t*>(dc + base_surv + nsh);
+    const int okey = t * 64 + nch_max;
+    if (ctx->scr_occ_key != okey) {
+      ctx->scr_occ = screen_wgs_per_cu(t, nch_max);
+      ctx->scr_occ_key = okey;
+    }
+    sr.sgrid = ctx->cus * ctx->scr_occ;
+  }
   if (ctx->cand_cap == 0) {
     int64_t mt = 0;
     for (const auto& s : sh) mt += s.m;
@@ -850,7 +1020,6 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     o.st_xy = ctx->st_xy.as<double>();
     o.st_S = ctx->st_S.as<double>();
   }
-  const bool per_launch = (flags & kPerLaunch) != 0;
   const bool use_graph = (flags & kGraph) != 0 && !stats;
   const int ref_blocks = ctx->cus * 4;
   std::vector<unsigned long long> cnt(ctx->counters_len);
@@ -862,9 +1031,19 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     // graph key: everything the captured launches depend on
     std::vector<char> key;
     if (use_graph) {
-      const size_t kb = sizeof(ShardDesc) * nsh * 2 + sizeof(RefDesc) * nsh + sizeof(o) + 16;
+      const size_t kb = sizeof(ShardDesc) * nsh * 2 + sizeof(RefDesc) * nsh + sizeof(o) + 16 +
+                        (screen ? (sizeof(ScreenDesc) + sizeof(ShardDesc)) * nsh + 16 : 0);
       key.resize(kb);
       char* kp = key.data();
+      if (screen) {
+        memcpy(kp, ss.data(), sizeof(ScreenDesc) * nsh);
+        kp += sizeof(ScreenDesc) * nsh;
+        memcpy(kp, sd2.data(), sizeof(ShardDesc) * nsh);
+        kp += sizeof(ShardDesc) * nsh;
+        const int64_t sx[2] = {sr.sgrid, sr.nch_max};
+        memcpy(kp, sx, 16);
+        kp += 16;
+      }
       memcpy(kp, sd.data(), sizeof(ShardDesc) * nsh);
       kp += sizeof(ShardDesc) * nsh;
       memcpy(kp, sd1.data(), sizeof(ShardDesc) * nsh);
@@ -886,15 +1065,32 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
         KG_CK(ctx, hipMemcpy(ctx->sdesc1.p, sd1.data(), sizeof(ShardDesc) * nsh,
                              hipMemcpyHostToDevice));
         KG_CK(ctx, hipMemcpy(ctx->rdesc.p, rd.data(), sizeof(RefDesc) * nsh, hipMemcpyHostToDevice));
+        if (screen) {
+          KG_CK(ctx, ctx->ssdesc.ensure(sizeof(ScreenDesc) * nsh));
+          KG_CK(ctx, ctx->sdesc2.ensure(sizeof(ShardDesc) * nsh));
+          KG_CK(ctx, hipMemcpy(ctx->ssdesc.p, ss.data(), sizeof(ScreenDesc) * nsh,
+                               hipMemcpyHostToDevice));
+          KG_CK(ctx, hipMemcpy(ctx->sdesc2.p, sd2.data(), sizeof(ShardDesc) * nsh,
+                               hipMemcpyHostToDevice));
+        }
+      }
+      if (screen) {
+        sr.d_ss = ctx->ssdesc.as<ScreenDesc>();
+        sr.d_sd2 = ctx->sdesc2.as<ShardDesc>();
       }
     } else {
       // descriptor tables: reuse an identical recent upload (every earlier scan
       // has completed, so no kernel still reads a slot that gets replaced)
       const size_t b0 = sizeof(ShardDesc) * nsh, b2 = sizeof(RefDesc) * nsh;
-      std::vector<char> dk(2 * b0 + b2);
+      const size_t b3 = screen ? sizeof(ScreenDesc) * nsh : 0, b4 = screen ? b0 : 0;
+      std::vector<char> dk(2 * b0 + b2 + b3 + b4);
       memcpy(dk.data(), sd.data(), b0);
       memcpy(dk.data() + b0, sd1.data(), b0);
       memcpy(dk.data() + 2 * b0, rd.data(), b2);
+      if (screen) {
+        memcpy(dk.data() + 2 * b0 + b2, ss.data(), b3);
+        memcpy(dk.data() + 2 * b0 + b2 + b3, sd2.data(), b4);
+      }
       kg_ctx::DescSlot* slot = nullptr;
       for (auto& e : ctx->dslot)
         if (e.key == dk) slot = &e;
@@ -909,12 +1105,22 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
         KG_CK(ctx, hipMemcpy(slot->sd.p, sd.data(), b0, hipMemcpyHostToDevice));
         KG_CK(ctx, hipMemcpy(slot->sd1.p, sd1.data(), b0, hipMemcpyHostToDevice));
         KG_CK(ctx, hipMemcpy(slot->rd.p, rd.data(), b2, hipMemcpyHostToDevice));
+        if (screen) {
+          KG_CK(ctx, slot->ss.ensure(b3));
+          KG_CK(ctx, slot->sd2.ensure(b4));
+          KG_CK(ctx, hipMemcpy(slot->ss.p, ss.data(), b3, hipMemcpyHostToDevice));
+          KG_CK(ctx, hipMemcpy(slot->sd2.p, sd2.data(), b4, hipMemcpyHostToDevice));
+        }
         slot->key = std::move(dk);
       }
       slot->used = ++ctx->dclock;
       d_sd = slot->sd.as<ShardDesc>();
       d_sd1 = slot->sd1.as<ShardDesc>();
       d_rd = slot->rd.as<RefDesc>();
+      if (screen) {
+        sr.d_ss = slot->ss.as<ScreenDesc>();
+        sr.d_sd2 = slot->sd2.as<ShardDesc>();
+      }
     }
     KG_CK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (use_graph) {
@@ -922,7 +1128,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
         drop_graph(ctx);
         KG_CK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_i8(ctx, NT, t, nsh, total_blocks, per_launch, nblk, o, ref_blocks, d_sd,
-                            d_sd1, d_rd);
+                            d_sd1, d_rd, sr);
         hipGraph_t g = nullptr;
         hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
         if (rc != KG_OK) {
@@ -938,7 +1144,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       KG_CK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     } else {
       int rc = enqueue_i8(ctx, NT, t, nsh, total_blocks, per_launch, nblk, o, ref_blocks, d_sd,
-                          d_sd1, d_rd);
+                          d_sd1, d_rd, sr);
       if (rc != KG_OK) return rc;
     }
     KG_CK(ctx, hipMemcpyAsync(ctx->hcnt.p, dc, sizeof(unsigned long long) * cnt.size(),
@@ -965,8 +1171,16 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     ctx->timing.bulk_ms = ms;  // graph: bulk + refine together
     ctx->timing.refine_ms = 0.0;
     ctx->timing.total_ms = ms;
+    ctx->timing.screen_ms = 0.0;
   } else {
-    KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+    ctx->timing.screen_ms = 0.0;
+    if (screen) {
+      KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->evs));
+      ctx->timing.screen_ms = ms;
+      KG_CK(ctx, hipEventElapsedTime(&ms, ctx->evs, ctx->ev[2]));
+    } else {
+      KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+    }
     ctx->timing.bulk_ms = ms;
     KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
     ctx->timing.refine_ms = ms;
@@ -974,9 +1188,16 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     ctx->timing.total_ms = ms;
   }
   ctx->timing.candidates = nc;
-  int64_t mt = 0;
-  for (const auto& s : sh) mt += s.m;
-  ctx->timing.kmers = mt;
+  ctx->timing.kmers = mtot;
+  ctx->timing.survivors = mtot;
+  if (screen) {
+    int64_t ns = 0;
+    for (int s = 0; s < nsh; ++s) ns += (int64_t)cnt[base_surv + s];
+    ctx->timing.survivors = ns;
+    // auto: a screen that keeps more than half of the k-mers costs more than
+    // it saves; skip it for the next 8 scans, then try again
+    if (ctx->screen_mode == KG_SCREEN_AUTO && 2 * ns > mtot) ctx->screen_skip = 8;
+  }
   for (int s = 0; s < nsh; ++s)
     for (int tr = 0; tr < t; ++tr) n_passed[s * t + tr] += (int64_t)cnt[1 + (size_t)s * t + tr];
   if (nc == 0) return KG_OK;
@@ -1239,6 +1460,8 @@ extern "C" int kg_scan(kg_ctx* ctx, const uint64_t* bits, int64_t m, int64_t row
     acc.refine_ms += ctx->timing.refine_ms;
     acc.total_ms += ctx->timing.total_ms;
     acc.candidates += ctx->timing.candidates;
+    acc.screen_ms += ctx->timing.screen_ms;
+    acc.survivors += ctx->timing.survivors;
   }
   KG_CK(ctx, hipStreamSynchronize(ctx->cstream));
   acc.kmers = m;
@@ -1319,6 +1542,9 @@ extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t*
     if (i8_tiles_for(M.NT, ctx->limbs, md.t, 2) != R_def) Rs.push_back(2);
     const int gms[3] = {1, 2, 4};
     const int keep_tiles = ctx->tiles, keep_gm = ctx->grid_mult;
+    // the full kernel's shape is tuned on every row (random rows fail the screen)
+    const int keep_mode = ctx->screen_mode, keep_skip = ctx->screen_skip;
+    ctx->screen_mode = KG_SCREEN_OFF;
     double best = 1e300;
     int best_R = keep_tiles, best_gm = keep_gm;
     for (int R : Rs)
@@ -1334,6 +1560,8 @@ extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t*
           if (rc) {
             ctx->tiles = keep_tiles;
             ctx->grid_mult = keep_gm;
+            ctx->screen_mode = keep_mode;
+            ctx->screen_skip = keep_skip;
             return rc;
           }
           if (rep > 0) tmin = std::min(tmin, ctx->timing.bulk_ms);
@@ -1346,6 +1574,8 @@ extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t*
       }
     ctx->tiles = best_R;
     ctx->grid_mult = best_gm;
+    ctx->screen_mode = keep_mode;
+    ctx->screen_skip = keep_skip;
     drop_graph(ctx);
   }
   if (tiles) *tiles = i8_tiles_for(M.NT, ctx->limbs, md.t, ctx->tiles);

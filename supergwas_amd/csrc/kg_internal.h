@@ -83,7 +83,30 @@ struct ShardDesc {
   unsigned long long* pass;      // [t] decided pre-filter passes of this shard
   int id;                        // index of this shard in the caller's table
   int pad2;
+  const uint32_t* rows;          // k-mer j of the scan is row rows[j] (null: row j)
   EpiI8 ep;
+};
+
+// Pre-filter screen of one shard: a cheap pass that contracts each k-mer with
+// y - mean(y) only (two int8 limbs per trait) and keeps every k-mer whose
+// pre-filter is not certainly failed for some trait.
+//
+// Weights: [chunk][shift k][lane] 16-byte MFMA A operands.  Lane (g, row),
+// operand byte e = 4 i + q holds accession 512 C + 128 g + 32 i + 8 q + k;
+// row = Q (k % KG) + 2 tr + l for trait tr, limb l (Q = 4, 8 or 16 rows per
+// shift group, KG = 16 / Q), zero in the other rows.
+struct ScreenDesc {
+  const uint64_t* bits;
+  int64_t m, rw;
+  int64_t blk0;                  // first global screen block of this shard
+  const void* wts;               // screen weights (above)
+  const void* mask;              // accession mask words (as ShardDesc)
+  uint32_t* list;                // survivors: rows within the shard, unordered
+  unsigned long long* nsurv;     // survivor count
+  int nchunks, t, n, pad;
+  double w[kMaxTraits];          // S~ = w_tr (254 A_0 + A_1), A_l = limb sums
+  double h[kMaxTraits];          // |S - S~| <= c h_tr (quantisation + rounding)
+  double thr[kMaxTraits];        // tau^2 |y - ybar|^2 / n; < 0: trait never passes
 };
 
 // FP64 refinement inputs of one shard.

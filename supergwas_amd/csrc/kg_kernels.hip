@@ -177,7 +177,8 @@ __device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wa
   for (int r = 0; r < R; ++r) {
     int64_t k = kbase + 16 * r + col;
     if (k > D.m - 1) k = D.m - 1;  // clamp: rows past m are read, never reported
-    rowp[r] = D.bits + k * D.rw + 2 * g;
+    const int64_t row = D.rows ? (int64_t)D.rows[k] : k;
+    rowp[r] = D.bits + row * D.rw + 2 * g;
   }
 }
 
@@ -231,10 +232,13 @@ __host__ __device__ constexpr int i8_tiles(int NT, int TT) { return (TT == 1 && 
 
 // L = limbs per column; TT = traits the epilogue carries (1, or kMaxTraits);
 // R = k-mer tiles of 16 per wave.
+// total_dev, when given, holds the block count (written on the device by the
+// screen's fixup; total_blocks is then only an upper bound).
 template <int NT, int L, int TT, int R>
 __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
                                                         int nshards, int64_t total_blocks,
-                                                        ScanOut out) {
+                                                        ScanOut out,
+                                                        const int64_t* __restrict__ total_dev) {
   constexpr int SS = i8_stage_steps(NT);             // MFMA steps per stage
   constexpr int SPC = 8 / SS;                        // stages per 512-sample chunk
   constexpr int FRAG2 = SS * NT * 64;                // v4i per stage
@@ -254,6 +258,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
   v4i* const mybits = bitsbuf + wave * (R + 1) * 64;
   if (threadIdx.x < kMaxTraits) wg_pass[threadIdx.x] = 0;
 
+  if (total_dev != nullptr) total_blocks = *total_dev;
   int64_t w = blockIdx.x;
   if (w >= total_blocks) return;
   int s = 0;
@@ -444,6 +449,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 
     const int64_t kidx = kbase + 16 * rme + col;
     const bool valid = primary && (kidx < D.m);
+    const int64_t krow = (D.rows != nullptr && valid) ? (int64_t)D.rows[kidx] : kidx;
     const double cd = (double)c, cn = cd * ep.inv_n;
     // |dz_k| <= c h_k  =>  |d |z|^2| <= 2c sum|z_k| h_k + c^2 sum h_k^2
     //                              <= 2c |z| |h| + c^2 |h|^2
@@ -485,7 +491,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         const unsigned long long pb = __ballot(pf_def);
         if (lane == 0 && pb) atomicAdd(&wg_pass[tr], (unsigned int)__popcll(pb));
         Cand rec;
-        rec.idx = kidx;
+        rec.idx = krow;
         rec.trait = tr;
         rec.flags = fl;
         rec.shard = D.id;
@@ -505,6 +511,208 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     w = wn;
     s = sn;
   }
+}
+
+// ----------------------------------------------------------------------------
+// Pre-filter screen (src/kgwas.cpp:183-218 runs the correlation filter before
+// the projections; this is the same order on the device).  Per k-mer it forms
+// S~ = x'(y - ybar) from two int8 limbs of y - ybar and the allele count c,
+// and keeps the k-mer unless |cor| < tau is certain for every trait:
+//   (|S~| + c h)^2 < tau^2 |y - ybar|^2 c (n - c) / n  (1 - 3e-12).
+// The kept rows go to scan_i8, which decides them exactly as it decides an
+// unscreened scan, so screening changes no count and no hit.
+//
+// B operands need no shift: step k (0..7) of a 512-sample chunk masks bit k of
+// every byte of the lane's four dwords (one AND per dword), so operand bytes
+// are 0 or 2^k (-128 for k = 7).  Each output row sees a single k: the
+// weights place shift k in rows Q (k % KG) .. + Q - 1 of accumulator k / KG
+// and zero elsewhere, and the epilogue divides each row by its exact power
+// of two.  Workgroup: 8 independent waves; the model's weights (8 KiB per
+// chunk) sit in LDS for the whole launch.  Wave v of global block w owns the
+// shard's k-mers [(b*8 + v) * 16R, +16R), b = w - blk0.  Kept rows are
+// staged per wave in LDS and appended to the shard's list in bursts.
+constexpr int kScrWaves = 8;
+constexpr int kScrThreads = 64 * kScrWaves;
+constexpr int kScrBuf = 192;    // staged survivor rows per wave
+constexpr int kScrMaxChunks = 16;
+__host__ __device__ constexpr int scr_rows(int t) { return t <= 2 ? 4 : (t <= 4 ? 8 : 16); }
+__host__ __device__ constexpr size_t scr_lds_bytes(int nch) {
+  return (size_t)kScrWaves * kScrBuf * 4 + (size_t)nch * 64 + (size_t)nch * 8 * 64 * 16;
+}
+
+// Appends the staged rows of one wave to shard D's list.  Wave-uniform.
+__device__ __forceinline__ void scr_flush(const ScreenDesc* D, const uint32_t* buf, int& nbuf,
+                                          int lane) {
+  if (nbuf == 0) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(D->nsurv, (unsigned long long)nbuf);
+  base = __shfl(base, 0);
+  for (int i = lane; i < nbuf; i += 64) D->list[base + i] = buf[i];
+  nbuf = 0;
+}
+
+template <int Q, int R>
+__global__ __launch_bounds__(kScrThreads) void screen_i8(const ScreenDesc* __restrict__ sd,
+                                                         int nshards, int64_t total_blocks) {
+  constexpr int KG = 16 / Q;   // shifts sharing an accumulator
+  constexpr int NA = 8 / KG;   // accumulators per tile
+  constexpr int TT = Q / 2;    // traits an instantiation covers
+  constexpr int DC = 4;        // chunks loaded per group
+  extern __shared__ v4i scr_lds[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  uint32_t* const buf = reinterpret_cast<uint32_t*>(scr_lds) + wave * kScrBuf;
+  const uint4* const mk_lds = reinterpret_cast<const uint4*>(scr_lds + kScrWaves * kScrBuf / 4);
+  // output rows 4 g .. 4 g + 3 of accumulator a: shift a KG + g / (Q / 4),
+  // traits tb and tb + 1 (limbs 0, 1 each)
+  const int ksh = g / (Q / 4);
+  const int tb = 2 * (g % (Q / 4));
+
+  int64_t w = blockIdx.x;
+  if (w >= total_blocks) return;
+  int s = 0, bs = -1, nbuf = 0;
+  const void* staged = nullptr;
+  int nch_staged = 0;
+  while (w < total_blocks) {
+    while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
+    const ScreenDesc& D = sd[s];
+    if (bs != s) {
+      if (bs >= 0) scr_flush(&sd[bs], buf, nbuf, lane);
+      bs = s;
+    }
+    const int nch = D.nchunks;
+    if (D.wts != staged) {  // workgroup-uniform: every wave walks the same blocks
+      __syncthreads();
+      const v4i* __restrict__ gw = reinterpret_cast<const v4i*>(D.wts);
+      const uint4* __restrict__ gm = reinterpret_cast<const uint4*>(D.mask);
+      v4i* const wl = scr_lds + kScrWaves * kScrBuf / 4 + 4 * nch;
+      for (int i = threadIdx.x; i < nch * 512; i += kScrThreads) wl[i] = gw[i];
+      for (int i = threadIdx.x; i < nch * 4; i += kScrThreads)
+        reinterpret_cast<uint4*>(scr_lds + kScrWaves * kScrBuf / 4)[i] = gm[i];
+      __syncthreads();
+      staged = D.wts;
+      nch_staged = nch;
+    }
+    const v4i* const wl = scr_lds + kScrWaves * kScrBuf / 4 + 4 * nch_staged;
+    const int64_t kbase = ((w - D.blk0) * kScrWaves + wave) * (16 * R);
+    const uint4* rp[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      int64_t k = kbase + 16 * r + col;
+      if (k > D.m - 1) k = D.m - 1;  // clamp: rows past m are read, never kept
+      rp[r] = reinterpret_cast<const uint4*>(D.bits + k * D.rw) + g;
+    }
+    v4i acc[R][NA];
+    uint32_t cnt[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cnt[r] = 0;
+#pragma unroll
+      for (int a = 0; a < NA; ++a) acc[r][a] = v4i{0, 0, 0, 0};
+    }
+    for (int C0 = 0; C0 < nch; C0 += DC) {
+      uint4 bq[DC][R];
+#pragma unroll
+      for (int u = 0; u < DC; ++u)
+        if (C0 + u < nch)
+#pragma unroll
+          for (int r = 0; r < R; ++r) bq[u][r] = rp[r][4 * (C0 + u)];
+#pragma unroll
+      for (int u = 0; u < DC; ++u) {
+        if (C0 + u < nch) {
+          const int C = C0 + u;
+          const uint4 mk = mk_lds[4 * C + g];
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            cnt[r] += __popc(bq[u][r].x & mk.x) + __popc(bq[u][r].y & mk.y) +
+                      __popc(bq[u][r].z & mk.z) + __popc(bq[u][r].w & mk.w);
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const v4i a = wl[(C * 8 + k) * 64 + lane];
+            const uint32_t M = 0x01010101u << k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const v4i b = v4i{(int)(bq[u][r].x & M), (int)(bq[u][r].y & M),
+                                (int)(bq[u][r].z & M), (int)(bq[u][r].w & M)};
+              acc[r][k / KG] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[r][k / KG], 0, 0, 0);
+            }
+          }
+        }
+      }
+    }
+
+    // rows -> per-lane partial 254 A_0 + A_1 of traits tb, tb + 1 (exact:
+    // every row of accumulator a is a multiple of 2^k, k = a KG + ksh)
+    int part[R][2];
+#pragma unroll
+    for (int r = 0; r < R; ++r) part[r][0] = part[r][1] = 0;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      const int kk = a * KG + ksh;
+      const int sg = (kk == 7) ? -1 : 1;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          part[r][j] += sg * (254 * (acc[r][a][2 * j] >> kk) + (acc[r][a][2 * j + 1] >> kk));
+    }
+    const int c = (int)group_scatter<R>(cnt);
+    const int64_t kidx = kbase + 16 * (g % R) + col;
+    const bool valid = (g < R) && kidx < D.m;
+    const int n = D.n;
+    const double cd = (double)c, v = cd * (double)(n - c);
+    bool keep = false;
+#pragma unroll
+    for (int tr = 0; tr < TT; ++tr) {
+      if (tr < D.t) {
+        uint32_t x[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          x[r] = (uint32_t)((tb == tr) ? part[r][0] : ((tb + 1 == tr) ? part[r][1] : 0));
+        const int A = (int)group_scatter<R>(x);
+        const double S = D.w[tr] * (double)A;
+        const double hi = fabs(S) + cd * D.h[tr] + 1e-12 * (fabs(S) + 1.0);
+        if (D.thr[tr] >= 0.0 && !(hi * hi < D.thr[tr] * v * (1.0 - 3e-12))) keep = true;
+      }
+    }
+    keep = keep && valid && c > 0 && c < n;
+    const unsigned long long bal = __ballot(keep);
+    if (keep) buf[nbuf + __popcll(bal & ((1ull << lane) - 1ull))] = (uint32_t)kidx;
+    nbuf += __popcll(bal);
+    if (nbuf > kScrBuf - 64) scr_flush(&D, buf, nbuf, lane);
+    w += gridDim.x;
+  }
+  if (bs >= 0) scr_flush(&sd[bs], buf, nbuf, lane);
+}
+
+// Pass-2 descriptors from the screen's survivor counts: sd2[s].m and .blk0
+// (blocks of bk k-mers, exclusive prefix) and the block total.  One workgroup
+// of 256 threads, each owning a contiguous run of shards.
+__global__ __launch_bounds__(256) void screen_fixup(const ScreenDesc* __restrict__ ss,
+                                                    ShardDesc* __restrict__ sd2, int nsh,
+                                                    int64_t bk, int64_t* __restrict__ total) {
+  __shared__ int64_t part[256];
+  const int tid = threadIdx.x;
+  const int per = (nsh + 255) / 256;
+  const int s0 = tid * per, s1 = min(nsh, s0 + per);
+  int64_t sum = 0;
+  for (int s = s0; s < s1; ++s) sum += ((int64_t)*ss[s].nsurv + bk - 1) / bk;
+  part[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int64_t add = (tid >= off) ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += add;
+    __syncthreads();
+  }
+  int64_t run = part[tid] - sum;
+  for (int s = s0; s < s1; ++s) {
+    const int64_t m = (int64_t)*ss[s].nsurv;
+    sd2[s].m = m;
+    sd2[s].blk0 = run;
+    run += (m + bk - 1) / bk;
+  }
+  if (tid == 255) *total = part[255];
 }
 
 // ----------------------------------------------------------------------------
@@ -744,9 +952,9 @@ __global__ void synth_bits(uint64_t* __restrict__ out, uint64_t seed, int64_t km
 
 template <int NT, int L, int TT, int R>
 static hipError_t launch_i8_tr(dim3 grid, hipStream_t s, const ShardDesc* sd, int nshards,
-                               int64_t total_blocks, const ScanOut& o) {
+                               int64_t total_blocks, const ScanOut& o, const int64_t* total_dev) {
   auto kfn = scan_i8<NT, L, TT, R>;
-  hipLaunchKernelGGL(kfn, grid, dim3(kI8Threads), 0, s, sd, nshards, total_blocks, o);
+  hipLaunchKernelGGL(kfn, grid, dim3(kI8Threads), 0, s, sd, nshards, total_blocks, o, total_dev);
   return hipGetLastError();
 }
 
@@ -770,20 +978,20 @@ int i8_column_capacity(int NT, int L) { return 4 * ((4 * NT) / L); }
 // NT ranges per limb count: 1-3 (L = 1), 1-6 (L = 2), 1-16 (L = 3).
 hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t s,
                           const ShardDesc* sd, int nshards, int64_t total_blocks,
-                          const ScanOut& o) {
+                          const ScanOut& o, const int64_t* total_dev) {
   dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
   const int TT = t > 1 ? kMaxTraits : 1;
   const int R = i8_tiles_for(NT, L, t, R_req);
   switch (((NT * 4 + L) * 16 + TT) * 8 + R) {
 #define KG_CASE(X, Y)                                                                       \
   case ((X * 4 + Y) * 16 + 1) * 8 + i8_tiles(X, 1):                                         \
-    return launch_i8_tr<X, Y, 1, i8_tiles(X, 1)>(g, s, sd, nshards, total_blocks, o);       \
+    return launch_i8_tr<X, Y, 1, i8_tiles(X, 1)>(g, s, sd, nshards, total_blocks, o, total_dev);       \
   case ((X * 4 + Y) * 16 + kMaxTraits) * 8 + i8_tiles(X, kMaxTraits):                       \
     return launch_i8_tr<X, Y, kMaxTraits, i8_tiles(X, kMaxTraits)>(g, s, sd, nshards,       \
-                                                                    total_blocks, o);
+                                                                    total_blocks, o, total_dev);
 #define KG_CASE_R2(X)                                                                       \
   case ((X * 4 + 3) * 16 + 1) * 8 + 2:                                                      \
-    return launch_i8_tr<X, 3, 1, 2>(g, s, sd, nshards, total_blocks, o);
+    return launch_i8_tr<X, 3, 1, 2>(g, s, sd, nshards, total_blocks, o, total_dev);
     KG_CASE(1, 1) KG_CASE(2, 1) KG_CASE(3, 1)
     KG_CASE(1, 2) KG_CASE(2, 2) KG_CASE(3, 2) KG_CASE(4, 2) KG_CASE(5, 2) KG_CASE(6, 2)
     KG_CASE(1, 3) KG_CASE(2, 3) KG_CASE(3, 3) KG_CASE(4, 3) KG_CASE(5, 3) KG_CASE(6, 3)
@@ -795,6 +1003,52 @@ hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t
     default:
       return hipErrorInvalidValue;
   }
+}
+
+// Pre-filter screen over a shard table; nch_max = the largest model's 512-sample
+// chunks (sizes the LDS weight table).  grid = persistent workgroups.
+int64_t screen_block_kmers() { return (int64_t)kScrWaves * 16 * 2; }
+size_t screen_lds_bytes(int nch_max) { return scr_lds_bytes(nch_max); }
+int screen_max_chunks() { return kScrMaxChunks; }
+
+template <int Q>
+static hipError_t launch_screen_q(dim3 grid, size_t shm, hipStream_t s, const ScreenDesc* sd,
+                                  int nshards, int64_t total_blocks) {
+  hipLaunchKernelGGL((screen_i8<Q, 2>), grid, dim3(kScrThreads), shm, s, sd, nshards,
+                     total_blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
+                         int nshards, int64_t total_blocks) {
+  if (nch_max > kScrMaxChunks || t < 1 || t > kMaxTraits) return hipErrorInvalidValue;
+  dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
+  const size_t shm = scr_lds_bytes(nch_max);
+  switch (scr_rows(t)) {
+    case 4: return launch_screen_q<4>(g, shm, s, sd, nshards, total_blocks);
+    case 8: return launch_screen_q<8>(g, shm, s, sd, nshards, total_blocks);
+    default: return launch_screen_q<16>(g, shm, s, sd, nshards, total_blocks);
+  }
+}
+
+// Workgroups of the screen one CU holds at once for a model of nch chunks.
+int screen_wgs_per_cu(int t, int nch) {
+  int nb = 0;
+  const size_t shm = scr_lds_bytes(nch);
+  hipError_t e;
+  switch (scr_rows(t)) {
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<4, 2>, kScrThreads, shm); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<8, 2>, kScrThreads, shm); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, 2>, kScrThreads, shm); break;
+  }
+  if (e != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
+hipError_t launch_screen_fixup(hipStream_t s, const ScreenDesc* ss, ShardDesc* sd2, int nsh,
+                               int64_t bk, int64_t* total) {
+  hipLaunchKernelGGL(screen_fixup, dim3(1), dim3(256), 0, s, ss, sd2, nsh, bk, total);
+  return hipGetLastError();
 }
 
 template <int NT>
@@ -847,6 +1101,12 @@ static hipError_t set_attr_nt() {
 
 hipError_t set_kernel_attributes() {
   hipError_t e = hipSuccess;
+  const int scr = (int)scr_lds_bytes(kScrMaxChunks);
+  e = hipFuncSetAttribute((const void*)screen_i8<4, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, scr);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)screen_i8<8, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, scr);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)screen_i8<16, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, scr);
 #define KG_ATTR(X) \
   if (e == hipSuccess) e = set_attr_nt<X>();
   KG_ATTR(1) KG_ATTR(2) KG_ATTR(3) KG_ATTR(4) KG_ATTR(5) KG_ATTR(6) KG_ATTR(7) KG_ATTR(8)

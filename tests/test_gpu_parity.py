@@ -339,3 +339,86 @@ def test_batched_shards_match_oracle_and_modes(built):
         assert n0[s, 0] == ref["n_passed"], s
         assert np.array_equal(got["idx"], exp), s
         assert np.allclose(got["T"], ref["T"][exp], rtol=1e-9, atol=0)
+
+
+SCREEN_CASES = {
+    # name: (dataset kwargs, m, corr, log10, log10_2d)
+    "n2048_p32_t1": (dict(seed=2, n_acc=2048, n_missing=0, npop=8, p=32, m=40000, n_causal=50,
+                          effect_sd=0.3), 40000, 0.2, 4.0, 8.0),
+    "n700_p6_t2_missing": (dict(seed=51, n_acc=700, n_missing=41, npop=4, p=6, m=9001,
+                                n_causal=10, effect_sd=0.5, traits=2, trait_corr=0.3),
+                           9001, 0.12, 2.0, 4.0),
+    "n1500_p12_t5": (dict(seed=52, n_acc=1500, n_missing=13, npop=6, p=12, m=7000, n_causal=10,
+                          effect_sd=0.5, traits=5, trait_corr=0.2), 7000, 0.15, 2.0, 4.0),
+    "n4096_p64_t8": (dict(seed=53, n_acc=4096, n_missing=0, npop=8, p=64, m=6000, n_causal=20,
+                          effect_sd=0.3, traits=8, trait_corr=0.2, n_struct=512),
+                     6000, 0.2, 3.0, 6.0),
+    "n8192_p8_t1_high_tau": (dict(seed=54, n_acc=8192, n_missing=100, npop=8, p=8, m=3000,
+                                  n_causal=5, effect_sd=0.3, n_struct=256), 3000, 0.6, 2.0, 4.0),
+}
+
+
+@pytest.mark.parametrize("name", list(SCREEN_CASES))
+def test_screen_on_off_identical(built, name):
+    """The pre-filter screen drops only k-mers whose pre-filter certainly fails,
+    so screened and unscreened scans report the same hits, statistics and
+    counts; the screen keeps at least every k-mer that passes for some trait."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    kw, m, corr, l10, l2d = SCREEN_CASES[name]
+    ds = dataset(name, **kw)
+    bits = ds.fill_bits(0, m)
+    prm = ScanParams(corr, l10, l2d)
+    out = {}
+    for mode in ("off", "on"):
+        with KgwasHip(screen=mode) as k:
+            k.set_model(ds.n_acc, ds.idxs, ds.U, ds.Y)
+            dev = k.to_device(bits)
+            h, n = k.scan_device(dev.ptr, m, bits.shape[1], prm)
+            out[mode] = (h, n, k.timing())
+            dev.free()
+    (h0, n0, t0), (h1, n1, t1) = out["off"], out["on"]
+    print(f"[screen] {name} t={ds.Y.shape[1]} n={ds.n} m={m} hits={len(h1)} passed={list(n1)} "
+          f"survivors={t1['survivors']} screen_ms={t1['screen_ms']:.3f}", flush=True)
+    assert t0["survivors"] == m and t0["screen_ms"] == 0.0
+    assert np.array_equal(n0, n1)
+    assert np.array_equal(h0["idx"], h1["idx"]) and np.array_equal(h0["trait"], h1["trait"])
+    assert np.array_equal(h0["c"], h1["c"]) and np.array_equal(h0["T"], h1["T"])
+    # survivors cover the union of passes: at least the largest per-trait count
+    assert int(n1.max()) <= t1["survivors"] <= m
+    # against the oracle, trait by trait
+    for tr in range(ds.Y.shape[1]):
+        ref = oracle.scan_f64(bits, ds.idxs, ds.Y[:, tr], ds.U, corr)
+        exp_idx, _ = expected_hits(ref, l10, l2d)
+        assert n1[tr] == ref["n_passed"], tr
+        assert np.array_equal(h1[h1["trait"] == tr]["idx"], exp_idx), tr
+
+
+def test_screen_shards_with_empty_survivor_lists(built):
+    """Batched shards where the screen keeps nothing of some shards (high
+    threshold): the device-built pass-2 table skips them; one launch, HIP-graph
+    replays and the unscreened per-shard launches agree."""
+    from supergwas_amd.api import SHARDS_GRAPH, SHARDS_PER_LAUNCH, KgwasHip, ScanParams
+    for corr in (0.08, 0.9):
+        prm = ScanParams(corr, 1.0, 3.0)
+        with KgwasHip(screen="on") as k:
+            shards, refs, devs = _shard_setup(k, seed=31)
+            res = []
+            for flags in (0, SHARDS_GRAPH, SHARDS_GRAPH, SHARDS_PER_LAUNCH):
+                h, n = k.scan_shards(shards, prm, flags)
+                res.append((h.copy(), n.copy(), k.timing()["survivors"]))
+            for d in devs:
+                d.free()
+        print(f"[screen shards] corr={corr} hits={len(res[0][0])} passed={res[0][1][:, 0].tolist()} "
+              f"survivors={res[0][2]}", flush=True)
+        h0, n0, _ = res[0]
+        for h, n, _ in res[1:]:
+            assert np.array_equal(n, n0)
+            assert np.array_equal(h["idx"], h0["idx"]) and np.array_equal(h["shard"], h0["shard"])
+            assert np.array_equal(h["T"], h0["T"])
+        for s, (bits, idxs, y, U) in enumerate(refs):
+            if len(bits) == 0:
+                continue
+            ref = oracle.scan_f64(bits, idxs, y, U, corr)
+            assert n0[s, 0] == ref["n_passed"], (corr, s)
+            exp, _ = expected_hits(ref, 1.0, 3.0)
+            assert np.array_equal(h0[h0["shard"] == s]["idx"], exp), (corr, s)
