@@ -190,18 +190,30 @@ def main():
                "stat_max_rel_err_vs_fp64": float(np.nanmax(err_c))}
         log(f"[bench] cpu: {cs / cel:.3e} k-mers/s on {threads} threads ({cel:.1f}s)")
 
-    # ---- roofline of the bulk kernel ----
-    # HBM bytes per k-mer measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (copy-calibrated)
-    traffic = None
-    tp = os.path.join(ROOT, "profiles", "traffic_scan_i8_n2048_p32.md")
+    # ---- roofline of the dominant kernel ----
+    # Each bulk kernel streams its k-mers' packed rows once: n_acc / 8 bytes per
+    # k-mer (the screen over all m rows, the full kernel over the survivors).
+    # HBM bytes per k-mer measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # (copy-calibrated) are read from the committed traffic summary.
+    traffic_per = {}
+    tp = os.path.join(ROOT, "profiles", "traffic_scan_n2048_p32.md")
     if os.path.exists(tp):
         for ln in open(tp):
-            if ln.startswith("traffic_bytes_per_kmer:"):
-                traffic = float(ln.split(":")[1]) * args.m
+            if ln.startswith("traffic_bytes_per_kmer "):
+                kname, val = ln[len("traffic_bytes_per_kmer "):].split(":")
+                traffic_per[kname.strip()] = float(val)
+    row_bytes = n_acc / 8.0
+    n_surv = float(np.mean(surv))
+    kern = {"screen_i8": {"ms": float(np.mean(scr_ms)), "kmers": args.m},
+            "scan_i8": {"ms": float(np.mean(bulk_ms)), "kmers": n_surv}}
+    for kname, kv in kern.items():
+        kv["achieved_GBs"] = (kv["kmers"] * row_bytes / (kv["ms"] / 1e3) / 1e9
+                              if kv["ms"] > 0 else 0.0)
+    dom = max(kern, key=lambda x: kern[x]["ms"])
+    achieved = kern[dom]["achieved_GBs"]
+    traffic = (traffic_per[dom] * kern[dom]["kmers"]) if dom in traffic_per else None
     bulk_s = np.mean(bulk_ms) / 1e3
-    alg_bytes = args.m * (n_acc / 8.0)
-    achieved = alg_bytes / bulk_s / 1e9
-    alg_ops = 2.0 * n * (p + 1) * args.m
+    ncol_exec = 16 * int(np.ceil(args.limbs * p / 16.0))
     line = {
         "metric": METRIC,
         "value": value,
@@ -218,11 +230,14 @@ def main():
         "config": {"workload": f"mixed: n=2048, p=32, m={args.m:.0e} k-mers, 1 trait, "
                                f"{args.limbs} int8 limbs", "n": n, "p": p, "m": args.m,
                    "corr_threshold": 0.2, "log10_thresh": 6.0},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "mfma": {"algorithmic_TOPs": alg_ops / bulk_s / 1e12,
-                 "executed_TOPs": 2.0 * n_acc * (16 * int(np.ceil(args.limbs * (p) / 16.0)))
-                 * args.m / bulk_s / 1e12, "peak_int8_TOPs": I8_PEAK_TOPS},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "kernels": kern,
+        "mfma": {"scan_i8_executed_TOPs": 2.0 * n_acc * ncol_exec * n_surv / bulk_s / 1e12
+                 if bulk_s > 0 else 0.0,
+                 "peak_int8_TOPs": I8_PEAK_TOPS},
+        "screen_kernel_ms": float(np.mean(scr_ms)),
+        "survivors_per_step": int(n_surv),
         "bulk_kernel_ms": float(np.mean(bulk_ms)),
         "refine_kernel_ms": float(np.mean(ref_ms)),
         "candidates_per_step": int(np.mean(cands)),
@@ -276,7 +291,8 @@ def leg_c2(args, synth, ds_c3):
         bulk, hits = 0.0, 0
         for b in range(nb):
             h, _ = k.scan_device(dev.ptr + b * bm * rw * 8, bm, rw, prm)
-            bulk += k.timing()["bulk_ms"]
+            tm = k.timing()
+            bulk += tm["bulk_ms"] + tm["screen_ms"]
             hits += len(h)
         el = time.perf_counter() - t0
         sT, sc = 0.0, 0
@@ -331,7 +347,10 @@ def leg_c4(args, synth, ds):
                        "n=2048, p=32, per-shard masks 0-15% missing",
            "autotune": {"tiles_per_wave": tune[0], "workgroups_per_cu": tune[1]}}
     ref = None
-    for name, flags in (("persistent", 0), ("graph", SHARDS_GRAPH), ("per_shard", SHARDS_PER_LAUNCH)):
+    runs = (("persistent", 0, "auto"), ("graph", SHARDS_GRAPH, "auto"),
+            ("persistent_unscreened", 0, "off"), ("per_shard", SHARDS_PER_LAUNCH, "off"))
+    for name, flags, scr in runs:
+        k.set_screen(scr)
         k.scan_shards(shards, prm, flags)  # warm-up (and graph capture)
         ts, ks = [], []
         for _ in range(3):
@@ -350,7 +369,9 @@ def leg_c4(args, synth, ds):
                      "hits": int(len(hits)), "matches_persistent": same}
         log(f"[bench] c4 {name}: {total / min(ts):.3e} k-mers/s (device {min(ks):.1f} ms,"
             f" wall {1e3 * min(ts):.1f} ms), hits {len(hits)}, same {same}")
-    out["speedup_persistent_vs_per_shard"] = out["per_shard"]["wall_ms"] / out["persistent"]["wall_ms"]
+    out["speedup_persistent_vs_per_shard"] = (out["per_shard"]["wall_ms"] /
+                                              out["persistent_unscreened"]["wall_ms"])
+    out["speedup_screen"] = out["persistent_unscreened"]["wall_ms"] / out["persistent"]["wall_ms"]
     dev.free()
     k.close()
     return out
@@ -376,7 +397,9 @@ def leg_c5(args, synth, ds_c3):
     bulk = []
     for _ in range(3):
         k.scan_device(dev.ptr, mk, rw, prm)
-        bulk.append(k.timing()["bulk_ms"])
+        tm = k.timing()
+        bulk.append(tm["bulk_ms"] + tm["screen_ms"])
+    surv = tm["survivors"]
     kb = min(bulk) / 1e3
     host = k.host_alloc((mh, rw))
     from supergwas_amd.api import lib
@@ -388,13 +411,13 @@ def leg_c5(args, synth, ds_c3):
         tm = k.timing()
         walls.append(tm["wall_ms"])
         copies.append(tm["copy_ms"])
-        kern.append(tm["bulk_ms"])
+        kern.append(tm["bulk_ms"] + tm["screen_ms"])
     k.host_free(host)
     w = min(walls) / 1e3
     out = {"workload": "n=4096, p=64, t=8 (corr 0.2, 100 causal each), 3 int8 limbs",
            "autotune": {"tiles_per_wave": tune[0], "workgroups_per_cu": tune[1]},
            "kernel_only": {"m": mk, "kmers_per_s": mk / kb, "GBps": mk * rw * 8 / kb / 1e9,
-                           "trait_tests_per_s": 8 * mk / kb},
+                           "trait_tests_per_s": 8 * mk / kb, "survivors": int(surv)},
            "streamed": {"m_per_pass": mh, "kmers_per_s": mh / w, "GBps": mh * rw * 8 / w / 1e9,
                         "copy_ms": min(copies), "kernel_ms": min(kern), "wall_ms": 1e3 * w,
                         "chunk_MiB": 64, "host_memory": "page-locked (kg_host_alloc)",

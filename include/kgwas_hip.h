@@ -40,9 +40,10 @@ extern "C" {
  * before the projections; the screen does the same on the device): a cheap
  * bulk pass drops the k-mers whose pre-filter fails for certain, and the full
  * kernel runs on the rest.  Counts and hits are identical either way.
- * AUTO screens whenever corr_threshold > 0 and keeps screening while at most
- * half of the k-mers survive; OFF never screens; ON always screens where the
- * model allows (n_acc <= 8192 accessions). */
+ * AUTO screens whenever corr_threshold > 0 and keeps screening while the
+ * screen plus the survivor scan take less time than the full kernel over every
+ * k-mer would (measured per scan; after a loss it skips 8 scans); OFF never
+ * screens; ON always screens where the model allows (n_acc <= 8192). */
 #define KG_SCREEN_AUTO 0
 #define KG_SCREEN_OFF 1
 #define KG_SCREEN_ON 2
@@ -80,14 +81,15 @@ typedef struct kg_hit {
 
 /* Timings of the last scan on the context's stream (HIP events), ms. */
 typedef struct kg_timing {
-  double bulk_ms;       /* bulk scan kernel (the full kernel; after the screen when screened) */
+  double bulk_ms;       /* bulk scan kernel (when screened: the survivor-table fix-up and the
+                           full kernel over the survivors) */
   double refine_ms;     /* FP64 refinement kernel (0 for KG_PREC_FP32) */
   double total_ms;      /* first to last device operation of the scan */
   int64_t candidates;   /* k-mers sent to refinement / host decision */
   int64_t kmers;        /* k-mers scanned */
   double copy_ms;       /* kg_scan: host-to-device copies (copy stream, summed) */
   double wall_ms;       /* kg_scan: host wall time of the whole call */
-  double screen_ms;     /* pre-filter screen kernel + its descriptor fix-up (0 if unscreened) */
+  double screen_ms;     /* pre-filter screen kernel (0 if unscreened) */
   int64_t survivors;    /* k-mers the screen passed to the full kernel (= kmers if unscreened) */
 } kg_timing;
 
@@ -159,6 +161,10 @@ int kg_stats_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t row_
                     int32_t* c, double* xxM, double* xyM, double* S);
 
 int kg_last_timing(const kg_ctx* ctx, kg_timing* out);
+
+/* Changes the context's pre-filter screen mode (KG_SCREEN_*) for later scans
+ * (no reference counterpart: the reference always filters first). */
+int kg_set_screen(kg_ctx* ctx, int32_t mode);
 
 /* Launch-shape autotuner (no reference counterpart; the reference has one
  * code path): times the int8 bulk scan of `sample` random rows with model 0

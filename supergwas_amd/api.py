@@ -89,6 +89,7 @@ SIGNATURES = {
                         ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "kg_stats_device": ([_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp], ctypes.c_int),
     "kg_last_timing": ([_vp, ctypes.POINTER(_Timing)], ctypes.c_int),
+    "kg_set_screen": ([_vp, _i32], ctypes.c_int),
     "kg_dev_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
     "kg_dev_free": ([_vp, _vp], ctypes.c_int),
     "kg_host_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
@@ -291,6 +292,10 @@ class KgwasHip:
         self._check(lib().kg_memcpy(self._h, d.ptr, _p(arr), arr.nbytes, 0))
         d.shape, d.dtype = arr.shape, arr.dtype
         return d
+
+    def set_screen(self, mode):
+        """Pre-filter screen mode for later scans: "auto", "off" or "on"."""
+        self._check(lib().kg_set_screen(self._h, SCREEN[mode]))
 
     def timing(self):
         t = _Timing()

@@ -201,7 +201,7 @@ struct kg_ctx {
   int screen_mode = KG_SCREEN_AUTO;
   int screen_skip = 0;
   DBuf surv, ssdesc, sdesc2;  // survivor rows; graph-path screen descriptor tables
-  hipEvent_t evs = nullptr;   // end of screen + fixup
+  hipEvent_t evs = nullptr;   // end of the screen kernel
   int scr_occ_key = -1, scr_occ = 1;  // cached screen workgroups per CU
   // captured graph of the last batched scan
   hipGraph_t graph = nullptr;
@@ -745,6 +745,7 @@ extern "C" int kg_set_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
   ctx->models.clear();
+  ctx->screen_skip = 0;
   return add_model(ctx, n_acc, idxs, n, U, p, Y, t, nullptr);
 }
 
@@ -838,9 +839,9 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
   KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (sr.on) {
     KG_CK(ctx, launch_screen(t, sr.nch_max, sr.sgrid, ctx->stream, sr.d_ss, nsh, sr.sblocks));
+    KG_CK(ctx, hipEventRecord(ctx->evs, ctx->stream));
     KG_CK(ctx, launch_screen_fixup(ctx->stream, sr.d_ss, sr.d_sd2, nsh,
                                    i8_block_kmers(NT, L, t, ctx->tiles), sr.d_total));
-    KG_CK(ctx, hipEventRecord(ctx->evs, ctx->stream));
     KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream, sr.d_sd2, nsh,
                               total_blocks, o, sr.d_total));
   } else if (!per_launch) {
@@ -1194,9 +1195,13 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     int64_t ns = 0;
     for (int s = 0; s < nsh; ++s) ns += (int64_t)cnt[base_surv + s];
     ctx->timing.survivors = ns;
-    // auto: a screen that keeps more than half of the k-mers costs more than
-    // it saves; skip it for the next 8 scans, then try again
-    if (ctx->screen_mode == KG_SCREEN_AUTO && 2 * ns > mtot) ctx->screen_skip = 8;
+    // auto: when the screen plus the survivor scan took longer than the full
+    // kernel would have over every k-mer (its per-survivor time scaled to all
+    // k-mers), skip the screen for the next 8 scans, then measure again
+    if (ctx->screen_mode == KG_SCREEN_AUTO && !use_graph && ns > 0 &&
+        ctx->timing.screen_ms + ctx->timing.bulk_ms >
+            ctx->timing.bulk_ms * ((double)mtot / (double)ns))
+      ctx->screen_skip = 8;
   }
   for (int s = 0; s < nsh; ++s)
     for (int tr = 0; tr < t; ++tr) n_passed[s * t + tr] += (int64_t)cnt[1 + (size_t)s * t + tr];
@@ -1604,6 +1609,15 @@ extern "C" int kg_stats_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m,
   KG_CK(ctx, hipMemcpy(xxM, ctx->st_xx.p, sizeof(double) * m, hipMemcpyDeviceToHost));
   KG_CK(ctx, hipMemcpy(xyM, ctx->st_xy.p, sizeof(double) * m * t, hipMemcpyDeviceToHost));
   KG_CK(ctx, hipMemcpy(S, ctx->st_S.p, sizeof(double) * m * t, hipMemcpyDeviceToHost));
+  return KG_OK;
+}
+
+extern "C" int kg_set_screen(kg_ctx* ctx, int32_t mode) {
+  if (!ctx) return KG_ERR_ARG;
+  if (mode < KG_SCREEN_AUTO || mode > KG_SCREEN_ON)
+    return fail(ctx, KG_ERR_ARG, "screen mode must be KG_SCREEN_AUTO, _OFF or _ON");
+  ctx->screen_mode = mode;
+  ctx->screen_skip = 0;
   return KG_OK;
 }
 
