@@ -27,13 +27,17 @@ namespace kg {
 hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t s,
                           const ShardDesc* sd, int nshards, int64_t total_blocks,
                           const ScanOut& o, const int64_t* total_dev);
-int64_t screen_block_kmers();
+int64_t screen_block_kmers(int t);
+int screen_rows(int t);
+bool screen_count_row(int t);
 int screen_max_chunks();
 int screen_wgs_per_cu(int t, int nch);
 hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
                          int nshards, int64_t total_blocks);
-hipError_t launch_screen_fixup(hipStream_t s, const ScreenDesc* ss, ShardDesc* sd2, int nsh,
-                               int64_t bk, int64_t* total);
+int screen_chunk_words();
+hipError_t launch_screen_compact(hipStream_t s, const ScrChunk* ch, int ncc, const ScreenDesc* ss,
+                                 uint32_t* ccount, int64_t* P, ShardDesc* sd2, int nsh,
+                                 int64_t bk, int64_t* total);
 int64_t i8_block_kmers(int NT, int L, int t, int R_req);
 int i8_tiles_for(int NT, int L, int t, int R_req);
 int i8_column_capacity(int NT, int L);
@@ -177,7 +181,7 @@ struct kg_ctx {
   // alternates between two), so repeated shapes skip the upload
   struct DescSlot {
     std::vector<char> key;
-    DBuf sd, sd1, rd, ss, sd2;
+    DBuf sd, sd1, rd, ss, sd2, ch;
     uint64_t used = 0;
   };
   DescSlot dslot[4];
@@ -200,7 +204,8 @@ struct kg_ctx {
   // scans left before the screen is tried again
   int screen_mode = KG_SCREEN_AUTO;
   int screen_skip = 0;
-  DBuf surv, ssdesc, sdesc2;  // survivor rows; graph-path screen descriptor tables
+  DBuf surv, ssdesc, sdesc2, chdesc;  // survivor rows; graph-path screen tables
+  DBuf bmap, scr_aux;                 // screen bitmaps; chunk counts and prefixes
   hipEvent_t evs = nullptr;   // end of the screen kernel
   int scr_occ_key = -1, scr_occ = 1;  // cached screen workgroups per CU
   // captured graph of the last batched scan
@@ -497,7 +502,8 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
 }
 
 // Pre-filter screen weights: y - ybar of every trait as two balanced int8
-// limbs, v_j = w (254 q_0 + q_1), in the screen kernel's operand layout
+// limbs, v_j = w (254 q_0 + q_1), and (for t <= 7) a count row holding 1 for
+// every phenotyped accession, in the screen kernel's operand layout
 // (ScreenDesc, kg_internal.h).  h bounds |(y_j - exact mean) - v_j|: the
 // measured limb residual plus the FP64 rounding of the mean and of y - ybar.
 static int build_screen(kg_ctx* ctx, ModelDev& M) {
@@ -505,7 +511,8 @@ static int build_screen(kg_ctx* ctx, ModelDev& M) {
   const int n = md.n, t = md.t, nch = md.nch_i8;
   M.screen_ok = false;
   if (t > kMaxTraits || nch > screen_max_chunks()) return KG_OK;
-  const int Q = t <= 2 ? 4 : (t <= 4 ? 8 : 16), KG = 16 / Q;
+  const int Q = screen_rows(t), KG = 16 / Q;
+  const bool count_row = screen_count_row(t);
   std::vector<std::vector<int8_t>> lim(t);
   std::vector<double> yc(n);
   for (int tr = 0; tr < t; ++tr) {
@@ -536,12 +543,14 @@ static int build_screen(kg_ctx* ctx, ModelDev& M) {
         const int g = lane >> 4, row = lane & 15;
         if (row / Q != k % KG) continue;  // rows of another shift group
         const int qn = row % Q, tr = qn >> 1, l = qn & 1;
-        if (tr >= t) continue;
+        const bool is_count = count_row && qn == 2 * t;
+        if (tr >= t && !is_count) continue;
         int8_t* dst = &W[(size_t)C * per + ((size_t)k * 64 + lane) * 16];
         for (int e = 0; e < 16; ++e) {
           // operand byte e = 4 i + q: bit 8 q + k of the lane group's dword i
           const int a = 512 * C + 128 * g + 32 * (e >> 2) + 8 * (e & 3) + k;
-          if (a < md.n_acc && samp[a] >= 0) dst[e] = lim[tr][(size_t)samp[a] * 2 + l];
+          if (a < md.n_acc && samp[a] >= 0)
+            dst[e] = is_count ? (int8_t)1 : lim[tr][(size_t)samp[a] * 2 + l];
         }
       }
   KG_CK(ctx, M.swts.ensure(W.size()));
@@ -821,7 +830,11 @@ struct ScreenRun {
   bool on = false;
   const ScreenDesc* d_ss = nullptr;
   ShardDesc* d_sd2 = nullptr;   // full-kernel table over the survivor lists
-  int64_t* d_total = nullptr;   // full-kernel block count, written by the fix-up
+  int64_t* d_total = nullptr;   // full-kernel block count, written by scr_scan
+  const ScrChunk* d_ch = nullptr;
+  int ncc = 0;                  // compaction chunks
+  uint32_t* ccount = nullptr;
+  int64_t* P = nullptr;
   int64_t sblocks = 0;          // screen blocks
   int sgrid = 0;
   int nch_max = 0;
@@ -840,8 +853,9 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
   if (sr.on) {
     KG_CK(ctx, launch_screen(t, sr.nch_max, sr.sgrid, ctx->stream, sr.d_ss, nsh, sr.sblocks));
     KG_CK(ctx, hipEventRecord(ctx->evs, ctx->stream));
-    KG_CK(ctx, launch_screen_fixup(ctx->stream, sr.d_ss, sr.d_sd2, nsh,
-                                   i8_block_kmers(NT, L, t, ctx->tiles), sr.d_total));
+    KG_CK(ctx, launch_screen_compact(ctx->stream, sr.d_ch, sr.ncc, sr.d_ss, sr.ccount, sr.P,
+                                     sr.d_sd2, nsh, i8_block_kmers(NT, L, t, ctx->tiles),
+                                     sr.d_total));
     KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream, sr.d_sd2, nsh,
                               total_blocks, o, sr.d_total));
   } else if (!per_launch) {
@@ -948,17 +962,25 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     R.t = md.t;
     R.n = md.n;
   }
-  // screen tables: survivors of shard s go to surv[off_s, off_s + m_s)
+  // screen tables: survivors of shard s go to surv[off_s, off_s + m_s); its
+  // bitmap covers its screen blocks; compaction chunks of at most
+  // screen_chunk_words() bitmap words never straddle shards
   std::vector<ScreenDesc> ss;
   std::vector<ShardDesc> sd2;
+  std::vector<ScrChunk> chunks;
   ScreenRun sr;
   if (screen) {
+    const int64_t sbk = screen_block_kmers(t);
+    const int64_t cw = screen_chunk_words();
+    int64_t bm_words = 0;
+    for (int s = 0; s < nsh; ++s) bm_words += (sh[s].m + sbk - 1) / sbk * (sbk / 32);
     KG_CK(ctx, ctx->surv.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(mtot, 1)));
+    KG_CK(ctx, ctx->bmap.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(bm_words, 1)));
     uint32_t* sv = ctx->surv.as<uint32_t>();
-    const int64_t sbk = screen_block_kmers();
+    uint32_t* bmw = ctx->bmap.as<uint32_t>();
     ss.resize(nsh);
     sd2 = sd;
-    int64_t off = 0;
+    int64_t off = 0, boff = 0;
     for (int s = 0; s < nsh; ++s) {
       const ModelDev& M = *ctx->models[sh[s].model];
       const Model& md = M.md;
@@ -970,8 +992,20 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       sr.sblocks += (sh[s].m + sbk - 1) / sbk;
       S.wts = M.swts.p;
       S.mask = M.mask.p;
+      S.bmap = bmw + boff;
+      boff += (sh[s].m + sbk - 1) / sbk * (sbk / 32);
       S.list = sv + off;
       S.nsurv = dc + base_surv + s;
+      S.chunk0 = (int)chunks.size();
+      const int64_t used_words = (sh[s].m + 31) / 32;
+      for (int64_t w0 = 0; w0 < used_words; w0 += cw) {
+        ScrChunk c{};
+        c.shard = s;
+        c.w0 = w0;
+        c.w1 = std::min<int64_t>(w0 + cw, used_words);
+        chunks.push_back(c);
+      }
+      S.chunk1 = (int)chunks.size();
       S.nchunks = md.nch_i8;
       S.t = md.t;
       S.n = md.n;
@@ -992,6 +1026,11 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     sr.on = true;
     sr.nch_max = nch_max;
     sr.d_total = reinterpret_cast<int64_t*>(dc + base_surv + nsh);
+    sr.ncc = (int)chunks.size();
+    KG_CK(ctx, ctx->scr_aux.ensure(sizeof(uint32_t) * (chunks.size() + 2) +
+                                   sizeof(int64_t) * (chunks.size() + 2)));
+    sr.P = ctx->scr_aux.as<int64_t>();
+    sr.ccount = reinterpret_cast<uint32_t*>(sr.P + chunks.size() + 2);
     const int okey = t * 64 + nch_max;
     if (ctx->scr_occ_key != okey) {
       ctx->scr_occ = screen_wgs_per_cu(t, nch_max);
@@ -1033,7 +1072,9 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     std::vector<char> key;
     if (use_graph) {
       const size_t kb = sizeof(ShardDesc) * nsh * 2 + sizeof(RefDesc) * nsh + sizeof(o) + 16 +
-                        (screen ? (sizeof(ScreenDesc) + sizeof(ShardDesc)) * nsh + 16 : 0);
+                        (screen ? (sizeof(ScreenDesc) + sizeof(ShardDesc)) * nsh + 32 +
+                                      sizeof(ScrChunk) * chunks.size()
+                                : 0);
       key.resize(kb);
       char* kp = key.data();
       if (screen) {
@@ -1041,9 +1082,13 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
         kp += sizeof(ScreenDesc) * nsh;
         memcpy(kp, sd2.data(), sizeof(ShardDesc) * nsh);
         kp += sizeof(ShardDesc) * nsh;
-        const int64_t sx[2] = {sr.sgrid, sr.nch_max};
-        memcpy(kp, sx, 16);
-        kp += 16;
+        // launch shape and the compaction scratch the captured kernels use
+        const int64_t sx[4] = {sr.sgrid, sr.nch_max, (int64_t)(uintptr_t)sr.P,
+                               (int64_t)(uintptr_t)sr.ccount};
+        memcpy(kp, sx, 32);
+        kp += 32;
+        if (!chunks.empty()) memcpy(kp, chunks.data(), sizeof(ScrChunk) * chunks.size());
+        kp += sizeof(ScrChunk) * chunks.size();
       }
       memcpy(kp, sd.data(), sizeof(ShardDesc) * nsh);
       kp += sizeof(ShardDesc) * nsh;
@@ -1073,24 +1118,31 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
                                hipMemcpyHostToDevice));
           KG_CK(ctx, hipMemcpy(ctx->sdesc2.p, sd2.data(), sizeof(ShardDesc) * nsh,
                                hipMemcpyHostToDevice));
+          KG_CK(ctx, ctx->chdesc.ensure(sizeof(ScrChunk) * std::max<size_t>(chunks.size(), 1)));
+          if (!chunks.empty())
+            KG_CK(ctx, hipMemcpy(ctx->chdesc.p, chunks.data(), sizeof(ScrChunk) * chunks.size(),
+                                 hipMemcpyHostToDevice));
         }
       }
       if (screen) {
         sr.d_ss = ctx->ssdesc.as<ScreenDesc>();
         sr.d_sd2 = ctx->sdesc2.as<ShardDesc>();
+        sr.d_ch = ctx->chdesc.as<ScrChunk>();
       }
     } else {
       // descriptor tables: reuse an identical recent upload (every earlier scan
       // has completed, so no kernel still reads a slot that gets replaced)
       const size_t b0 = sizeof(ShardDesc) * nsh, b2 = sizeof(RefDesc) * nsh;
       const size_t b3 = screen ? sizeof(ScreenDesc) * nsh : 0, b4 = screen ? b0 : 0;
-      std::vector<char> dk(2 * b0 + b2 + b3 + b4);
+      const size_t b5 = screen ? sizeof(ScrChunk) * chunks.size() : 0;
+      std::vector<char> dk(2 * b0 + b2 + b3 + b4 + b5);
       memcpy(dk.data(), sd.data(), b0);
       memcpy(dk.data() + b0, sd1.data(), b0);
       memcpy(dk.data() + 2 * b0, rd.data(), b2);
       if (screen) {
         memcpy(dk.data() + 2 * b0 + b2, ss.data(), b3);
         memcpy(dk.data() + 2 * b0 + b2 + b3, sd2.data(), b4);
+        if (b5) memcpy(dk.data() + 2 * b0 + b2 + b3 + b4, chunks.data(), b5);
       }
       kg_ctx::DescSlot* slot = nullptr;
       for (auto& e : ctx->dslot)
@@ -1111,6 +1163,8 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
           KG_CK(ctx, slot->sd2.ensure(b4));
           KG_CK(ctx, hipMemcpy(slot->ss.p, ss.data(), b3, hipMemcpyHostToDevice));
           KG_CK(ctx, hipMemcpy(slot->sd2.p, sd2.data(), b4, hipMemcpyHostToDevice));
+          KG_CK(ctx, slot->ch.ensure(std::max<size_t>(b5, 1)));
+          if (b5) KG_CK(ctx, hipMemcpy(slot->ch.p, chunks.data(), b5, hipMemcpyHostToDevice));
         }
         slot->key = std::move(dk);
       }
@@ -1121,6 +1175,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       if (screen) {
         sr.d_ss = slot->ss.as<ScreenDesc>();
         sr.d_sd2 = slot->sd2.as<ShardDesc>();
+        sr.d_ch = slot->ch.as<ScrChunk>();
       }
     }
     KG_CK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));

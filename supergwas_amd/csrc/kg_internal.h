@@ -93,20 +93,30 @@ struct ShardDesc {
 //
 // Weights: [chunk][shift k][lane] 16-byte MFMA A operands.  Lane (g, row),
 // operand byte e = 4 i + q holds accession 512 C + 128 g + 32 i + 8 q + k;
-// row = Q (k % KG) + 2 tr + l for trait tr, limb l (Q = 4, 8 or 16 rows per
-// shift group, KG = 16 / Q), zero in the other rows.
+// row = Q (k % KG) + 2 tr + l for trait tr, limb l, and row Q (k % KG) + 2 t
+// is the count row (1 for phenotyped accessions; t <= 7); Q = 4, 8 or 16
+// rows per shift group, KG = 16 / Q; zero in the other rows.
 struct ScreenDesc {
   const uint64_t* bits;
   int64_t m, rw;
   int64_t blk0;                  // first global screen block of this shard
   const void* wts;               // screen weights (above)
   const void* mask;              // accession mask words (as ShardDesc)
-  uint32_t* list;                // survivors: rows within the shard, unordered
-  unsigned long long* nsurv;     // survivor count
+  uint32_t* bmap;                // kept k-mers: bit j % 32 of word j / 32
+  uint32_t* list;                // survivors: rows within the shard, increasing
+  unsigned long long* nsurv;     // survivor count (written by the compaction)
+  int chunk0, chunk1;            // this shard's compaction chunks [chunk0, chunk1)
   int nchunks, t, n, pad;
   double w[kMaxTraits];          // S~ = w_tr (254 A_0 + A_1), A_l = limb sums
   double h[kMaxTraits];          // |S - S~| <= c h_tr (quantisation + rounding)
   double thr[kMaxTraits];        // tau^2 |y - ybar|^2 / n; < 0: trait never passes
+};
+
+// Compaction chunk: words [w0, w1) of shard `shard`'s screen bitmap.
+struct ScrChunk {
+  int shard;
+  int pad;
+  int64_t w0, w1;
 };
 
 // FP64 refinement inputs of one shard.

@@ -27,6 +27,19 @@ namespace kg {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// Global-address-space view of a pointer read from a descriptor.  Loads
+// through a generic pointer become flat loads, which also count against the
+// LDS wait counter: every LDS wait of the compute would then wait for the
+// loads still in flight as well.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gptr(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gptr_w(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 constexpr int kWaves = 4;        // waves per workgroup (FP32 scan, refinement)
 constexpr int kThreads = 64 * kWaves;
 // The int8 scan runs one 8-wave workgroup per CU (two waves per SIMD): two
@@ -529,177 +542,285 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 // and zero elsewhere, and the epilogue divides each row by its exact power
 // of two.  Workgroup: 8 independent waves; the model's weights (8 KiB per
 // chunk) sit in LDS for the whole launch.  Wave v of global block w owns the
-// shard's k-mers [(b*8 + v) * 16R, +16R), b = w - blk0.  Kept rows are
-// staged per wave in LDS and appended to the shard's list in bursts.
+// shard's k-mers [(b*8 + v) * 16R, +16R), b = w - blk0.  The wave's ballot of
+// kept k-mers is stored as bitmap words (no atomics); scr_count, scr_scan and
+// scr_write then compact the bitmaps into increasing row lists.
 constexpr int kScrWaves = 8;
 constexpr int kScrThreads = 64 * kScrWaves;
-constexpr int kScrBuf = 192;    // staged survivor rows per wave
 constexpr int kScrMaxChunks = 16;
-__host__ __device__ constexpr int scr_rows(int t) { return t <= 2 ? 4 : (t <= 4 ? 8 : 16); }
+constexpr int kScrChunkWords = 2048;  // bitmap words per compaction chunk
+// Weight rows per shift group: 2 limbs per trait plus, for t <= 7, a count
+// row (the phenotype mask, so that the MFMA also yields c); t = 8 counts with
+// popcounts instead.
+__host__ __device__ constexpr bool scr_count_row(int t) { return 2 * t + 1 <= 16; }
+__host__ __device__ constexpr int scr_rows(int t) {
+  return (2 * t + (scr_count_row(t) ? 1 : 0)) <= 4 ? 4 : ((2 * t + (scr_count_row(t) ? 1 : 0)) <= 8 ? 8 : 16);
+}
+// k-mer tiles per wave: 4 for the two-accumulator form (Q = 4), else 2; the
+// chunks per load group keep a group at 8 KiB of rows per wave.
+__host__ __device__ constexpr int scr_tiles(int Q) { return Q == 4 ? 4 : 2; }
+__host__ __device__ constexpr int scr_group_chunks(int R) { return 8 / R; }
+// LDS: the accession mask (64 B per chunk), then the weights (8 KiB per chunk).
 __host__ __device__ constexpr size_t scr_lds_bytes(int nch) {
-  return (size_t)kScrWaves * kScrBuf * 4 + (size_t)nch * 64 + (size_t)nch * 8 * 64 * 16;
+  return (size_t)nch * 64 + (size_t)nch * 8 * 64 * 16;
 }
 
-// Appends the staged rows of one wave to shard D's list.  Wave-uniform.
-__device__ __forceinline__ void scr_flush(const ScreenDesc* D, const uint32_t* buf, int& nbuf,
-                                          int lane) {
-  if (nbuf == 0) return;
-  unsigned long long base = 0;
-  if (lane == 0) base = atomicAdd(D->nsurv, (unsigned long long)nbuf);
-  base = __shfl(base, 0);
-  for (int i = lane; i < nbuf; i += 64) D->list[base + i] = buf[i];
-  nbuf = 0;
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+// Load group (block w of shard D, chunks C0 .. C0 + DC - 1) of this lane's R
+// rows.  Chunks past the model's last are loaded again as the last chunk
+// (never used): unconditional loads keep the wait counts exact.
+template <int R, int DC>
+__device__ __forceinline__ void scr_load(const ScreenDesc& D, int64_t w, int C0, int wave, int g,
+                                         int col, v4u (&q)[DC][R]) {
+  const int64_t kbase = ((w - D.blk0) * kScrWaves + wave) * (16 * R);
+  const int nch = D.nchunks;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int64_t k = kbase + 16 * r + col;
+    if (k > D.m - 1) k = D.m - 1;  // clamp: rows past m are read, never kept
+    const auto* rp = gptr(reinterpret_cast<const v4u*>(D.bits + k * D.rw)) + g;
+#pragma unroll
+    for (int u = 0; u < DC; ++u) q[u][r] = rp[4 * min(C0 + u, nch - 1)];
+  }
 }
 
-template <int Q, int R>
+// MFMA steps of one load group into the block's accumulators (and, without a
+// count row, the popcount of the phenotyped bits).
+template <int KG, int NA, int R, int DC, bool CR>
+__device__ __forceinline__ void scr_group(const v4u (&q)[DC][R], int C0, int nch, const v4i* wl,
+                                          const v4u* mk_lds, int g, int lane, v4i (&acc)[R][NA],
+                                          uint32_t (&cnt)[R]) {
+#pragma unroll
+  for (int u = 0; u < DC; ++u) {
+    if (C0 + u < nch) {
+      const int C = C0 + u;
+      if (!CR) {
+        const v4u mk = mk_lds[4 * C + g];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          cnt[r] += __popc(q[u][r].x & mk.x) + __popc(q[u][r].y & mk.y) +
+                    __popc(q[u][r].z & mk.z) + __popc(q[u][r].w & mk.w);
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const v4i a = wl[(C * 8 + k) * 64 + lane];
+        const uint32_t M = 0x01010101u << k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const v4i b = v4i{(int)(q[u][r].x & M), (int)(q[u][r].y & M), (int)(q[u][r].z & M),
+                            (int)(q[u][r].w & M)};
+          acc[r][k / KG] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[r][k / KG], 0, 0, 0);
+        }
+      }
+    }
+  }
+}
+
+// Software pipeline over load groups (block, first chunk): the next group's
+// rows (the same block's next chunks, or the next block's first) are loaded
+// into a second register set while the current group computes, then copied
+// into the current set (the copy is the only wait for them).
+template <int Q, bool CR>
 __global__ __launch_bounds__(kScrThreads) void screen_i8(const ScreenDesc* __restrict__ sd,
                                                          int nshards, int64_t total_blocks) {
   constexpr int KG = 16 / Q;   // shifts sharing an accumulator
   constexpr int NA = 8 / KG;   // accumulators per tile
-  constexpr int TT = Q / 2;    // traits an instantiation covers
-  constexpr int DC = 4;        // chunks loaded per group
+  constexpr int TT = Q / 2;    // trait slots an instantiation covers
+  constexpr int R = scr_tiles(Q);
+  constexpr int DC = scr_group_chunks(R);
   extern __shared__ v4i scr_lds[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, col = lane & 15;
-  uint32_t* const buf = reinterpret_cast<uint32_t*>(scr_lds) + wave * kScrBuf;
-  const uint4* const mk_lds = reinterpret_cast<const uint4*>(scr_lds + kScrWaves * kScrBuf / 4);
+  const v4u* const mk_lds = reinterpret_cast<const v4u*>(scr_lds);
   // output rows 4 g .. 4 g + 3 of accumulator a: shift a KG + g / (Q / 4),
-  // traits tb and tb + 1 (limbs 0, 1 each)
+  // trait slots tb and tb + 1 (limbs 0, 1 each); the count row is limb 0 of
+  // slot t
   const int ksh = g / (Q / 4);
   const int tb = 2 * (g % (Q / 4));
 
   int64_t w = blockIdx.x;
   if (w >= total_blocks) return;
-  int s = 0, bs = -1, nbuf = 0;
+  int s = 0;
+  while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
+  // first block of the shard after s (read again only when s moves)
+  int64_t next0 = (s + 1 < nshards) ? sd[s + 1].blk0 : total_blocks;
+  int C0 = 0;
   const void* staged = nullptr;
   int nch_staged = 0;
-  while (w < total_blocks) {
-    while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
+  v4i acc[R][NA];
+  uint32_t cnt[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    cnt[r] = 0;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) acc[r][a] = v4i{0, 0, 0, 0};
+  }
+  v4u cq[DC][R], nq[DC][R];
+  // first group through the prefetch set as well, so that the current set is
+  // complete at the loop head on every path
+  scr_load<R, DC>(sd[s], w, C0, wave, g, col, nq);
+#pragma unroll
+  for (int u = 0; u < DC; ++u)
+#pragma unroll
+    for (int r = 0; r < R; ++r) cq[u][r] = nq[u][r];
+
+  while (true) {
     const ScreenDesc& D = sd[s];
-    if (bs != s) {
-      if (bs >= 0) scr_flush(&sd[bs], buf, nbuf, lane);
-      bs = s;
-    }
     const int nch = D.nchunks;
-    if (D.wts != staged) {  // workgroup-uniform: every wave walks the same blocks
-      __syncthreads();
-      const v4i* __restrict__ gw = reinterpret_cast<const v4i*>(D.wts);
-      const uint4* __restrict__ gm = reinterpret_cast<const uint4*>(D.mask);
-      v4i* const wl = scr_lds + kScrWaves * kScrBuf / 4 + 4 * nch;
-      for (int i = threadIdx.x; i < nch * 512; i += kScrThreads) wl[i] = gw[i];
-      for (int i = threadIdx.x; i < nch * 4; i += kScrThreads)
-        reinterpret_cast<uint4*>(scr_lds + kScrWaves * kScrBuf / 4)[i] = gm[i];
-      __syncthreads();
-      staged = D.wts;
-      nch_staged = nch;
+    const bool last_group = C0 + DC >= nch;
+    const int64_t w2 = last_group ? w + gridDim.x : w;
+    const int C02 = last_group ? 0 : C0 + DC;
+    int s2 = s;
+    while (w2 >= next0 && s2 + 1 < nshards) {
+      ++s2;
+      next0 = (s2 + 1 < nshards) ? sd[s2 + 1].blk0 : total_blocks;
     }
-    const v4i* const wl = scr_lds + kScrWaves * kScrBuf / 4 + 4 * nch_staged;
-    const int64_t kbase = ((w - D.blk0) * kScrWaves + wave) * (16 * R);
-    const uint4* rp[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int64_t k = kbase + 16 * r + col;
-      if (k > D.m - 1) k = D.m - 1;  // clamp: rows past m are read, never kept
-      rp[r] = reinterpret_cast<const uint4*>(D.bits + k * D.rw) + g;
+    if (w2 < total_blocks) scr_load<R, DC>(sd[s2], w2, C02, wave, g, col, nq);
+
+    if (C0 == 0) {
+      if (D.wts != staged) {  // workgroup-uniform: every wave walks the same groups
+        __syncthreads();
+        const v4i* __restrict__ gw = reinterpret_cast<const v4i*>(D.wts);
+        const v4u* __restrict__ gm = reinterpret_cast<const v4u*>(D.mask);
+        v4i* const wd = scr_lds + 4 * nch;
+        for (int i = threadIdx.x; i < nch * 512; i += kScrThreads) wd[i] = gw[i];
+        for (int i = threadIdx.x; i < nch * 4; i += kScrThreads)
+          reinterpret_cast<v4u*>(scr_lds)[i] = gm[i];
+        __syncthreads();
+        staged = D.wts;
+        nch_staged = nch;
+      }
     }
-    v4i acc[R][NA];
-    uint32_t cnt[R];
+    const v4i* const wl = scr_lds + 4 * nch_staged;
+    scr_group<KG, NA, R, DC, CR>(cq, C0, nch, wl, mk_lds, g, lane, acc, cnt);
+
+    if (last_group) {
+      // exact row totals over the shifts (every row of accumulator a is a
+      // multiple of 2^k, k = a KG + ksh), then per-lane partials
+      // 254 A_0 + A_1 of trait slots tb, tb + 1 and of the count row
+      int part[R][2];
+      uint32_t cpart[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      cnt[r] = 0;
+      for (int r = 0; r < R; ++r) {
+        int v[4] = {0, 0, 0, 0};
 #pragma unroll
-      for (int a = 0; a < NA; ++a) acc[r][a] = v4i{0, 0, 0, 0};
-    }
-    for (int C0 = 0; C0 < nch; C0 += DC) {
-      uint4 bq[DC][R];
+        for (int a = 0; a < NA; ++a) {
+          const int kk = a * KG + ksh;
+          const int sg = (kk == 7) ? -1 : 1;
 #pragma unroll
-      for (int u = 0; u < DC; ++u)
-        if (C0 + u < nch)
+          for (int i = 0; i < 4; ++i) v[i] += sg * (acc[r][a][i] >> kk);
+        }
+        part[r][0] = 254 * v[0] + v[1];
+        part[r][1] = 254 * v[2] + v[3];
+        if (CR) cpart[r] = (uint32_t)((tb == D.t) ? v[0] : ((tb + 1 == D.t) ? v[2] : 0));
+      }
+      const int c = CR ? (int)group_scatter<R>(cpart) : (int)group_scatter<R>(cnt);
+      const int64_t kbase = ((w - D.blk0) * kScrWaves + wave) * (16 * R);
+      const int64_t kidx = kbase + 16 * (g % R) + col;
+      const bool valid = (g < R) && kidx < D.m;
+      const int n = D.n;
+      const double cd = (double)c, vv = cd * (double)(n - c);
+      bool keep = false;
 #pragma unroll
-          for (int r = 0; r < R; ++r) bq[u][r] = rp[r][4 * (C0 + u)];
-#pragma unroll
-      for (int u = 0; u < DC; ++u) {
-        if (C0 + u < nch) {
-          const int C = C0 + u;
-          const uint4 mk = mk_lds[4 * C + g];
+      for (int tr = 0; tr < TT; ++tr) {
+        if (tr < D.t) {
+          uint32_t x[R];
 #pragma unroll
           for (int r = 0; r < R; ++r)
-            cnt[r] += __popc(bq[u][r].x & mk.x) + __popc(bq[u][r].y & mk.y) +
-                      __popc(bq[u][r].z & mk.z) + __popc(bq[u][r].w & mk.w);
-#pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            const v4i a = wl[(C * 8 + k) * 64 + lane];
-            const uint32_t M = 0x01010101u << k;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-              const v4i b = v4i{(int)(bq[u][r].x & M), (int)(bq[u][r].y & M),
-                                (int)(bq[u][r].z & M), (int)(bq[u][r].w & M)};
-              acc[r][k / KG] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[r][k / KG], 0, 0, 0);
-            }
-          }
+            x[r] = (uint32_t)((tb == tr) ? part[r][0] : ((tb + 1 == tr) ? part[r][1] : 0));
+          const int A = (int)group_scatter<R>(x);
+          const double S = D.w[tr] * (double)A;
+          const double hi = fabs(S) + cd * D.h[tr] + 1e-12 * (fabs(S) + 1.0);
+          if (D.thr[tr] >= 0.0 && !(hi * hi < D.thr[tr] * vv * (1.0 - 3e-12))) keep = true;
         }
       }
-    }
-
-    // rows -> per-lane partial 254 A_0 + A_1 of traits tb, tb + 1 (exact:
-    // every row of accumulator a is a multiple of 2^k, k = a KG + ksh)
-    int part[R][2];
+      keep = keep && valid && c > 0 && c < n;
+      // lane l owns k-mer kbase + l (lanes < 16 R): the ballot is the bitmap
+      const unsigned long long bal = __ballot(keep);
+      if (lane == 0) {
+        auto* bm = gptr_w(D.bmap) + (kbase >> 5);
+        bm[0] = (uint32_t)bal;
+        if (R == 4) bm[1] = (uint32_t)(bal >> 32);
+      }
 #pragma unroll
-    for (int r = 0; r < R; ++r) part[r][0] = part[r][1] = 0;
+      for (int r = 0; r < R; ++r) {
+        cnt[r] = 0;
 #pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      const int kk = a * KG + ksh;
-      const int sg = (kk == 7) ? -1 : 1;
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          part[r][j] += sg * (254 * (acc[r][a][2 * j] >> kk) + (acc[r][a][2 * j + 1] >> kk));
-    }
-    const int c = (int)group_scatter<R>(cnt);
-    const int64_t kidx = kbase + 16 * (g % R) + col;
-    const bool valid = (g < R) && kidx < D.m;
-    const int n = D.n;
-    const double cd = (double)c, v = cd * (double)(n - c);
-    bool keep = false;
-#pragma unroll
-    for (int tr = 0; tr < TT; ++tr) {
-      if (tr < D.t) {
-        uint32_t x[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-          x[r] = (uint32_t)((tb == tr) ? part[r][0] : ((tb + 1 == tr) ? part[r][1] : 0));
-        const int A = (int)group_scatter<R>(x);
-        const double S = D.w[tr] * (double)A;
-        const double hi = fabs(S) + cd * D.h[tr] + 1e-12 * (fabs(S) + 1.0);
-        if (D.thr[tr] >= 0.0 && !(hi * hi < D.thr[tr] * v * (1.0 - 3e-12))) keep = true;
+        for (int a = 0; a < NA; ++a) acc[r][a] = v4i{0, 0, 0, 0};
       }
     }
-    keep = keep && valid && c > 0 && c < n;
-    const unsigned long long bal = __ballot(keep);
-    if (keep) buf[nbuf + __popcll(bal & ((1ull << lane) - 1ull))] = (uint32_t)kidx;
-    nbuf += __popcll(bal);
-    if (nbuf > kScrBuf - 64) scr_flush(&D, buf, nbuf, lane);
-    w += gridDim.x;
+    w = w2;
+    C0 = C02;
+    s = s2;
+    if (w >= total_blocks) break;
+#pragma unroll
+    for (int u = 0; u < DC; ++u)
+#pragma unroll
+      for (int r = 0; r < R; ++r) cq[u][r] = nq[u][r];
   }
-  if (bs >= 0) scr_flush(&sd[bs], buf, nbuf, lane);
 }
 
-// Pass-2 descriptors from the screen's survivor counts: sd2[s].m and .blk0
-// (blocks of bk k-mers, exclusive prefix) and the block total.  One workgroup
-// of 256 threads, each owning a contiguous run of shards.
-__global__ __launch_bounds__(256) void screen_fixup(const ScreenDesc* __restrict__ ss,
-                                                    ShardDesc* __restrict__ sd2, int nsh,
-                                                    int64_t bk, int64_t* __restrict__ total) {
-  __shared__ int64_t part[256];
+// Compaction of the screen bitmaps into row lists, in three launches over
+// chunks of at most kScrChunkWords bitmap words (a chunk lies in one shard):
+//   scr_count: kept k-mers per chunk;
+//   scr_scan : one workgroup: exclusive prefix P over the chunks, survivor
+//              counts per shard, and the full kernel's block table (sd2[s].m,
+//              .blk0 in blocks of bk k-mers, and the block total);
+//   scr_write: each chunk writes its rows at P[c] - P[chunk0 of its shard].
+__global__ __launch_bounds__(256) void scr_count(const ScrChunk* __restrict__ ch,
+                                                 const ScreenDesc* __restrict__ ss,
+                                                 uint32_t* __restrict__ ccount) {
+  __shared__ uint32_t red[4];
+  const ScrChunk C = ch[blockIdx.x];
+  const uint32_t* bm = ss[C.shard].bmap;
+  uint32_t v = 0;
+  for (int64_t w = C.w0 + threadIdx.x; w < C.w1; w += 256) v += __popc(bm[w]);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) ccount[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(1024) void scr_scan(const uint32_t* __restrict__ ccount, int ncc,
+                                                 int64_t* __restrict__ P,
+                                                 const ScreenDesc* __restrict__ ss,
+                                                 ShardDesc* __restrict__ sd2, int nsh, int64_t bk,
+                                                 int64_t* __restrict__ total) {
+  __shared__ int64_t part[1024];
   const int tid = threadIdx.x;
-  const int per = (nsh + 255) / 256;
+  // P[0..ncc]: exclusive prefix of the chunk counts
+  int64_t carry = 0;
+  for (int base = 0; base < ncc; base += 1024) {
+    const int i = base + tid;
+    const int64_t v = (i < ncc) ? (int64_t)ccount[i] : 0;
+    part[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int64_t add = (tid >= off) ? part[tid - off] : 0;
+      __syncthreads();
+      part[tid] += add;
+      __syncthreads();
+    }
+    if (i < ncc) P[i] = carry + part[tid] - v;
+    const int64_t blk = part[1023];
+    __syncthreads();
+    carry += blk;
+  }
+  if (tid == 0) P[ncc] = carry;
+  __syncthreads();
+  // survivors per shard, then the full kernel's block table
+  const int per = (nsh + 1023) / 1024;
   const int s0 = tid * per, s1 = min(nsh, s0 + per);
   int64_t sum = 0;
-  for (int s = s0; s < s1; ++s) sum += ((int64_t)*ss[s].nsurv + bk - 1) / bk;
+  for (int s = s0; s < s1; ++s) {
+    const int64_t m = P[ss[s].chunk1] - P[ss[s].chunk0];
+    *ss[s].nsurv = (unsigned long long)m;
+    sum += (m + bk - 1) / bk;
+  }
   part[tid] = sum;
   __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
+  for (int off = 1; off < 1024; off <<= 1) {
     const int64_t add = (tid >= off) ? part[tid - off] : 0;
     __syncthreads();
     part[tid] += add;
@@ -707,12 +828,49 @@ __global__ __launch_bounds__(256) void screen_fixup(const ScreenDesc* __restrict
   }
   int64_t run = part[tid] - sum;
   for (int s = s0; s < s1; ++s) {
-    const int64_t m = (int64_t)*ss[s].nsurv;
+    const int64_t m = P[ss[s].chunk1] - P[ss[s].chunk0];
     sd2[s].m = m;
     sd2[s].blk0 = run;
     run += (m + bk - 1) / bk;
   }
-  if (tid == 255) *total = part[255];
+  if (tid == 1023) *total = part[1023];
+}
+
+// Thread j of chunk c owns words [w0 + 8 j, +8): its rows go after those of
+// the chunk's lower threads.
+__global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch,
+                                                 const ScreenDesc* __restrict__ ss,
+                                                 const int64_t* __restrict__ P) {
+  __shared__ uint32_t pre[256];
+  const ScrChunk C = ch[blockIdx.x];
+  const ScreenDesc& D = ss[C.shard];
+  uint32_t wv[8];
+  uint32_t v = 0;
+  const int64_t wb = C.w0 + 8 * (int64_t)threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    wv[i] = (wb + i < C.w1) ? D.bmap[wb + i] : 0u;
+    v += __popc(wv[i]);
+  }
+  pre[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t add = (threadIdx.x >= (unsigned)off) ? pre[threadIdx.x - off] : 0u;
+    __syncthreads();
+    pre[threadIdx.x] += add;
+    __syncthreads();
+  }
+  int64_t o = P[blockIdx.x] - P[D.chunk0] + (int64_t)(pre[threadIdx.x] - v);
+  uint32_t* list = D.list;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint32_t x = wv[i];
+    while (x) {
+      const int b = __ffs(x) - 1;
+      x &= x - 1;
+      list[o++] = (uint32_t)(32 * (wb + i) + b);
+    }
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -890,12 +1048,34 @@ __global__ __launch_bounds__(kThreads) void refine_f64(const Cand* __restrict__ 
     for (int w = 0; w < R.nwords; ++w) {
       uint64_t word = row[w] & R.maskw[w];
       c += __popcll(word);
+      // set bits in batches of 8: the batch's column loads are all issued
+      // before its adds, which still run in accession order
       while (word) {
-        const int bit = __ffsll((long long)word) - 1;
-        word &= word - 1;
-        const double* Br = R.B64 + (size_t)(64 * w + bit) * nbf;
-        if (lane < nbf) a0 += Br[lane];
-        if (lane + 64 < nbf) a1 += Br[lane + 64];
+        int bit[8];
+        int nb = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          bit[i] = 0;
+          if (word) {
+            bit[i] = __ffsll((long long)word) - 1;
+            word &= word - 1;
+            nb = i + 1;
+          }
+        }
+        double v0[8], v1[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const double* Br = R.B64 + (size_t)(64 * w + bit[i]) * nbf;
+          v0[i] = (i < nb && lane < nbf) ? Br[lane] : 0.0;
+          v1[i] = (i < nb && lane + 64 < nbf) ? Br[lane + 64] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          if (i < nb) {
+            a0 += v0[i];
+            a1 += v1[i];
+          }
+        }
       }
     }
     const int tr = cd.trait;
@@ -1007,14 +1187,15 @@ hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t
 
 // Pre-filter screen over a shard table; nch_max = the largest model's 512-sample
 // chunks (sizes the LDS weight table).  grid = persistent workgroups.
-int64_t screen_block_kmers() { return (int64_t)kScrWaves * 16 * 2; }
-size_t screen_lds_bytes(int nch_max) { return scr_lds_bytes(nch_max); }
+int64_t screen_block_kmers(int t) { return (int64_t)kScrWaves * 16 * scr_tiles(scr_rows(t)); }
 int screen_max_chunks() { return kScrMaxChunks; }
+int screen_rows(int t) { return scr_rows(t); }
+bool screen_count_row(int t) { return scr_count_row(t); }
 
-template <int Q>
+template <int Q, bool CR>
 static hipError_t launch_screen_q(dim3 grid, size_t shm, hipStream_t s, const ScreenDesc* sd,
                                   int nshards, int64_t total_blocks) {
-  hipLaunchKernelGGL((screen_i8<Q, 2>), grid, dim3(kScrThreads), shm, s, sd, nshards,
+  hipLaunchKernelGGL((screen_i8<Q, CR>), grid, dim3(kScrThreads), shm, s, sd, nshards,
                      total_blocks);
   return hipGetLastError();
 }
@@ -1024,10 +1205,11 @@ hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const Scre
   if (nch_max > kScrMaxChunks || t < 1 || t > kMaxTraits) return hipErrorInvalidValue;
   dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
   const size_t shm = scr_lds_bytes(nch_max);
+  if (!scr_count_row(t)) return launch_screen_q<16, false>(g, shm, s, sd, nshards, total_blocks);
   switch (scr_rows(t)) {
-    case 4: return launch_screen_q<4>(g, shm, s, sd, nshards, total_blocks);
-    case 8: return launch_screen_q<8>(g, shm, s, sd, nshards, total_blocks);
-    default: return launch_screen_q<16>(g, shm, s, sd, nshards, total_blocks);
+    case 4: return launch_screen_q<4, true>(g, shm, s, sd, nshards, total_blocks);
+    case 8: return launch_screen_q<8, true>(g, shm, s, sd, nshards, total_blocks);
+    default: return launch_screen_q<16, true>(g, shm, s, sd, nshards, total_blocks);
   }
 }
 
@@ -1036,18 +1218,27 @@ int screen_wgs_per_cu(int t, int nch) {
   int nb = 0;
   const size_t shm = scr_lds_bytes(nch);
   hipError_t e;
-  switch (scr_rows(t)) {
-    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<4, 2>, kScrThreads, shm); break;
-    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<8, 2>, kScrThreads, shm); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, 2>, kScrThreads, shm); break;
-  }
+  if (!scr_count_row(t))
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, false>, kScrThreads, shm);
+  else if (scr_rows(t) == 4)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<4, true>, kScrThreads, shm);
+  else if (scr_rows(t) == 8)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<8, true>, kScrThreads, shm);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, true>, kScrThreads, shm);
   if (e != hipSuccess || nb < 1) nb = 1;
   return nb;
 }
 
-hipError_t launch_screen_fixup(hipStream_t s, const ScreenDesc* ss, ShardDesc* sd2, int nsh,
-                               int64_t bk, int64_t* total) {
-  hipLaunchKernelGGL(screen_fixup, dim3(1), dim3(256), 0, s, ss, sd2, nsh, bk, total);
+int screen_chunk_words() { return kScrChunkWords; }
+
+// Bitmap compaction and the full kernel's block table (see scr_scan).
+hipError_t launch_screen_compact(hipStream_t s, const ScrChunk* ch, int ncc, const ScreenDesc* ss,
+                                 uint32_t* ccount, int64_t* P, ShardDesc* sd2, int nsh,
+                                 int64_t bk, int64_t* total) {
+  if (ncc > 0) hipLaunchKernelGGL(scr_count, dim3((unsigned)ncc), dim3(256), 0, s, ch, ss, ccount);
+  hipLaunchKernelGGL(scr_scan, dim3(1), dim3(1024), 0, s, ccount, ncc, P, ss, sd2, nsh, bk, total);
+  if (ncc > 0) hipLaunchKernelGGL(scr_write, dim3((unsigned)ncc), dim3(256), 0, s, ch, ss, P);
   return hipGetLastError();
 }
 
@@ -1102,11 +1293,10 @@ static hipError_t set_attr_nt() {
 hipError_t set_kernel_attributes() {
   hipError_t e = hipSuccess;
   const int scr = (int)scr_lds_bytes(kScrMaxChunks);
-  e = hipFuncSetAttribute((const void*)screen_i8<4, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, scr);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)screen_i8<8, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, scr);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)screen_i8<16, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, scr);
+  const void* scr_fns[4] = {(const void*)screen_i8<4, true>, (const void*)screen_i8<8, true>,
+                            (const void*)screen_i8<16, true>, (const void*)screen_i8<16, false>};
+  for (const void* f : scr_fns)
+    if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, scr);
 #define KG_ATTR(X) \
   if (e == hipSuccess) e = set_attr_nt<X>();
   KG_ATTR(1) KG_ATTR(2) KG_ATTR(3) KG_ATTR(4) KG_ATTR(5) KG_ATTR(6) KG_ATTR(7) KG_ATTR(8)
