@@ -1010,13 +1010,15 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       S.t = md.t;
       S.n = md.n;
       S.pad = 0;
+      // FP32 screen constants: w and the threshold to nearest (their relative
+      // error sits inside the kernel's 1e-5 margin), h rounded up
       for (int tr = 0; tr < kMaxTraits; ++tr) {
-        S.w[tr] = tr < md.t ? M.sw[tr] : 0.0;
-        S.h[tr] = tr < md.t ? M.sh[tr] : 0.0;
-        S.thr[tr] = (tr < md.t && md.ycn[tr] >= 1e-12)
-                        ? prm.corr_threshold * prm.corr_threshold * md.ycn[tr] * md.ycn[tr] /
-                              (double)md.n
-                        : -1.0;
+        S.wf[tr] = tr < md.t ? (float)M.sw[tr] : 0.0f;
+        S.hf[tr] = tr < md.t ? (float)(M.sh[tr] * (1.0 + 1e-6)) : 0.0f;
+        S.thrf[tr] = (tr < md.t && md.ycn[tr] >= 1e-12)
+                         ? (float)(prm.corr_threshold * prm.corr_threshold * md.ycn[tr] *
+                                   md.ycn[tr] / (double)md.n)
+                         : -1.0f;
       }
       sd2[s].rows = S.list;
       sd2[s].m = 0;     // written on the device by the fix-up

@@ -107,9 +107,9 @@ struct ScreenDesc {
   unsigned long long* nsurv;     // survivor count (written by the compaction)
   int chunk0, chunk1;            // this shard's compaction chunks [chunk0, chunk1)
   int nchunks, t, n, pad;
-  double w[kMaxTraits];          // S~ = w_tr (254 A_0 + A_1), A_l = limb sums
-  double h[kMaxTraits];          // |S - S~| <= c h_tr (quantisation + rounding)
-  double thr[kMaxTraits];        // tau^2 |y - ybar|^2 / n; < 0: trait never passes
+  float wf[kMaxTraits];          // S~ = w_tr (254 A_0 + A_1), A_l = limb sums
+  float hf[kMaxTraits];          // |S - S~| <= c h_tr (quantisation + rounding), rounded up
+  float thrf[kMaxTraits];        // tau^2 |y - ybar|^2 / n; < 0: trait never passes
 };
 
 // Compaction chunk: words [w0, w1) of shard `shard`'s screen bitmap.

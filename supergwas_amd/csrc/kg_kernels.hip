@@ -719,8 +719,11 @@ __global__ __launch_bounds__(kScrThreads) void screen_i8(const ScreenDesc* __res
       const int64_t kbase = ((w - D.blk0) * kScrWaves + wave) * (16 * R);
       const int64_t kidx = kbase + 16 * (g % R) + col;
       const bool valid = (g < R) && kidx < D.m;
+      // Decision in FP32: every FP32 value below is within 2e-6 relative of
+      // its exact counterpart (a handful of correctly rounded operations on
+      // positive terms), so the margin 1e-5 keeps "certainly fails" certain.
       const int n = D.n;
-      const double cd = (double)c, vv = cd * (double)(n - c);
+      const float cf = (float)c, vv = cf * (float)(n - c);
       bool keep = false;
 #pragma unroll
       for (int tr = 0; tr < TT; ++tr) {
@@ -730,9 +733,8 @@ __global__ __launch_bounds__(kScrThreads) void screen_i8(const ScreenDesc* __res
           for (int r = 0; r < R; ++r)
             x[r] = (uint32_t)((tb == tr) ? part[r][0] : ((tb + 1 == tr) ? part[r][1] : 0));
           const int A = (int)group_scatter<R>(x);
-          const double S = D.w[tr] * (double)A;
-          const double hi = fabs(S) + cd * D.h[tr] + 1e-12 * (fabs(S) + 1.0);
-          if (D.thr[tr] >= 0.0 && !(hi * hi < D.thr[tr] * vv * (1.0 - 3e-12))) keep = true;
+          const float hi = fabsf(D.wf[tr] * (float)A) * (1.0f + 4e-6f) + cf * D.hf[tr];
+          if (D.thrf[tr] >= 0.0f && !(hi * hi < D.thrf[tr] * vv * (1.0f - 1e-5f))) keep = true;
         }
       }
       keep = keep && valid && c > 0 && c < n;
@@ -836,40 +838,35 @@ __global__ __launch_bounds__(1024) void scr_scan(const uint32_t* __restrict__ cc
   if (tid == 1023) *total = part[1023];
 }
 
-// Thread j of chunk c owns words [w0 + 8 j, +8): its rows go after those of
-// the chunk's lower threads.
+// Wave v of chunk c owns a quarter of its words; a wave pass covers two words
+// (lane l: bit l % 32 of word w + l / 32), so the ballot order is row order
+// and each pass stores its rows contiguously.
 __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch,
                                                  const ScreenDesc* __restrict__ ss,
                                                  const int64_t* __restrict__ P) {
-  __shared__ uint32_t pre[256];
+  __shared__ uint32_t wcnt[4];
   const ScrChunk C = ch[blockIdx.x];
   const ScreenDesc& D = ss[C.shard];
-  uint32_t wv[8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t per = (C.w1 - C.w0 + 3) / 4;
+  const int64_t a0 = min(C.w1, C.w0 + wave * per), a1 = min(C.w1, a0 + per);
+  const uint32_t* __restrict__ bm = D.bmap;
   uint32_t v = 0;
-  const int64_t wb = C.w0 + 8 * (int64_t)threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    wv[i] = (wb + i < C.w1) ? D.bmap[wb + i] : 0u;
-    v += __popc(wv[i]);
-  }
-  pre[threadIdx.x] = v;
+  for (int64_t w = a0 + lane; w < a1; w += 64) v += __popc(bm[w]);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (lane == 0) wcnt[wave] = v;
   __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const uint32_t add = (threadIdx.x >= (unsigned)off) ? pre[threadIdx.x - off] : 0u;
-    __syncthreads();
-    pre[threadIdx.x] += add;
-    __syncthreads();
-  }
-  int64_t o = P[blockIdx.x] - P[D.chunk0] + (int64_t)(pre[threadIdx.x] - v);
-  uint32_t* list = D.list;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    uint32_t x = wv[i];
-    while (x) {
-      const int b = __ffs(x) - 1;
-      x &= x - 1;
-      list[o++] = (uint32_t)(32 * (wb + i) + b);
-    }
+  int64_t o = P[blockIdx.x] - P[D.chunk0];
+  for (int i = 0; i < wave; ++i) o += wcnt[i];
+  uint32_t* __restrict__ list = D.list;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int64_t w = a0; w < a1; w += 2) {
+    const int64_t wl = w + (lane >> 5);
+    const uint32_t word = (wl < a1) ? bm[wl] : 0u;
+    const bool set = ((word >> (lane & 31)) & 1u) != 0;
+    const unsigned long long bal = __ballot(set);
+    if (set) list[o + __popcll(bal & below)] = (uint32_t)(32 * wl + (lane & 31));
+    o += __popcll(bal);
   }
 }
 
