@@ -531,7 +531,8 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 // the projections; this is the same order on the device).  Per k-mer it forms
 // S~ = x'(y - ybar) from two int8 limbs of y - ybar and the allele count c,
 // and keeps the k-mer unless |cor| < tau is certain for every trait:
-//   (|S~| + c h)^2 < tau^2 |y - ybar|^2 c (n - c) / n  (1 - 3e-12).
+//   (|S~| + c h)^2 < tau^2 |y - ybar|^2 c (n - c) / n  (1 - 1e-5)
+// (evaluated in FP32 with every rounding on the safe side).
 // The kept rows go to scan_i8, which decides them exactly as it decides an
 // unscreened scan, so screening changes no count and no hit.
 //
@@ -540,13 +541,12 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 // are 0 or 2^k (-128 for k = 7).  Each output row sees a single k: the
 // weights place shift k in rows Q (k % KG) .. + Q - 1 of accumulator k / KG
 // and zero elsewhere, and the epilogue divides each row by its exact power
-// of two.  Workgroup: 8 independent waves; the model's weights (8 KiB per
-// chunk) sit in LDS for the whole launch.  Wave v of global block w owns the
-// shard's k-mers [(b*8 + v) * 16R, +16R), b = w - blk0.  The wave's ballot of
+// of two.  Workgroup: W independent waves (scr_waves); the model's weights
+// (8 KiB per chunk) sit in LDS for the whole launch.  Wave v of global block
+// w owns the shard's k-mers [(b*W + v) * 16R, +16R), b = w - blk0.  The
+// wave's ballot of
 // kept k-mers is stored as bitmap words (no atomics); scr_count, scr_scan and
 // scr_write then compact the bitmaps into increasing row lists.
-constexpr int kScrWaves = 8;
-constexpr int kScrThreads = 64 * kScrWaves;
 constexpr int kScrMaxChunks = 16;
 constexpr int kScrChunkWords = 2048;  // bitmap words per compaction chunk
 // Weight rows per shift group: 2 limbs per trait plus, for t <= 7, a count
@@ -559,6 +559,10 @@ __host__ __device__ constexpr int scr_rows(int t) {
 // k-mer tiles per wave: 4 for the two-accumulator form (Q = 4), else 2; the
 // chunks per load group keep a group at 8 KiB of rows per wave.
 __host__ __device__ constexpr int scr_tiles(int Q) { return Q == 4 ? 4 : 2; }
+// Waves per workgroup: 16 (the whole CU's registers at 128 VGPRs) for Q = 4,
+// else 8; one workgroup then owns whole CUs, so a screen launched beside
+// another kernel takes whole CUs and leaves whole CUs.
+__host__ __device__ constexpr int scr_waves(int Q) { return Q == 4 ? 16 : 8; }
 __host__ __device__ constexpr int scr_group_chunks(int R) { return 8 / R; }
 // LDS: the accession mask (64 B per chunk), then the weights (8 KiB per chunk).
 __host__ __device__ constexpr size_t scr_lds_bytes(int nch) {
@@ -570,10 +574,10 @@ typedef unsigned v4u __attribute__((ext_vector_type(4)));
 // Load group (block w of shard D, chunks C0 .. C0 + DC - 1) of this lane's R
 // rows.  Chunks past the model's last are loaded again as the last chunk
 // (never used): unconditional loads keep the wait counts exact.
-template <int R, int DC>
+template <int W, int R, int DC>
 __device__ __forceinline__ void scr_load(const ScreenDesc& D, int64_t w, int C0, int wave, int g,
                                          int col, v4u (&q)[DC][R]) {
-  const int64_t kbase = ((w - D.blk0) * kScrWaves + wave) * (16 * R);
+  const int64_t kbase = ((w - D.blk0) * W + wave) * (16 * R);
   const int nch = D.nchunks;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -622,8 +626,10 @@ __device__ __forceinline__ void scr_group(const v4u (&q)[DC][R], int C0, int nch
 // into a second register set while the current group computes, then copied
 // into the current set (the copy is the only wait for them).
 template <int Q, bool CR>
-__global__ __launch_bounds__(kScrThreads) void screen_i8(const ScreenDesc* __restrict__ sd,
-                                                         int nshards, int64_t total_blocks) {
+__global__ __launch_bounds__(64 * scr_waves(Q)) void screen_i8(const ScreenDesc* __restrict__ sd,
+                                                              int nshards, int64_t total_blocks) {
+  constexpr int W = scr_waves(Q);
+  constexpr int NTH = 64 * W;
   constexpr int KG = 16 / Q;   // shifts sharing an accumulator
   constexpr int NA = 8 / KG;   // accumulators per tile
   constexpr int TT = Q / 2;    // trait slots an instantiation covers
@@ -659,7 +665,7 @@ __global__ __launch_bounds__(kScrThreads) void screen_i8(const ScreenDesc* __res
   v4u cq[DC][R], nq[DC][R];
   // first group through the prefetch set as well, so that the current set is
   // complete at the loop head on every path
-  scr_load<R, DC>(sd[s], w, C0, wave, g, col, nq);
+  scr_load<W, R, DC>(sd[s], w, C0, wave, g, col, nq);
 #pragma unroll
   for (int u = 0; u < DC; ++u)
 #pragma unroll
@@ -676,7 +682,7 @@ __global__ __launch_bounds__(kScrThreads) void screen_i8(const ScreenDesc* __res
       ++s2;
       next0 = (s2 + 1 < nshards) ? sd[s2 + 1].blk0 : total_blocks;
     }
-    if (w2 < total_blocks) scr_load<R, DC>(sd[s2], w2, C02, wave, g, col, nq);
+    if (w2 < total_blocks) scr_load<W, R, DC>(sd[s2], w2, C02, wave, g, col, nq);
 
     if (C0 == 0) {
       if (D.wts != staged) {  // workgroup-uniform: every wave walks the same groups
@@ -684,8 +690,8 @@ __global__ __launch_bounds__(kScrThreads) void screen_i8(const ScreenDesc* __res
         const v4i* __restrict__ gw = reinterpret_cast<const v4i*>(D.wts);
         const v4u* __restrict__ gm = reinterpret_cast<const v4u*>(D.mask);
         v4i* const wd = scr_lds + 4 * nch;
-        for (int i = threadIdx.x; i < nch * 512; i += kScrThreads) wd[i] = gw[i];
-        for (int i = threadIdx.x; i < nch * 4; i += kScrThreads)
+        for (int i = threadIdx.x; i < nch * 512; i += NTH) wd[i] = gw[i];
+        for (int i = threadIdx.x; i < nch * 4; i += NTH)
           reinterpret_cast<v4u*>(scr_lds)[i] = gm[i];
         __syncthreads();
         staged = D.wts;
@@ -716,7 +722,7 @@ __global__ __launch_bounds__(kScrThreads) void screen_i8(const ScreenDesc* __res
         if (CR) cpart[r] = (uint32_t)((tb == D.t) ? v[0] : ((tb + 1 == D.t) ? v[2] : 0));
       }
       const int c = CR ? (int)group_scatter<R>(cpart) : (int)group_scatter<R>(cnt);
-      const int64_t kbase = ((w - D.blk0) * kScrWaves + wave) * (16 * R);
+      const int64_t kbase = ((w - D.blk0) * W + wave) * (16 * R);
       const int64_t kidx = kbase + 16 * (g % R) + col;
       const bool valid = (g < R) && kidx < D.m;
       // Decision in FP32: every FP32 value below is within 2e-6 relative of
@@ -1184,7 +1190,10 @@ hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t
 
 // Pre-filter screen over a shard table; nch_max = the largest model's 512-sample
 // chunks (sizes the LDS weight table).  grid = persistent workgroups.
-int64_t screen_block_kmers(int t) { return (int64_t)kScrWaves * 16 * scr_tiles(scr_rows(t)); }
+int64_t screen_block_kmers(int t) {
+  const int Q = scr_rows(t);
+  return (int64_t)scr_waves(Q) * 16 * scr_tiles(Q);
+}
 int screen_max_chunks() { return kScrMaxChunks; }
 int screen_rows(int t) { return scr_rows(t); }
 bool screen_count_row(int t) { return scr_count_row(t); }
@@ -1192,7 +1201,7 @@ bool screen_count_row(int t) { return scr_count_row(t); }
 template <int Q, bool CR>
 static hipError_t launch_screen_q(dim3 grid, size_t shm, hipStream_t s, const ScreenDesc* sd,
                                   int nshards, int64_t total_blocks) {
-  hipLaunchKernelGGL((screen_i8<Q, CR>), grid, dim3(kScrThreads), shm, s, sd, nshards,
+  hipLaunchKernelGGL((screen_i8<Q, CR>), grid, dim3(64 * scr_waves(Q)), shm, s, sd, nshards,
                      total_blocks);
   return hipGetLastError();
 }
@@ -1216,13 +1225,13 @@ int screen_wgs_per_cu(int t, int nch) {
   const size_t shm = scr_lds_bytes(nch);
   hipError_t e;
   if (!scr_count_row(t))
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, false>, kScrThreads, shm);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, false>, 512, shm);
   else if (scr_rows(t) == 4)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<4, true>, kScrThreads, shm);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<4, true>, 1024, shm);
   else if (scr_rows(t) == 8)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<8, true>, kScrThreads, shm);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<8, true>, 512, shm);
   else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, true>, kScrThreads, shm);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, true>, 512, shm);
   if (e != hipSuccess || nb < 1) nb = 1;
   return nb;
 }
