@@ -96,7 +96,7 @@ def main():
     # ---- mixed precision: warmup + timed steps ----
     for _ in range(args.warmup):
         mixed.scan_device(dev.ptr, args.m, rw, prm)
-    bulk_ms, ref_ms, cands, scr_ms, surv = [], [], [], [], []
+    bulk_ms, ref_ms, cands, scr_ms, surv, tot_ms = [], [], [], [], [], []
     t2 = time.perf_counter()
     for _ in range(args.steps):
         hits, npass = mixed.scan_device(dev.ptr, args.m, rw, prm)
@@ -106,12 +106,14 @@ def main():
         cands.append(tm["candidates"])
         scr_ms.append(tm["screen_ms"])
         surv.append(tm["survivors"])
+        tot_ms.append(tm["total_ms"])
     el = time.perf_counter() - t2
     ms_step = 1e3 * el / args.steps
     value = args.m * args.steps / el
     log(f"[bench] mixed: {ms_step:.2f} ms/step, {value:.3e} k-mers/s, bulk {np.mean(bulk_ms):.3f} ms,"
         f" refine {np.mean(ref_ms):.3f} ms, candidates {int(np.mean(cands))}, hits {len(hits)},"
-        f" passed {int(npass[0])}, screen {np.mean(scr_ms):.3f} ms, survivors {int(np.mean(surv))}")
+        f" passed {int(npass[0])}, screen {np.mean(scr_ms):.3f} ms, survivors {int(np.mean(surv))},"
+        f" device total {np.mean(tot_ms):.3f} ms")
 
     if args.profile_only:
         return

@@ -1026,29 +1026,33 @@ __global__ __launch_bounds__(kThreads, 2) void scan_f32(
 }
 
 // ----------------------------------------------------------------------------
-// Exact FP64 statistic of each candidate.  One wave per candidate; lane b
-// accumulates base column b (and b + 64) over the set, phenotyped bits in
-// accession order.  The candidate count is read on the device (so the launch
-// can sit in a captured graph).  Columns of RefDesc::B64: [0,p) rotated
-// centred U, [p, p+t) centred residuals.  res[i] = {c, xxM, xyM, S}.
+// Exact FP64 statistic of each candidate.  One workgroup per candidate: wave
+// v sums the columns over the set, phenotyped bits of the v-th quarter of the
+// row's words in accession order (lane b owns base column b and b + 64), and
+// wave 0 adds the quarters in order, so the result is deterministic.  The
+// candidate count is read on the device (so the launch can sit in a captured
+// graph).  Columns of RefDesc::B64: [0,p) rotated centred U, [p, p+t) centred
+// residuals.  res[i] = {c, xxM, xyM, S}.
 __global__ __launch_bounds__(kThreads) void refine_f64(const Cand* __restrict__ cand,
                                                        const unsigned long long* __restrict__ count,
                                                        int64_t cap,
                                                        const RefDesc* __restrict__ rd,
                                                        double* __restrict__ res) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
-  const int64_t nw = (int64_t)gridDim.x * kWaves;
+  __shared__ double pa[kWaves][2][64];
+  __shared__ int pc[kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int64_t ncand = (int64_t)count[0];
   if (ncand > cap) ncand = cap;
-  for (int64_t i = wid; i < ncand; i += nw) {
+  for (int64_t i = blockIdx.x; i < ncand; i += gridDim.x) {
     const Cand cd = cand[i];
     const RefDesc& R = rd[cd.shard];
     const uint64_t* row = R.bits + cd.idx * R.rw;
     const int nbf = R.p + R.t, p = R.p;
+    const int per = (R.nwords + kWaves - 1) / kWaves;
+    const int wb = min(R.nwords, wave * per), we = min(R.nwords, wb + per);
     double a0 = 0.0, a1 = 0.0;
     int c = 0;
-    for (int w = 0; w < R.nwords; ++w) {
+    for (int w = wb; w < we; ++w) {
       uint64_t word = row[w] & R.maskw[w];
       c += __popcll(word);
       // set bits in batches of 8: the batch's column loads are all issued
@@ -1057,57 +1061,72 @@ __global__ __launch_bounds__(kThreads) void refine_f64(const Cand* __restrict__ 
         int bit[8];
         int nb = 0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          bit[i] = 0;
+        for (int j = 0; j < 8; ++j) {
+          bit[j] = 0;
           if (word) {
-            bit[i] = __ffsll((long long)word) - 1;
+            bit[j] = __ffsll((long long)word) - 1;
             word &= word - 1;
-            nb = i + 1;
+            nb = j + 1;
           }
         }
         double v0[8], v1[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const double* Br = R.B64 + (size_t)(64 * w + bit[i]) * nbf;
-          v0[i] = (i < nb && lane < nbf) ? Br[lane] : 0.0;
-          v1[i] = (i < nb && lane + 64 < nbf) ? Br[lane + 64] : 0.0;
+        for (int j = 0; j < 8; ++j) {
+          const double* Br = R.B64 + (size_t)(64 * w + bit[j]) * nbf;
+          v0[j] = (j < nb && lane < nbf) ? Br[lane] : 0.0;
+          v1[j] = (j < nb && lane + 64 < nbf) ? Br[lane + 64] : 0.0;
         }
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          if (i < nb) {
-            a0 += v0[i];
-            a1 += v1[i];
+        for (int j = 0; j < 8; ++j) {
+          if (j < nb) {
+            a0 += v0[j];
+            a1 += v1[j];
           }
         }
       }
     }
-    const int tr = cd.trait;
-    double zz = 0.0, zb = 0.0, z0 = 0.0, sv = 0.0;
-    if (lane < p) {
-      zz += a0 * a0;
-      zb += a0 * R.bpfull[tr * p + lane];
-      if (lane == 0) z0 = a0;
+    pa[wave][0][lane] = a0;
+    pa[wave][1][lane] = a1;
+    if (lane == 0) pc[wave] = c;
+    __syncthreads();
+    if (wave == 0) {
+      a0 = pa[0][0][lane];
+      a1 = pa[0][1][lane];
+      c = pc[0];
+      for (int v = 1; v < kWaves; ++v) {
+        a0 += pa[v][0][lane];
+        a1 += pa[v][1][lane];
+        c += pc[v];
+      }
+      const int tr = cd.trait;
+      double zz = 0.0, zb = 0.0, z0 = 0.0, sv = 0.0;
+      if (lane < p) {
+        zz += a0 * a0;
+        zb += a0 * R.bpfull[tr * p + lane];
+        if (lane == 0) z0 = a0;
+      }
+      if (lane + 64 < p) {
+        zz += a1 * a1;
+        zb += a1 * R.bpfull[tr * p + lane + 64];
+      }
+      if (lane == p + tr) sv = a0;
+      if (lane + 64 == p + tr) sv = a1;
+      for (int off = 32; off > 0; off >>= 1) {
+        zz += __shfl_xor(zz, off);
+        zb += __shfl_xor(zb, off);
+        z0 += __shfl_xor(z0, off);
+        sv += __shfl_xor(sv, off);
+      }
+      if (lane == 0) {
+        const double cdv = (double)c, cn = cdv / (double)R.n;
+        const double xxM = cdv - cdv * cdv * R.base_coef - 2.0 * cn * R.sigma * z0 - zz;
+        res[4 * i + 0] = cdv;
+        res[4 * i + 1] = xxM;
+        res[4 * i + 2] = sv + cdv * R.rbar[tr];
+        res[4 * i + 3] = sv + zb;
+      }
     }
-    if (lane + 64 < p) {
-      zz += a1 * a1;
-      zb += a1 * R.bpfull[tr * p + lane + 64];
-    }
-    if (lane == p + tr) sv = a0;
-    if (lane + 64 == p + tr) sv = a1;
-    for (int off = 32; off > 0; off >>= 1) {
-      zz += __shfl_xor(zz, off);
-      zb += __shfl_xor(zb, off);
-      z0 += __shfl_xor(z0, off);
-      sv += __shfl_xor(sv, off);
-    }
-    if (lane == 0) {
-      const double cdv = (double)c, cn = cdv / (double)R.n;
-      const double xxM = cdv - cdv * cdv * R.base_coef - 2.0 * cn * R.sigma * z0 - zz;
-      res[4 * i + 0] = cdv;
-      res[4 * i + 1] = xxM;
-      res[4 * i + 2] = sv + cdv * R.rbar[tr];
-      res[4 * i + 3] = sv + zb;
-    }
+    __syncthreads();  // the partials are rewritten for the next candidate
   }
 }
 
