@@ -187,6 +187,7 @@ struct kg_ctx {
   DescSlot dslot[4];
   uint64_t dclock = 0;
   HBuf hcnt;  // page-locked counter read-back
+  HBuf hcand, hres;  // page-locked read-back of the first kEagerCands candidates
   hipEvent_t cev[2] = {nullptr, nullptr};
   hipEvent_t ctev[4] = {nullptr, nullptr, nullptr, nullptr};
   HBuf bounce[2];
@@ -876,6 +877,7 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
 }
 
 enum { kPerLaunch = 1, kGraph = 2 };
+constexpr int64_t kEagerCands = 4096;  // candidates read back with the counters
 
 // Scans every shard (int8 path), appending hits (idx offset by shard idx0,
 // shard index set) and adding pre-filter passes to n_passed[s * t + tr].
@@ -1205,6 +1207,15 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
                           d_sd1, d_rd, sr);
       if (rc != KG_OK) return rc;
     }
+    // the first candidates come back with the counters (one synchronisation
+    // for the usual scan; more candidates are copied after it)
+    const int64_t eager = std::min<int64_t>(ctx->cand_cap, kEagerCands);
+    KG_CK(ctx, ctx->hcand.ensure(sizeof(Cand) * eager));
+    KG_CK(ctx, ctx->hres.ensure(sizeof(double) * 4 * eager));
+    KG_CK(ctx, hipMemcpyAsync(ctx->hcand.p, o.cand, sizeof(Cand) * eager, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    KG_CK(ctx, hipMemcpyAsync(ctx->hres.p, ctx->res.p, sizeof(double) * 4 * eager,
+                              hipMemcpyDeviceToHost, ctx->stream));
     KG_CK(ctx, hipMemcpyAsync(ctx->hcnt.p, dc, sizeof(unsigned long long) * cnt.size(),
                               hipMemcpyDeviceToHost, ctx->stream));
     if (overlap && attempt == 0) {
@@ -1265,8 +1276,15 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   if (nc == 0) return KG_OK;
   std::vector<Cand> cands(nc);
   std::vector<double> res((size_t)4 * nc);
-  KG_CK(ctx, hipMemcpy(cands.data(), ctx->cand.p, sizeof(Cand) * nc, hipMemcpyDeviceToHost));
-  KG_CK(ctx, hipMemcpy(res.data(), ctx->res.p, sizeof(double) * 4 * nc, hipMemcpyDeviceToHost));
+  const int64_t ne = std::min<int64_t>(nc, std::min<int64_t>(ctx->cand_cap, kEagerCands));
+  memcpy(cands.data(), ctx->hcand.p, sizeof(Cand) * ne);
+  memcpy(res.data(), ctx->hres.p, sizeof(double) * 4 * ne);
+  if (nc > ne) {
+    KG_CK(ctx, hipMemcpy(cands.data() + ne, ctx->cand.as<Cand>() + ne, sizeof(Cand) * (nc - ne),
+                         hipMemcpyDeviceToHost));
+    KG_CK(ctx, hipMemcpy(res.data() + 4 * ne, ctx->res.as<double>() + 4 * ne,
+                         sizeof(double) * 4 * (nc - ne), hipMemcpyDeviceToHost));
+  }
   for (int64_t i = 0; i < nc; ++i) {
     const Cand& cd = cands[i];
     const HShard& s = sh[cd.shard];

@@ -844,21 +844,30 @@ __global__ __launch_bounds__(1024) void scr_scan(const uint32_t* __restrict__ cc
   if (tid == 1023) *total = part[1023];
 }
 
-// Wave v of chunk c owns a quarter of its words; a wave pass covers two words
-// (lane l: bit l % 32 of word w + l / 32), so the ballot order is row order
-// and each pass stores its rows contiguously.
+// Wave v of chunk c owns a quarter of its words (at most 512: 8 per lane,
+// loaded at once); a wave pass covers two words (lane l: bit l % 32 of word
+// 2 j + l / 32 of the quarter), so the ballot order is row order and each
+// pass stores its rows contiguously.
 __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch,
                                                  const ScreenDesc* __restrict__ ss,
                                                  const int64_t* __restrict__ P) {
+  static_assert(kScrChunkWords == 4 * 512, "a wave's quarter is 8 words per lane");
   __shared__ uint32_t wcnt[4];
   const ScrChunk C = ch[blockIdx.x];
   const ScreenDesc& D = ss[C.shard];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t per = (C.w1 - C.w0 + 3) / 4;
   const int64_t a0 = min(C.w1, C.w0 + wave * per), a1 = min(C.w1, a0 + per);
+  const int nw = (int)(a1 - a0);
   const uint32_t* __restrict__ bm = D.bmap;
+  uint32_t wv[8];
   uint32_t v = 0;
-  for (int64_t w = a0 + lane; w < a1; w += 64) v += __popc(bm[w]);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int j = 64 * i + lane;
+    wv[i] = (j < nw) ? bm[a0 + j] : 0u;
+    v += __popc(wv[i]);
+  }
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   if (lane == 0) wcnt[wave] = v;
   __syncthreads();
@@ -866,13 +875,20 @@ __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch
   for (int i = 0; i < wave; ++i) o += wcnt[i];
   uint32_t* __restrict__ list = D.list;
   const unsigned long long below = (1ull << lane) - 1ull;
-  for (int64_t w = a0; w < a1; w += 2) {
-    const int64_t wl = w + (lane >> 5);
-    const uint32_t word = (wl < a1) ? bm[wl] : 0u;
-    const bool set = ((word >> (lane & 31)) & 1u) != 0;
-    const unsigned long long bal = __ballot(set);
-    if (set) list[o + __popcll(bal & below)] = (uint32_t)(32 * wl + (lane & 31));
-    o += __popcll(bal);
+  const int h = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    if (64 * i >= nw) break;
+    for (int j = 0; j < 32; ++j) {
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)wv[i], 2 * j);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)wv[i], 2 * j + 1);
+      const uint32_t word = h ? hi : lo;
+      const bool set = ((word >> (lane & 31)) & 1u) != 0;
+      const unsigned long long bal = __ballot(set);
+      if (set)
+        list[o + __popcll(bal & below)] = (uint32_t)(32 * (a0 + 64 * i + 2 * j + h) + (lane & 31));
+      o += __popcll(bal);
+    }
   }
 }
 
