@@ -190,7 +190,7 @@ __device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wa
   for (int r = 0; r < R; ++r) {
     int64_t k = kbase + 16 * r + col;
     if (k > D.m - 1) k = D.m - 1;  // clamp: rows past m are read, never reported
-    const int64_t row = D.rows ? (int64_t)D.rows[k] : k;
+    const int64_t row = D.rows ? (int64_t)gptr(D.rows)[k] : k;
     rowp[r] = D.bits + row * D.rw + 2 * g;
   }
 }
@@ -276,13 +276,12 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
   if (w >= total_blocks) return;
   int s = 0;
   while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
-  {
-    const uint64_t* rp[R];
-    block_rows<R>(sd[s], w, wave, g, col, rp);
-    dma_bits<R>(rp, 0, mybits, lane);
-    dma_mask(sd[s].mask, 0, g, mybits + R * 64);
-    dma_stage<NT>(reinterpret_cast<const v4i*>(sd[s].wts), buf0, wave, lane);
-  }
+  const uint64_t* rowp[R];  // this lane's rows of the current block
+  block_rows<R>(sd[s], w, wave, g, col, rowp);
+  dma_bits<R>(rowp, 0, mybits, lane);
+  dma_mask(sd[s].mask, 0, g, mybits + R * 64);
+  dma_stage<NT>(reinterpret_cast<const v4i*>(sd[s].wts), buf0, wave, lane);
+  const double* staged_cr = nullptr;  // model whose epilogue constants are in LDS
 
   while (w < total_blocks) {
     const ShardDesc& D = sd[s];
@@ -290,8 +289,13 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     const v4i* __restrict__ wts = reinterpret_cast<const v4i*>(D.wts);
     const int nchunks = D.nchunks;
     const int64_t kbase = ((w - D.blk0) * kI8Waves + wave) * (16 * R);
-    const uint64_t* rowp[R];
-    block_rows<R>(D, w, wave, g, col, rowp);
+    // the next block's shard and rows, located now: with survivor lists the
+    // row pointers come from a load whose latency the chunk loop hides
+    const int64_t wn = w + gridDim.x;
+    int sn = s;
+    while (sn + 1 < nshards && sd[sn + 1].blk0 <= wn) ++sn;
+    const uint64_t* nrowp[R];
+    if (wn < total_blocks) block_rows<R>(sd[sn], wn, wave, g, col, nrowp);
 
     v4i acc[R][NT];
 #pragma unroll
@@ -367,24 +371,22 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
       }
     }
 
-    // this shard's epilogue constants into LDS while no direct-to-LDS load is
-    // in flight (the epilogue then reads only LDS)
-    {
+    // this model's epilogue constants into LDS (when the model changes) while
+    // no direct-to-LDS load is in flight (the epilogue then reads only LDS);
+    // workgroup-uniform, and the previous epilogue's reads ended at its
+    // closing barrier
+    if (ep.colrec != staged_cr) {
       const double* __restrict__ gcr = ep.colrec;
       for (int i = threadIdx.x; i < ep.ncol * kColRec; i += kI8Threads) colrec_lds[i] = gcr[i];
       for (int i = threadIdx.x; i < ep.t * kTrcRec; i += kI8Threads) trc_lds[i] = ep.trc[i];
       __builtin_amdgcn_s_waitcnt(0);
+      staged_cr = ep.colrec;
     }
 
-    // next block: locate its shard, prefetch its first stage (buf0 is free:
-    // the last stage used buf1) and its first bits chunk
-    const int64_t wn = w + gridDim.x;
-    int sn = s;
-    while (sn + 1 < nshards && sd[sn + 1].blk0 <= wn) ++sn;
+    // next block: prefetch its first stage (buf0 is free: the last stage used
+    // buf1) and its first bits chunk
     if (wn < total_blocks) {
-      const uint64_t* rp[R];
-      block_rows<R>(sd[sn], wn, wave, g, col, rp);
-      dma_bits<R>(rp, 0, mybits, lane);
+      dma_bits<R>(nrowp, 0, mybits, lane);
       dma_mask(sd[sn].mask, 0, g, mybits + R * 64);
       dma_stage<NT>(reinterpret_cast<const v4i*>(sd[sn].wts), buf0, wave, lane);
     }
@@ -462,7 +464,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 
     const int64_t kidx = kbase + 16 * rme + col;
     const bool valid = primary && (kidx < D.m);
-    const int64_t krow = (D.rows != nullptr && valid) ? (int64_t)D.rows[kidx] : kidx;
+    const int64_t krow = (D.rows != nullptr && valid) ? (int64_t)gptr(D.rows)[kidx] : kidx;
     const double cd = (double)c, cn = cd * ep.inv_n;
     // |dz_k| <= c h_k  =>  |d |z|^2| <= 2c sum|z_k| h_k + c^2 sum h_k^2
     //                              <= 2c |z| |h| + c^2 |h|^2
@@ -523,6 +525,10 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     }
     w = wn;
     s = sn;
+    if (w < total_blocks) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) rowp[r] = nrowp[r];
+    }
   }
 }
 

@@ -422,3 +422,41 @@ def test_screen_shards_with_empty_survivor_lists(built):
             assert n0[s, 0] == ref["n_passed"], (corr, s)
             exp, _ = expected_hits(ref, 1.0, 3.0)
             assert np.array_equal(h0[h0["shard"] == s]["idx"], exp), (corr, s)
+
+
+def test_screen_at_the_threshold(built):
+    """Thresholds placed exactly on k-mers' own FP64 |cor| (the k-mers the
+    screen's margin exists for): screened, unscreened and auto scans agree
+    bit for bit, and both match the FP64 oracle outside the +-1e-6 band."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    kw, m, _, l10, l2d = CASES["n2048_p32"]
+    ds = dataset("n2048_p32", **kw)
+    bits = ds.fill_bits(0, m)
+    cor = np.abs(pearson(bits, ds.idxs, ds.y))
+    order = np.argsort(-np.nan_to_num(cor))
+    # taus at the |cor| of the 10th, 300th and 3000th strongest k-mers
+    taus = [float(cor[order[j]]) for j in (9, 299, 2999)]
+    with KgwasHip(screen="on") as kon, KgwasHip(screen="off") as koff, KgwasHip() as kauto:
+        for k in (kon, koff, kauto):
+            k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        dev = kon.to_device(bits)
+        for tau in taus:
+            prm = ScanParams(tau, l10, l2d)
+            h_on, n_on = kon.scan_device(dev.ptr, m, bits.shape[1], prm)
+            surv = kon.timing()["survivors"]
+            h_off, n_off = koff.scan(bits, prm)
+            h_au, n_au = kauto.scan(bits, prm)
+            ref = oracle.scan_f64(bits, ds.idxs, ds.y, ds.U, tau)
+            print(f"[screen tau] tau={tau:.9f} passed={int(n_on[0])} oracle={ref['n_passed']} "
+                  f"survivors={surv} hits={len(h_on)}", flush=True)
+            for h, n in ((h_off, n_off), (h_au, n_au)):
+                assert np.array_equal(n, n_on)
+                assert np.array_equal(h["idx"], h_on["idx"]) and np.array_equal(h["T"], h_on["T"])
+            with np.errstate(divide="ignore", invalid="ignore"):
+                near_c = np.abs(cor / tau - 1.0) <= BAND
+            assert abs(int(n_on[0]) - ref["n_passed"]) <= int(near_c.sum())
+            assert surv >= int(n_on[0])
+            exp_idx, _ = expected_hits(ref, l10, l2d)
+            sym = np.setxor1d(exp_idx, h_on["idx"])
+            assert np.all(near_c[sym]), sym[:10]
+        dev.free()
