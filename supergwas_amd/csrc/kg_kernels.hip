@@ -265,11 +265,14 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
   __shared__ v4i bitsbuf[kI8Waves * (R + 1) * 64];   // [wave][r | mask][lane] next chunk
   __shared__ double colrec_lds[4 * J * kColRec];
   __shared__ double trc_lds[kMaxTraits * kTrcRec];
-  __shared__ unsigned int wg_pass[kMaxTraits];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, col = lane & 15;
   v4i* const mybits = bitsbuf + wave * (R + 1) * 64;
-  if (threadIdx.x < kMaxTraits) wg_pass[threadIdx.x] = 0;
+  // this wave's decided pre-filter passes of the current shard (flushed when
+  // the shard changes, so no workgroup barrier closes a block)
+  uint32_t wpass[TT];
+#pragma unroll
+  for (int tr = 0; tr < TT; ++tr) wpass[tr] = 0;
 
   if (total_dev != nullptr) total_blocks = *total_dev;
   int64_t w = blockIdx.x;
@@ -365,17 +368,23 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
           }
           __builtin_amdgcn_sched_barrier(0);
         }
-        // next stage's weights (and next chunk's bits) landed in every wave's share
+        // next stage's weights (and next chunk's bits) landed in every wave's
+        // share.  After the block's last stage a wave goes straight to its
+        // epilogue when the stage count is even: the last stage read buf1,
+        // and buf0 (which the next block's first stage is loaded into) was
+        // released at the previous barrier, so the early waves' epilogues
+        // overlap the late waves' last MFMAs.
         __builtin_amdgcn_s_waitcnt(0);
-        lds_barrier();
+        if (S + 1 < nchunks * SPC || ((nchunks * SPC) & 1)) lds_barrier();
       }
     }
 
     // this model's epilogue constants into LDS (when the model changes) while
     // no direct-to-LDS load is in flight (the epilogue then reads only LDS);
-    // workgroup-uniform, and the previous epilogue's reads ended at its
-    // closing barrier
-    if (ep.colrec != staged_cr) {
+    // workgroup-uniform, and every wave finished the previous block's
+    // epilogue before this block's opening barrier
+    const bool restage = ep.colrec != staged_cr;
+    if (restage) {
       const double* __restrict__ gcr = ep.colrec;
       for (int i = threadIdx.x; i < ep.ncol * kColRec; i += kI8Threads) colrec_lds[i] = gcr[i];
       for (int i = threadIdx.x; i < ep.t * kTrcRec; i += kI8Threads) trc_lds[i] = ep.trc[i];
@@ -390,7 +399,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
       dma_mask(sd[sn].mask, 0, g, mybits + R * 64);
       dma_stage<NT>(reinterpret_cast<const v4i*>(sd[sn].wts), buf0, wave, lane);
     }
-    lds_barrier();  // column records staged by every wave
+    if (restage) lds_barrier();  // column records staged by every wave
 
     // ---- epilogue, from registers ----
     // Lane (g, col) holds, for k-mer col of tile r, output rows
@@ -503,8 +512,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
           out.st_xy[kidx * t + tr] = xyM;
           out.st_S[kidx * t + tr] = S;
         }
-        const unsigned long long pb = __ballot(pf_def);
-        if (lane == 0 && pb) atomicAdd(&wg_pass[tr], (unsigned int)__popcll(pb));
+        wpass[tr] += (uint32_t)__popcll(__ballot(pf_def));
         Cand rec;
         rec.idx = krow;
         rec.trait = tr;
@@ -516,12 +524,15 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         append_cand(fl != 0, rec, out);
       }
     }
-    __syncthreads();
-    // flush this block's pre-filter passes to its shard
-    if (threadIdx.x < t) {
-      const unsigned int v = wg_pass[threadIdx.x];
-      if (v) atomicAdd(&D.pass[threadIdx.x], (unsigned long long)v);
-      wg_pass[threadIdx.x] = 0;
+    // the shard's passes counted by this wave, when the next block is another
+    // shard's (or there is none)
+    if (sn != s || wn >= total_blocks) {
+#pragma unroll
+      for (int tr = 0; tr < TT; ++tr) {
+        if (lane == 0 && tr < t && wpass[tr] != 0)
+          atomicAdd(&D.pass[tr], (unsigned long long)wpass[tr]);
+        wpass[tr] = 0;
+      }
     }
     w = wn;
     s = sn;
