@@ -177,15 +177,10 @@ struct kg_ctx {
   // kg_scan streaming: copy stream, copy-done events and copy timing events
   // per staging buffer, page-locked bounce buffers for pageable rows
   hipStream_t cstream = nullptr;
-  // descriptor tables of recent non-graph int8 scans (a streamed scan
-  // alternates between two), so repeated shapes skip the upload
-  struct DescSlot {
-    std::vector<char> key;
-    DBuf sd, sd1, rd, ss, sd2, ch;
-    uint64_t used = 0;
-  };
-  DescSlot dslot[4];
-  uint64_t dclock = 0;
+  // descriptor tables of a non-graph int8 scan: page-locked staging and the
+  // device copy (one asynchronous upload per scan)
+  HBuf hdesc;
+  DBuf ddesc;
   HBuf hcnt;  // page-locked counter read-back
   HBuf hcand, hres;  // page-locked read-back of the first kEagerCands candidates
   hipEvent_t cev[2] = {nullptr, nullptr};
@@ -1134,52 +1129,35 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
         sr.d_ch = ctx->chdesc.as<ScrChunk>();
       }
     } else {
-      // descriptor tables: reuse an identical recent upload (every earlier scan
-      // has completed, so no kernel still reads a slot that gets replaced)
+      // descriptor tables: packed into one page-locked staging buffer and
+      // copied with one asynchronous copy on the scan's stream (the previous
+      // scan ended with a synchronisation, so both buffers are free)
+      auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
       const size_t b0 = sizeof(ShardDesc) * nsh, b2 = sizeof(RefDesc) * nsh;
       const size_t b3 = screen ? sizeof(ScreenDesc) * nsh : 0, b4 = screen ? b0 : 0;
       const size_t b5 = screen ? sizeof(ScrChunk) * chunks.size() : 0;
-      std::vector<char> dk(2 * b0 + b2 + b3 + b4 + b5);
-      memcpy(dk.data(), sd.data(), b0);
-      memcpy(dk.data() + b0, sd1.data(), b0);
-      memcpy(dk.data() + 2 * b0, rd.data(), b2);
+      const size_t o1 = al(b0), o2 = o1 + al(b0), o3 = o2 + al(b2), o4 = o3 + al(b3),
+                   o5 = o4 + al(b4), tot = o5 + al(std::max<size_t>(b5, 1));
+      KG_CK(ctx, ctx->hdesc.ensure(tot));
+      KG_CK(ctx, ctx->ddesc.ensure(tot));
+      char* hp = static_cast<char*>(ctx->hdesc.p);
+      memcpy(hp, sd.data(), b0);
+      memcpy(hp + o1, sd1.data(), b0);
+      memcpy(hp + o2, rd.data(), b2);
       if (screen) {
-        memcpy(dk.data() + 2 * b0 + b2, ss.data(), b3);
-        memcpy(dk.data() + 2 * b0 + b2 + b3, sd2.data(), b4);
-        if (b5) memcpy(dk.data() + 2 * b0 + b2 + b3 + b4, chunks.data(), b5);
+        memcpy(hp + o3, ss.data(), b3);
+        memcpy(hp + o4, sd2.data(), b4);
+        if (b5) memcpy(hp + o5, chunks.data(), b5);
       }
-      kg_ctx::DescSlot* slot = nullptr;
-      for (auto& e : ctx->dslot)
-        if (e.key == dk) slot = &e;
-      if (!slot) {
-        slot = &ctx->dslot[0];
-        for (auto& e : ctx->dslot)
-          if (e.used < slot->used) slot = &e;
-        slot->key.clear();
-        KG_CK(ctx, slot->sd.ensure(b0));
-        KG_CK(ctx, slot->sd1.ensure(b0));
-        KG_CK(ctx, slot->rd.ensure(b2));
-        KG_CK(ctx, hipMemcpy(slot->sd.p, sd.data(), b0, hipMemcpyHostToDevice));
-        KG_CK(ctx, hipMemcpy(slot->sd1.p, sd1.data(), b0, hipMemcpyHostToDevice));
-        KG_CK(ctx, hipMemcpy(slot->rd.p, rd.data(), b2, hipMemcpyHostToDevice));
-        if (screen) {
-          KG_CK(ctx, slot->ss.ensure(b3));
-          KG_CK(ctx, slot->sd2.ensure(b4));
-          KG_CK(ctx, hipMemcpy(slot->ss.p, ss.data(), b3, hipMemcpyHostToDevice));
-          KG_CK(ctx, hipMemcpy(slot->sd2.p, sd2.data(), b4, hipMemcpyHostToDevice));
-          KG_CK(ctx, slot->ch.ensure(std::max<size_t>(b5, 1)));
-          if (b5) KG_CK(ctx, hipMemcpy(slot->ch.p, chunks.data(), b5, hipMemcpyHostToDevice));
-        }
-        slot->key = std::move(dk);
-      }
-      slot->used = ++ctx->dclock;
-      d_sd = slot->sd.as<ShardDesc>();
-      d_sd1 = slot->sd1.as<ShardDesc>();
-      d_rd = slot->rd.as<RefDesc>();
+      KG_CK(ctx, hipMemcpyAsync(ctx->ddesc.p, hp, tot, hipMemcpyHostToDevice, ctx->stream));
+      char* dp = static_cast<char*>(ctx->ddesc.p);
+      d_sd = reinterpret_cast<const ShardDesc*>(dp);
+      d_sd1 = reinterpret_cast<const ShardDesc*>(dp + o1);
+      d_rd = reinterpret_cast<const RefDesc*>(dp + o2);
       if (screen) {
-        sr.d_ss = slot->ss.as<ScreenDesc>();
-        sr.d_sd2 = slot->sd2.as<ShardDesc>();
-        sr.d_ch = slot->ch.as<ScrChunk>();
+        sr.d_ss = reinterpret_cast<const ScreenDesc*>(dp + o3);
+        sr.d_sd2 = reinterpret_cast<ShardDesc*>(dp + o4);
+        sr.d_ch = reinterpret_cast<const ScrChunk*>(dp + o5);
       }
     }
     KG_CK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
