@@ -460,3 +460,46 @@ def test_screen_at_the_threshold(built):
             sym = np.setxor1d(exp_idx, h_on["idx"])
             assert np.all(near_c[sym]), sym[:10]
         dev.free()
+
+
+def test_screen_models_of_different_width(built):
+    """One shard table whose models have different accession counts (1100, 600
+    and 2600: 3, 2 and 6 chunks of 512), interleaved, so the screen re-stages a
+    differently sized LDS table between shards; screened, unscreened and the
+    oracle agree."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    specs = [(1100, 2100, 61), (600, 1700, 62), (2600, 2400, 63), (1100, 900, 64)]
+    prm = ScanParams(0.12, 2.0, 4.0)
+    data = []
+    for n_acc, m, seed in specs:
+        ds = synth.make_dataset(seed=seed, n_acc=n_acc, n_missing=n_acc // 20, npop=4, p=7,
+                                m=m, n_causal=8, effect_sd=0.5, n_struct=512)
+        data.append((ds, ds.fill_bits(0, m)))
+    res = {}
+    for mode in ("on", "off"):
+        with KgwasHip(screen=mode) as k:
+            shards, devs = [], []
+            for s, (ds, bits) in enumerate(data):
+                mid = 0
+                if s == 0:
+                    k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+                else:
+                    mid = k.add_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+                dev = k.to_device(bits)
+                devs.append(dev)
+                shards.append((dev.ptr, len(bits), bits.shape[1], mid))
+            h, n = k.scan_shards(shards, prm, 0)
+            res[mode] = (h.copy(), n.copy(), k.timing()["survivors"])
+            for d in devs:
+                d.free()
+    (h1, n1, sv1), (h0, n0, sv0) = res["on"], res["off"]
+    print(f"[screen widths] hits={len(h1)} passed={n1[:, 0].tolist()} survivors={sv1}", flush=True)
+    assert np.array_equal(n1, n0)
+    assert np.array_equal(h1["idx"], h0["idx"]) and np.array_equal(h1["shard"], h0["shard"])
+    assert np.array_equal(h1["T"], h0["T"])
+    assert sv1 < sum(len(b) for _, b in data)
+    for s, (ds, bits) in enumerate(data):
+        ref = oracle.scan_f64(bits, ds.idxs, ds.y, ds.U, 0.12)
+        exp, _ = expected_hits(ref, 2.0, 4.0)
+        assert n1[s, 0] == ref["n_passed"], s
+        assert np.array_equal(h1[h1["shard"] == s]["idx"], exp), s
