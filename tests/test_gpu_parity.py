@@ -503,3 +503,34 @@ def test_screen_models_of_different_width(built):
         exp, _ = expected_hits(ref, 2.0, 4.0)
         assert n1[s, 0] == ref["n_passed"], s
         assert np.array_equal(h1[h1["shard"] == s]["idx"], exp), s
+
+
+@pytest.mark.parametrize("n_acc,m,t", [(65, 1, 1), (65, 31, 1), (511, 33, 2), (513, 1000, 3),
+                                       (1024, 4097, 1), (3000, 1537, 8), (2048, 70000, 1)])
+def test_screen_odd_shapes(built, n_acc, m, t):
+    """Bitmap words, compaction chunks and survivor blocks at awkward sizes:
+    k-mer counts that are not multiples of 32 or of a screen block, accession
+    counts just off a 512-sample chunk; screened == unscreened, and == oracle."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    ds = synth.make_dataset(seed=70 + n_acc + t, n_acc=n_acc, n_missing=n_acc // 13, npop=3,
+                            p=4, m=m, n_causal=min(4, m), effect_sd=0.6, traits=t,
+                            trait_corr=0.2, n_struct=256)
+    bits = ds.fill_bits(0, m)
+    prm = ScanParams(0.1, 1.5, 3.0)
+    out = {}
+    for mode in ("on", "off"):
+        with KgwasHip(screen=mode) as k:
+            k.set_model(ds.n_acc, ds.idxs, ds.U, ds.Y)
+            h, n = k.scan(bits, prm)
+            out[mode] = (h, n, k.timing()["survivors"])
+    (h1, n1, sv), (h0, n0, _) = out["on"], out["off"]
+    print(f"[odd] n_acc={n_acc} m={m} t={t} hits={len(h1)} passed={list(n1)} survivors={sv}",
+          flush=True)
+    assert np.array_equal(n1, n0)
+    assert np.array_equal(h1["idx"], h0["idx"]) and np.array_equal(h1["trait"], h0["trait"])
+    assert np.array_equal(h1["T"], h0["T"])
+    for tr in range(t):
+        ref = oracle.scan_f64(bits, ds.idxs, ds.Y[:, tr], ds.U, 0.1)
+        exp, _ = expected_hits(ref, 1.5, 3.0)
+        assert n1[tr] == ref["n_passed"], tr
+        assert np.array_equal(h1[h1["trait"] == tr]["idx"], exp), tr
