@@ -239,11 +239,19 @@ class KgwasHip:
             self._check(rc)
             return out[:n_out.value], np_.reshape(len(shards), -1).copy()
 
+    def _hit_buffer(self, cap):
+        # reused across calls; callers get a copy of the filled part
+        buf = getattr(self, "_hits", None)
+        if buf is None or len(buf) < cap:
+            buf = np.empty(max(cap, 1), HIT_DTYPE)
+            self._hits = buf
+        return buf
+
     def _scan(self, fn, ptr, m, row_words, params, cap):
         params = params or ScanParams()
         np_ = np.zeros(max(self.t, 1), np.int64)
         while True:
-            out = np.zeros(max(cap, 1), HIT_DTYPE)
+            out = self._hit_buffer(cap)
             n_out = _i64(0)
             rc = fn(self._h, ptr, int(m), int(row_words), ctypes.byref(params), _p(out), cap,
                     ctypes.byref(n_out), _p(np_))
@@ -251,7 +259,7 @@ class KgwasHip:
                 cap = int(n_out.value)
                 continue
             self._check(rc)
-            return out[:n_out.value], np_.copy()
+            return out[:n_out.value].copy(), np_.copy()
 
     def scan(self, bits, params=None, cap=1 << 16):
         """Host rows (uint64 [m, row_words]) -> (hits, n_passed per trait)."""
