@@ -154,6 +154,19 @@ int kg_scan_shards(kg_ctx* ctx, const kg_shard* shards, int32_t nshards,
                    const kg_scan_params* prm, uint32_t flags, kg_hit* out, int64_t cap,
                    int64_t* n_out, int64_t* n_passed);
 
+/* All-pairs epistasis scan (replaces emit_pairs_self, src/kgwas_2d.cpp:258-401):
+ * for rows i < j of the N device rows at d_bits, with i in [i0, i1) (the
+ * reference's --task-id/--num-tasks row range, src/kgwas_2d.cpp:283-285), the
+ * k-mer x_i AND x_j (src/kgwas_2d.cpp:372) is tested against model 0 with the
+ * statistic and threshold of kg_scan (p <= 10^-log10_thresh), without the
+ * pre-filter; corr_threshold and log10_thresh_2d are ignored.  Each reported
+ * pair is a kg_hit with shard = i and idx = j, sorted by (i, j, trait)
+ * (src/kgwas_2d.cpp:415-418); c is the pair's allele count.  Needs
+ * KG_PREC_MIXED. */
+int kg_scan_pairs(kg_ctx* ctx, const uint64_t* d_bits, int64_t N, int64_t row_words,
+                  int64_t i0, int64_t i1, const kg_scan_params* prm, kg_hit* out, int64_t cap,
+                  int64_t* n_out);
+
 /* Full-statistic mode (validation and checksums): the bulk kernel's values for
  * every k-mer, written to host arrays: c[m], xxM[m], xyM[m*t], S[m*t]
  * (S = x'(y - mean y), the pre-filter numerator), k-mer-major. */

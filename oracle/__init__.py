@@ -46,6 +46,11 @@ def lib():
                                   _dp, _dp, ctypes.c_int, ctypes.c_double,
                                   _i32p, _u8p, _dp, _dp, _dp, _dp, _dp]
         L.or_scan_f64.restype = ctypes.c_int64
+        L.or_pairs_f64.argtypes = [_u64p, ctypes.c_int64, ctypes.c_int64, _i32p, ctypes.c_int,
+                                   _dp, _dp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                   ctypes.c_double, ctypes.c_int64, _i64p, _i32p, _dp, _dp, _dp,
+                                   _dp, _dp]
+        L.or_pairs_f64.restype = ctypes.c_int64
         L.or_unpack_rows.argtypes = [_u64p, ctypes.c_int64, ctypes.c_int64, _i32p, ctypes.c_int,
                                      _fp]
         L.or_unpack_rows.restype = None
@@ -104,6 +109,37 @@ def scan_f64(bits, idxs, y, U, corr_thr):
                               _ptr(ef, _dp))
     return dict(c=c, passed=ok.astype(bool), xxM=xx, xyM=xy, T=T, p=pv, effect=ef,
                 n_passed=int(npass))
+
+
+def pairs_f64(bits, idxs, y, U, log10_thresh, i0=0, i1=None):
+    """FP64 all-pairs scan (src/kgwas_2d.cpp:258-401) of rows i < j, i in [i0, i1).
+
+    Returns dict(i, j, c, xxM, xyM, effect, T, p) of the pairs with
+    p <= 10^-log10_thresh, sorted by (i, j).
+    """
+    bits = np.ascontiguousarray(bits, dtype=np.uint64)
+    N, rw = bits.shape
+    idxs = np.ascontiguousarray(idxs, dtype=np.int32)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    Uf = np.asfortranarray(U, dtype=np.float64)
+    n, p = Uf.shape
+    assert len(idxs) == n and len(y) == n
+    i1 = N if i1 is None else i1
+    cap = 1024
+    while True:
+        ij = np.empty((cap, 2), np.int64)
+        c = np.empty(cap, np.int32)
+        xx, xy, ef, T, pv = (np.empty(cap) for _ in range(5))
+        k = lib().or_pairs_f64(_ptr(bits, _u64p), N, rw, _ptr(idxs, _i32p), n, _ptr(y, _dp),
+                               _ptr(Uf, _dp), p, int(i0), int(i1), 10.0 ** (-log10_thresh), cap,
+                               _ptr(ij, _i64p), _ptr(c, _i32p), _ptr(xx, _dp), _ptr(xy, _dp),
+                               _ptr(ef, _dp), _ptr(T, _dp), _ptr(pv, _dp))
+        if k <= cap:
+            break
+        cap = int(k)
+    o = np.lexsort((ij[:k, 1], ij[:k, 0]))
+    return dict(i=ij[:k, 0][o], j=ij[:k, 1][o], c=c[:k][o], xxM=xx[:k][o], xyM=xy[:k][o],
+                effect=ef[:k][o], T=T[:k][o], p=pv[:k][o])
 
 
 def unpack_rows(bits, idxs):

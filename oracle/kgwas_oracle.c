@@ -16,6 +16,8 @@
 //   src/kgwas.cpp:472-474  Ux = U'x, xxM = x'x - |Ux|^2, xyM = x'y - Ux'Uy
 //   src/chisq.hpp:6-9      chi2_sf_k1 (identical to the erfc above)
 //
+//   src/kgwas_2d.cpp:219-251, 258-401  all-pairs scan (or_pairs_f64)
+//
 // Pinned by tests/test_oracle.py: golden vectors written by
 // tests/golden/make_golden.py from an independent least-squares fit
 // (T = n ln(RSS0/RSS1), effect = beta_x / yyM), scipy's chi2.sf, numpy's
@@ -154,6 +156,94 @@ int64_t or_scan_f64(const uint64_t* bits, int64_t m, int64_t row_words, const in
   free(yc);
   free(Uy);
   return passed;
+}
+
+// All-pairs epistasis scan in FP64 (src/kgwas_2d.cpp:258-401, emit_pairs_self
+// with the caches of build_stats2d_self, src/kgwas_2d.cpp:219-251): for rows
+// i < j, i in [i0, i1), x12 = x_i * x_j over the phenotyped accessions
+// (src/kgwas_2d.cpp:372); skip unless x_i, x_j and x12 are nonzero
+// (:364, :369, :374); t12 = U'x12, xxM = |x12|^2 - |t12|^2, xyM = x12'y -
+// t12'U'y (:373-382); skip if xxM or yyM <= 1e-12 (:380); compute_hit and keep
+// p <= p_thresh (:383-388).  Rows i are spread over OpenMP threads, so kept
+// pairs land in no particular order (the Python wrapper sorts them by (i, j),
+// the reference's output order, :415-418); at most `cap` are written and the
+// return value is how many pass (a caller can retry with a larger buffer).
+int64_t or_pairs_f64(const uint64_t* bits, int64_t N, int64_t row_words, const int32_t* idxs,
+                     int n, const double* y, const double* U, int p, int64_t i0, int64_t i1,
+                     double p_thresh, int64_t cap, int64_t* ij_out, int32_t* c_out,
+                     double* xxM_out, double* xyM_out, double* eff_out, double* T_out,
+                     double* p_out) {
+  const double eps = 1e-12;
+  double* Uy = (double*)malloc(sizeof(double) * (size_t)(p > 0 ? p : 1));
+  for (int k = 0; k < p; ++k) {
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s += U[(size_t)k * n + j] * y[j];
+    Uy[k] = s;
+  }
+  const double yyM = or_yyM(y, n, U, p);
+  // x_i over the phenotyped accessions, one byte per sample
+  uint8_t* X = (uint8_t*)malloc((size_t)(N > 0 ? N : 1) * (size_t)n);
+  for (int64_t r = 0; r < N; ++r)
+    for (int j = 0; j < n; ++j) {
+      const int a = idxs[j];
+      X[(size_t)r * n + j] = (uint8_t)((bits[(size_t)r * row_words + (a >> 6)] >> (a & 63)) & 1ULL);
+    }
+  int64_t found = 0;
+  const int64_t iend = i1 < N ? i1 : N;
+#pragma omp parallel
+  {
+  double* t12 = (double*)malloc(sizeof(double) * (size_t)(p > 0 ? p : 1));
+#pragma omp for schedule(dynamic, 1)
+  for (int64_t i = i0; i < iend; ++i) {
+    const uint8_t* xi = X + (size_t)i * n;
+    int ci = 0;
+    for (int j = 0; j < n; ++j) ci += xi[j];
+    if (ci == 0) continue;
+    for (int64_t jr = i + 1; jr < N; ++jr) {
+      const uint8_t* xj = X + (size_t)jr * n;
+      int cj = 0, c12 = 0;
+      double b3 = 0.0;
+      for (int k = 0; k < p; ++k) t12[k] = 0.0;
+      for (int j = 0; j < n; ++j) {
+        cj += xj[j];
+        if (xi[j] & xj[j]) {
+          ++c12;
+          b3 += y[j];
+          for (int k = 0; k < p; ++k) t12[k] += U[(size_t)k * n + j];
+        }
+      }
+      if (cj == 0 || c12 == 0) continue;
+      double uu = 0.0, uy = 0.0;
+      for (int k = 0; k < p; ++k) {
+        uu += t12[k] * t12[k];
+        uy += t12[k] * Uy[k];
+      }
+      const double xxM = (double)c12 - uu;
+      if (xxM <= eps || yyM <= eps) continue;
+      const double xyM = b3 - uy;
+      double e, T, pv;
+      or_compute_hit(xxM, xyM, yyM, n, &e, &T, &pv);
+      if (pv > p_thresh) continue;
+      int64_t pos;
+#pragma omp atomic capture
+      pos = found++;
+      if (pos < cap) {
+        ij_out[2 * pos] = i;
+        ij_out[2 * pos + 1] = jr;
+        c_out[pos] = c12;
+        xxM_out[pos] = xxM;
+        xyM_out[pos] = xyM;
+        eff_out[pos] = e;
+        T_out[pos] = T;
+        p_out[pos] = pv;
+      }
+    }
+  }
+  free(t12);
+  }
+  free(X);
+  free(Uy);
+  return found;
 }
 
 // Smallest double T with erfc(sqrt(T/2)) <= p_thresh (bisection on the

@@ -87,6 +87,8 @@ SIGNATURES = {
                      ctypes.c_int),
     "kg_scan_shards": ([_vp, _vp, _i32, ctypes.POINTER(ScanParams), ctypes.c_uint32, _vp, _i64,
                         ctypes.POINTER(_i64), _vp], ctypes.c_int),
+    "kg_scan_pairs": ([_vp, _vp, _i64, _i64, _i64, _i64, ctypes.POINTER(ScanParams), _vp, _i64,
+                       ctypes.POINTER(_i64)], ctypes.c_int),
     "kg_stats_device": ([_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp], ctypes.c_int),
     "kg_last_timing": ([_vp, ctypes.POINTER(_Timing)], ctypes.c_int),
     "kg_set_screen": ([_vp, _i32], ctypes.c_int),
@@ -269,6 +271,23 @@ class KgwasHip:
     def scan_device(self, dev_ptr, m, row_words, params=None, cap=1 << 16):
         """Rows already in device memory at dev_ptr (e.g. a torch tensor's data_ptr())."""
         return self._scan(lib().kg_scan_device, dev_ptr, m, row_words, params, cap)
+
+    def scan_pairs(self, dev_ptr, n_rows, row_words, params=None, i0=0, i1=None, cap=1 << 16):
+        """All-pairs scan of device rows (src/kgwas_2d.cpp:258-401): hits with
+        shard = i and idx = j (i < j, i in [i0, i1)), sorted by (i, j, trait)."""
+        params = params or ScanParams()
+        i1 = n_rows if i1 is None else i1
+        while True:
+            out = self._hit_buffer(cap)
+            n_out = _i64(0)
+            rc = lib().kg_scan_pairs(self._h, dev_ptr, int(n_rows), int(row_words), int(i0),
+                                     int(i1), ctypes.byref(params), _p(out), cap,
+                                     ctypes.byref(n_out))
+            if rc == KG_ERR_CAPACITY:
+                cap = int(n_out.value)
+                continue
+            self._check(rc)
+            return out[:n_out.value].copy()
 
     def stats_device(self, dev_ptr, m, row_words):
         """Bulk-kernel values for every k-mer: c, xxM, xyM[m, t], S[m, t]."""

@@ -48,6 +48,9 @@ hipError_t launch_scan_f32(int NT, int nch, int grid, hipStream_t s, const uint6
 hipError_t launch_refine(hipStream_t s, const Cand* cand, const unsigned long long* count,
                          int64_t cap, const RefDesc* rd, double* res, int blocks);
 hipError_t launch_zero(hipStream_t s, unsigned long long* p, int n);
+hipError_t launch_pair_masks(hipStream_t s, const uint64_t* bits, int64_t rw,
+                            const uint64_t* mask, int64_t W, int64_t i0, int64_t cnt,
+                            uint64_t* out);
 hipError_t launch_synth(hipStream_t s, uint64_t* out, uint64_t seed, int64_t kmer0, int64_t m,
                         uint32_t n_acc, int64_t rw, const uint8_t* pop, const uint32_t* thr,
                         uint32_t npop, uint32_t nprof);
@@ -202,6 +205,7 @@ struct kg_ctx {
   int screen_skip = 0;
   DBuf surv, ssdesc, sdesc2, chdesc;  // survivor rows; graph-path screen tables
   DBuf bmap, scr_aux;                 // screen bitmaps; chunk counts and prefixes
+  DBuf pmask;                         // all-pairs scan: mask AND row i, per shard
   hipEvent_t evs = nullptr;   // end of the screen kernel
   int scr_occ_key = -1, scr_occ = 1;  // cached screen workgroups per CU
   // captured graph of the last batched scan
@@ -768,9 +772,12 @@ struct Decided {
 };
 
 static Decided decide(const Model& md, int64_t idx, int tr, int c, double xxM, double xyM,
-                      double S, bool have_S, const kg_scan_params& prm) {
+                      double S, bool have_S, const kg_scan_params& prm, bool pairs = false) {
   Decided d{};
   d.pass = true;
+  // all-pairs scan: no pre-filter, and a pair with xxM or yyM <= 1e-12 is
+  // skipped before its p-value (src/kgwas_2d.cpp:380), even at p_thresh = 1
+  if (pairs && !(xxM > 1e-12 && md.yyM[tr] > 1e-12)) return d;
   if (have_S) {  // FP64 pre-filter, src/kgwas.cpp:462-468
     const int n = md.n;
     const double v = (double)c * (double)(n - c) / (double)n;
@@ -805,6 +812,7 @@ struct HShard {
   const uint64_t* bits;
   int64_t m, rw, idx0;
   int model;
+  const uint64_t* mask = nullptr;  // device accession mask words (null: the model's)
 };
 
 static int check_shard(kg_ctx* ctx, const HShard& s) {
@@ -871,7 +879,7 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
   return KG_OK;
 }
 
-enum { kPerLaunch = 1, kGraph = 2 };
+enum { kPerLaunch = 1, kGraph = 2, kPairs = 4 };
 constexpr int64_t kEagerCands = 4096;  // candidates read back with the counters
 
 // Scans every shard (int8 path), appending hits (idx offset by shard idx0,
@@ -900,9 +908,10 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   std::vector<int64_t> nblk(nsh);
   int64_t total_blocks = 0;
   const bool per_launch = (flags & kPerLaunch) != 0;
+  const bool pairs = (flags & kPairs) != 0;
   // pre-filter screen: only where it can drop k-mers (tau > 0) and every
   // shard's model has screen weights; row lists are 32-bit
-  bool screen = !stats && !per_launch && prm.corr_threshold > 0.0 &&
+  bool screen = !stats && !per_launch && !pairs && prm.corr_threshold > 0.0 &&
                 ctx->screen_mode != KG_SCREEN_OFF;
   int nch_max = 0;
   int64_t mtot = 0;
@@ -931,7 +940,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     nblk[s] = (sh[s].m + bk - 1) / bk;
     total_blocks += nblk[s];
     D.wts = M.wts.p;
-    D.mask = M.mask.p;
+    D.mask = sh[s].mask ? static_cast<const void*>(sh[s].mask) : M.mask.p;
     D.nchunks = md.nch_i8;
     D.pad = 0;
     D.pass = dc + 1 + (size_t)s * t;
@@ -941,6 +950,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     D.ep.tau = prm.corr_threshold;
     D.ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;
     D.ep.all_sig = p_thr >= 1.0 ? 1 : 0;
+    D.ep.pairs = pairs ? 1 : 0;
     const double Tmax = (double)md.n * std::log(1.0 / (1.0 - (1 - 1e-15)));
     D.ep.r2star = (Ts > Tmax) ? INFINITY : -std::expm1(-Ts / (double)md.n);
     sd1[s] = D;
@@ -948,7 +958,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     RefDesc& R = rd[s];
     R.bits = sh[s].bits;
     R.rw = sh[s].rw;
-    R.maskw = M.mask.as<uint64_t>();
+    R.maskw = sh[s].mask ? sh[s].mask : M.mask.as<uint64_t>();
     R.B64 = M.B64.as<double>();
     R.bpfull = M.bpfull.as<double>();
     R.rbar = M.rbar.as<double>();
@@ -1269,7 +1279,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     const Model& md = ctx->models[s.model]->md;
     const int c = (int)res[4 * i + 0];
     Decided d = decide(md, s.idx0 + cd.idx, cd.trait, c, res[4 * i + 1], res[4 * i + 2],
-                       res[4 * i + 3], true, prm);
+                       res[4 * i + 3], !pairs, prm, pairs);
     if ((cd.flags & kPfAmb) && d.pass) n_passed[cd.shard * t + cd.trait] += 1;
     if (d.hit) {
       d.h.shard = cd.shard;
@@ -1569,6 +1579,76 @@ extern "C" int kg_scan_shards(kg_ctx* ctx, const kg_shard* shards, int32_t nshar
   rc = scan_dispatch(ctx, sh, *prm, f, hits, np.data(), false);
   if (rc) return rc;
   if (n_passed) memcpy(n_passed, np.data(), sizeof(int64_t) * np.size());
+  return finish_hits(ctx, hits, out, cap, n_out);
+}
+
+// All-pairs epistasis scan (src/kgwas_2d.cpp:258-401, emit_pairs_self): every
+// row pair i < j with i in [i0, i1) is tested as the k-mer x_i AND x_j, with
+// no pre-filter (src/kgwas_2d.cpp:362-388).  Shard k of a batch is row
+// i = bi + k: rows i+1 .. N-1 under the accession mask (model mask AND row
+// i), written on the device, so the bulk kernel, the FP64 refinement and the
+// host decision run unchanged.  Batches of consecutive rows bound the shard
+// tables and the mask buffer.
+constexpr int64_t kPairBatch = int64_t(1) << 28;  // pairs per batch (soft)
+constexpr int64_t kPairShards = 32768;            // rows i per batch
+
+extern "C" int kg_scan_pairs(kg_ctx* ctx, const uint64_t* d_bits, int64_t N, int64_t row_words,
+                             int64_t i0, int64_t i1, const kg_scan_params* prm, kg_hit* out,
+                             int64_t cap, int64_t* n_out) {
+  int rc = check_common(ctx, prm);
+  if (rc) return rc;
+  if (ctx->precision != KG_PREC_MIXED)
+    return fail(ctx, KG_ERR_ARG, "kg_scan_pairs needs KG_PREC_MIXED");
+  if (N < 0) return fail(ctx, KG_ERR_ARG, "negative row count");
+  if (i0 < 0 || i1 < i0 || i1 > N)
+    return fail(ctx, KG_ERR_ARG, "row range [i0, i1) must lie in [0, N]");
+  rc = check_shard(ctx, HShard{d_bits, N, row_words, 0, 0});
+  if (rc) return rc;
+  KG_CK(ctx, hipSetDevice(ctx->device));
+  const ModelDev& M = *ctx->models[0];
+  const int t = M.md.t;
+  const int64_t W = 8 * (int64_t)M.md.nch_i8;  // mask words per row i
+  kg_scan_params pp = *prm;
+  pp.corr_threshold = 0.0;
+  pp.log10_thresh_2d = 0.0;
+  std::vector<kg_hit> hits;
+  kg_timing acc{};
+  const int64_t last = std::min<int64_t>(i1, N - 1);  // row N-1 has no partner j > i
+  for (int64_t bi = i0; bi < last;) {
+    int64_t be = bi, np_ = 0;
+    while (be < last && be - bi < kPairShards &&
+           (np_ == 0 || np_ + (N - 1 - be) <= kPairBatch)) {
+      np_ += N - 1 - be;
+      ++be;
+    }
+    const int64_t cnt = be - bi;
+    KG_CK(ctx, ctx->pmask.ensure(sizeof(uint64_t) * (size_t)(cnt * W)));
+    uint64_t* pm = ctx->pmask.as<uint64_t>();
+    KG_CK(ctx, launch_pair_masks(ctx->stream, d_bits, row_words, M.mask.as<uint64_t>(), W, bi,
+                                 cnt, pm));
+    std::vector<HShard> sh((size_t)cnt);
+    for (int64_t k = 0; k < cnt; ++k) {
+      const int64_t i = bi + k;
+      sh[k] = HShard{d_bits + (i + 1) * row_words, N - 1 - i, row_words, i + 1, 0};
+      sh[k].mask = pm + k * W;
+    }
+    std::vector<int64_t> passes((size_t)cnt * t, 0);
+    std::vector<kg_hit> bh;
+    rc = scan_i8_shards(ctx, sh, pp, kPairs, bh, passes.data(), false);
+    if (rc) return rc;
+    for (kg_hit& h : bh) {
+      h.shard = (int32_t)(bi + h.shard);  // row i; h.idx is row j
+      hits.push_back(h);
+    }
+    acc.bulk_ms += ctx->timing.bulk_ms;
+    acc.refine_ms += ctx->timing.refine_ms;
+    acc.total_ms += ctx->timing.total_ms;
+    acc.candidates += ctx->timing.candidates;
+    acc.kmers += ctx->timing.kmers;
+    bi = be;
+  }
+  acc.survivors = acc.kmers;
+  ctx->timing = acc;
   return finish_hits(ctx, hits, out, cap, n_out);
 }
 

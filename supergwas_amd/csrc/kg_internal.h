@@ -51,6 +51,8 @@ struct EpiI8 {
   int tau_trivial;  // corr threshold <= 0: every polymorphic k-mer passes
   int all_sig;      // p threshold >= 1: every pre-filter pass is reported
   int has_d;        // z column 0 is the kept intercept-remainder column d
+  int pairs;        // all-pairs scan: no pre-filter, and c = n is tested
+  int pad_e;
   double sigma;     // (Q'U'1)_0: the rotated U'1 = sigma e_0
   double base_coef; // sigma^2 / n^2  (x'x - |cn m'|^2 = c - c^2 base_coef)
   double dmax_drop; // max|d| of the dropped intercept-remainder column (0 if kept)

@@ -97,8 +97,11 @@ __device__ __forceinline__ uint32_t classify_i8(int c, double xxM, double bxx, d
                                                 bool* pf_def) {
   *pf_def = false;
   const int n = ep.n;
-  if (c <= 0 || c >= n) return 0;               // |x - mean x| = 0 (src/kgwas.cpp:465)
-  if (!(ycn >= 1e-12)) return 0;                // |y - mean y| < eps
+  if (c <= 0) return 0;                         // x = 0 (src/kgwas_2d.cpp:364-374)
+  if (!ep.pairs) {
+    if (c >= n) return 0;                       // |x - mean x| = 0 (src/kgwas.cpp:465)
+    if (!(ycn >= 1e-12)) return 0;              // |y - mean y| < eps
+  }
   uint32_t f;
   if (ep.tau_trivial) {
     f = kPfPass;
@@ -145,7 +148,8 @@ __device__ __forceinline__ uint32_t classify_i8(int c, double xxM, double bxx, d
 // 512-sample chunk (8 MFMA steps) each, filled by direct global->LDS loads
 // for chunk C+1 while chunk C computes; each lane's bits and accession mask
 // for chunk C+1 are delivered the same way into its wave's LDS slot; the next
-// block's first chunk is prefetched before the epilogue runs.
+// block's first bits chunk loads during the block's last chunk, and its first
+// weight stage during the last stage (even stage counts) or the epilogue.
 // MFMA steps per LDS stage: a whole 512-sample chunk while two stages of
 // weights fit in LDS, a half or a quarter chunk for the widest tiles.
 __host__ __device__ constexpr int i8_stage_steps(int NT) { return NT <= 8 ? 8 : (NT <= 15 ? 4 : 2); }
@@ -320,6 +324,11 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         const v4i* cur = odd ? buf1 : buf0;
         if (S + 1 < nchunks * SPC)
           dma_stage<NT>(wts + (size_t)(S + 1) * FRAG2, odd ? buf0 : buf1, wave, lane);
+        else if (!((nchunks * SPC) & 1) && wn < total_blocks)
+          // even stage count: the last stage reads buf1 and buf0 was released
+          // at the previous barrier, so the next block's first stage loads
+          // now, under this stage's MFMAs and the epilogue
+          dma_stage<NT>(reinterpret_cast<const v4i*>(sd[sn].wts), buf0, wave, lane);
         if (hc == 0) {
           const v4i mk = mybits[R * 64 + lane];
 #pragma unroll
@@ -335,6 +344,9 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
           if (C + 1 < nchunks) {
             dma_bits<R>(rowp, C + 1, mybits, lane);
             dma_mask(D.mask, C + 1, g, mybits + R * 64);
+          } else if (wn < total_blocks) {  // the next block's first chunk
+            dma_bits<R>(nrowp, 0, mybits, lane);
+            dma_mask(sd[sn].mask, 0, g, mybits + R * 64);
           }
         }
         // Software pipeline over the stage's SS NT weight fragments q = NT jj + t:
@@ -374,14 +386,18 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         // and buf0 (which the next block's first stage is loaded into) was
         // released at the previous barrier, so the early waves' epilogues
         // overlap the late waves' last MFMAs.
-        __builtin_amdgcn_s_waitcnt(0);
-        if (S + 1 < nchunks * SPC || ((nchunks * SPC) & 1)) lds_barrier();
+        // (No wait after an unbarriered last stage: what is in flight then is
+        // the next block's data, which the block's opening wait covers.)
+        if (S + 1 < nchunks * SPC || ((nchunks * SPC) & 1)) {
+          __builtin_amdgcn_s_waitcnt(0);
+          lds_barrier();
+        }
       }
     }
 
-    // this model's epilogue constants into LDS (when the model changes) while
-    // no direct-to-LDS load is in flight (the epilogue then reads only LDS);
-    // workgroup-uniform, and every wave finished the previous block's
+    // this model's epilogue constants into LDS (when the model changes; the
+    // wait also drains the next block's loads, so the epilogue then reads only
+    // LDS); workgroup-uniform, and every wave finished the previous block's
     // epilogue before this block's opening barrier
     const bool restage = ep.colrec != staged_cr;
     if (restage) {
@@ -392,13 +408,12 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
       staged_cr = ep.colrec;
     }
 
-    // next block: prefetch its first stage (buf0 is free: the last stage used
-    // buf1) and its first bits chunk
-    if (wn < total_blocks) {
-      dma_bits<R>(nrowp, 0, mybits, lane);
-      dma_mask(sd[sn].mask, 0, g, mybits + R * 64);
+    // next block's first stage, when the stage count is odd (the last stage
+    // read buf0, released by the barrier that closed it); with an even count
+    // it was loaded at the start of the last stage, and the first bits chunk
+    // at the start of the last chunk
+    if (wn < total_blocks && ((nchunks * SPC) & 1))
       dma_stage<NT>(reinterpret_cast<const v4i*>(sd[sn].wts), buf0, wave, lane);
-    }
     if (restage) lds_barrier();  // column records staged by every wave
 
     // ---- epilogue, from registers ----
@@ -1168,6 +1183,20 @@ __global__ void zero_u64(unsigned long long* p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
+// Accession masks of an all-pairs scan (src/kgwas_2d.cpp:372: x_ij = x_i * x_j):
+// out[k][w] = mask[w] & row (i0 + k)[w], w < W, for k < cnt.  The pair (i, j)
+// is then row j of the scan under mask k = i - i0.
+__global__ void pair_masks(const uint64_t* __restrict__ bits, int64_t rw,
+                           const uint64_t* __restrict__ mask, int64_t W, int64_t i0, int64_t cnt,
+                           uint64_t* __restrict__ out) {
+  const int64_t total = cnt * W;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = e / W, w = e - k * W;
+    out[e] = mask[w] & bits[(i0 + k) * rw + w];
+  }
+}
+
 __global__ void synth_bits(uint64_t* __restrict__ out, uint64_t seed, int64_t kmer0, int64_t m,
                            uint32_t n_acc, int64_t row_words, const uint8_t* __restrict__ pop,
                            const uint32_t* __restrict__ thr, uint32_t npop, uint32_t nprof) {
@@ -1360,6 +1389,17 @@ hipError_t set_kernel_attributes() {
   KG_ATTR(1) KG_ATTR(2) KG_ATTR(3) KG_ATTR(4) KG_ATTR(5) KG_ATTR(6) KG_ATTR(7) KG_ATTR(8)
 #undef KG_ATTR
   return e;
+}
+
+hipError_t launch_pair_masks(hipStream_t s, const uint64_t* bits, int64_t rw,
+                            const uint64_t* mask, int64_t W, int64_t i0, int64_t cnt,
+                            uint64_t* out) {
+  if (cnt <= 0) return hipSuccess;
+  const int64_t total = cnt * W;
+  const int64_t blocks = std::min<int64_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(pair_masks, dim3((unsigned)blocks), dim3(256), 0, s, bits, rw, mask, W, i0,
+                     cnt, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_zero(hipStream_t s, unsigned long long* p, int n) {

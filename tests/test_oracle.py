@@ -129,3 +129,75 @@ def test_fp32_restatement_close_to_fp64(built):
     c = ref["c"][keep][both].astype(np.float64)
     assert np.max(np.abs(cpu["xxM"][both] - ref["xxM"][keep][both]) / c) < 1e-4
     assert np.max(np.abs(cpu["xyM"][both] - ref["xyM"][keep][both]) / c) < 1e-4
+
+
+# ---- all-pairs scan (src/kgwas_2d.cpp:258-401) ----
+
+def _pair_setup(seed, intercept, n_acc=40, N=26, p=3):
+    from tests.golden.make_golden import orthonormal, pack
+    rng = np.random.default_rng(seed)
+    xacc = (rng.random((N, n_acc)) < rng.uniform(0.3, 0.9, (N, 1))).astype(np.uint8)
+    idxs = np.sort(rng.choice(n_acc, size=n_acc - 6, replace=False)).astype(np.int32)
+    # two rows carried by every phenotyped sample: their pair has c = n
+    xacc[3, idxs] = 1
+    xacc[7, idxs] = 1
+    n = len(idxs)
+    U = orthonormal(rng, n, p, intercept)
+    y = rng.standard_normal(n) + 0.8 * xacc[1, idxs] * xacc[2, idxs]
+    return pack(xacc), xacc[:, idxs].astype(np.float64), idxs, y, U
+
+
+def _pairs_by_lstsq(X, y, U, log10, op=np.logical_and):
+    """Expected pairs from independent least-squares fits of y on [U] and [U, x_i AND x_j]."""
+    from tests.golden.make_golden import expected
+    N = X.shape[0]
+    ii, jj = np.triu_indices(N, 1)
+    X12 = op(X[ii] > 0, X[jj] > 0).astype(np.float64)
+    c = X12.sum(axis=1)
+    T, p, eff = expected(X12, y, U)[:3]
+    keep = (c > 0) & (T > 0) & (p <= 10.0 ** (-log10))
+    return ii[keep], jj[keep], c[keep].astype(np.int32), T[keep], eff[keep], p[keep]
+
+
+@pytest.mark.parametrize("intercept", [True, False])
+def test_pairs_oracle_matches_lstsq(built, intercept):
+    bits, X, idxs, y, U = _pair_setup(11 if intercept else 12, intercept)
+    log10 = 0.4
+    r = oracle.pairs_f64(bits, idxs, y, U, log10)
+    ei, ej, ec, eT, eeff, ep = _pairs_by_lstsq(X, y, U, log10)
+    assert len(ei) > 20
+    assert np.array_equal(r["i"], ei) and np.array_equal(r["j"], ej)
+    assert np.array_equal(r["c"], ec)
+    assert np.allclose(r["T"], eT, rtol=1e-9, atol=1e-12)
+    assert np.allclose(r["effect"], eeff, rtol=1e-9, atol=1e-13)
+    assert np.allclose(r["p"], ep, rtol=1e-9, atol=1e-300)
+    # the pair (3, 7) has c = n: tested only when 1 is outside span(U)
+    full = (r["i"] == 3) & (r["j"] == 7)
+    assert r["c"][full].tolist() in ([], [len(idxs)])
+    T37 = _pairs_by_lstsq(X, y, U, 0.0)
+    k37 = (T37[0] == 3) & (T37[1] == 7)
+    assert k37.sum() == (0 if intercept else 1)
+
+
+def test_pairs_oracle_equals_1d_on_and_rows(built):
+    """Each pair's statistic is the 1D statistic of the AND row (src/kgwas_2d.cpp:372-390)."""
+    bits, X, idxs, y, U = _pair_setup(13, True)
+    r = oracle.pairs_f64(bits, idxs, y, U, 0.0)
+    and_rows = bits[r["i"]] & bits[r["j"]]
+    one = oracle.scan_f64(and_rows, idxs, y, U, 0.0)
+    assert np.array_equal(one["c"], r["c"])
+    assert np.allclose(one["T"], r["T"], rtol=1e-12, atol=1e-12)
+    assert np.allclose(one["effect"], r["effect"], rtol=1e-12, atol=1e-14)
+
+
+def test_pairs_oracle_row_ranges_and_planted_or(built):
+    bits, X, idxs, y, U = _pair_setup(14, True)
+    whole = oracle.pairs_f64(bits, idxs, y, U, 0.4)
+    a = oracle.pairs_f64(bits, idxs, y, U, 0.4, 0, 9)
+    b = oracle.pairs_f64(bits, idxs, y, U, 0.4, 9, None)
+    assert np.array_equal(np.concatenate([a["i"], b["i"]]), whole["i"])
+    assert np.array_equal(np.concatenate([a["j"], b["j"]]), whole["j"])
+    assert np.all(a["i"] < 9) and np.all(b["i"] >= 9)
+    # x_i OR x_j in place of AND would be caught
+    oi, oj = _pairs_by_lstsq(X, y, U, 0.4, op=np.logical_or)[:2]
+    assert not (np.array_equal(oi, whole["i"]) and np.array_equal(oj, whole["j"]))
