@@ -46,10 +46,11 @@ def main():
                     help="only the mixed scans (for rocprofv3 counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="copy this many bytes device-to-device and exit (counter calibration)")
-    ap.add_argument("--legs", default="c2,c4,c5",
+    ap.add_argument("--legs", default="c2,c4,c5,pairs",
                     help="further workloads beside the headline: c2 (FP32 kernel, n=1024 p=20, "
                          "50 batches of 1e6), c4 (400 uneven shards), c5 (n=4096 p=64 8 traits, "
-                         "streamed from page-locked host memory); '' for none")
+                         "streamed from page-locked host memory), pairs (all-pairs epistasis "
+                         "scan of 20,000 rows); '' for none")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -432,7 +433,66 @@ def leg_c5(args, synth, ds_c3):
     return out
 
 
-LEGS = {"c2": leg_c2, "c4": leg_c4, "c5": leg_c5}
+def leg_pairs(args, synth, ds):
+    """All-pairs epistasis scan (kgwas_2d, src/kgwas_2d.cpp:258-401) of 20,000 rows at n=2048,
+    p=32: every pair i < j tested as x_i AND x_j, p <= 1e-6 (the 2D default threshold).  Rows
+    are HBM-resident; the CPU baseline is the FP64 oracle on the pairs of the first rows, which
+    the GPU must reproduce exactly."""
+    import oracle
+    from supergwas_amd.api import KgwasHip, ScanParams
+    N, l10 = 20_000, 6.0
+    rw = synth.row_words_for(ds.n_acc)
+    k = KgwasHip(precision="mixed")
+    k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+    dev = k.alloc(N * rw * 8)
+    k.synth_bits(dev.ptr, ds, 0, N, rw)
+    prm = ScanParams(0.0, l10, 0.0)
+    k.scan_pairs(dev.ptr, N, rw, prm)
+    walls, kern = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        hits = k.scan_pairs(dev.ptr, N, rw, prm)
+        walls.append(time.perf_counter() - t0)
+        tm = k.timing()
+        kern.append(tm["bulk_ms"] + tm["refine_ms"])
+    npairs = N * (N - 1) // 2
+    w, kb = min(walls), min(kern) / 1e3
+    # MFMA work the kernel executes per pair, counted as for the headline: 3 limbs x p weight
+    # columns (the intercept direction comes from the allele count) in 16-row tiles
+    rows16 = 16 * int(np.ceil(3 * ds.p / 16.0))
+    exec_ops = 2.0 * ds.n_acc * rows16 * npairs
+    out = {"workload": f"all pairs of {N} rows, n={ds.n}, p={ds.p}, p <= 1e-{l10:g}",
+           "pairs": npairs, "pairs_per_s": npairs / w, "kernel_pairs_per_s": npairs / kb,
+           "kernel_ms": 1e3 * kb, "wall_ms": 1e3 * w, "hits": int(len(hits)),
+           "candidates": int(tm["candidates"]),
+           "executed_int8_TOPs": exec_ops / kb / 1e12,
+           "algorithmic_TFLOPs": 2.0 * ds.n * (ds.p + 1) * npairs / kb / 1e12}
+    log(f"[bench] pairs: {npairs / w:.3e} pairs/s end to end, {npairs / kb:.3e} pairs/s kernel "
+        f"({1e3 * kb:.1f} ms, {exec_ops / kb / 1e12:.0f} executed int8 TOP/s), {len(hits)} hits")
+    if not args.no_cpu:
+        R = 1500
+        bits = dev.to_host(np.uint64, (R, rw), nbytes=R * rw * 8)
+        t0 = time.perf_counter()
+        ref = oracle.pairs_f64(bits, ds.idxs, ds.y, ds.U, l10)
+        tc = time.perf_counter() - t0
+        sub = k.scan_pairs(dev.ptr, R, rw, prm)
+        same = (np.array_equal(sub["shard"].astype(np.int64), ref["i"]) and
+                np.array_equal(sub["idx"], ref["j"]) and np.array_equal(sub["c"], ref["c"]))
+        terr = float(np.max(np.abs(sub["T"] - ref["T"]) / np.maximum(ref["T"], 1.0),
+                            initial=0.0)) if same else None
+        rp = R * (R - 1) // 2
+        out["cpu_baseline"] = {"value": rp / tc, "unit": "pairs/s", "cores": os.cpu_count(),
+                               "kind": "port", "sample": f"all {rp} pairs of the first {R} rows "
+                               "(FP64 oracle, OpenMP over i)"}
+        out["parity_first_rows"] = {"rows": R, "hits": int(len(sub)), "identical_pairs": same,
+                                    "T_max_rel_err_vs_fp64": terr}
+        log(f"[bench] pairs CPU oracle {rp / tc:.3e} pairs/s on {R} rows; GPU identical: {same}")
+    dev.free()
+    k.close()
+    return out
+
+
+LEGS = {"c2": leg_c2, "c4": leg_c4, "c5": leg_c5, "pairs": leg_pairs}
 
 
 if __name__ == "__main__":
