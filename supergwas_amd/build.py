@@ -3,6 +3,7 @@
   lib/libkgwas_hip.so : gfx950 kernels + C ABI (hipcc)
   lib/libkgsynth.so   : host fill of the seeded synthetic bits (gcc)
   bin/kgwas           : drop-in `kgwas` executable (g++/hipcc link against libkgwas_hip)
+  bin/kgwas_2d        : drop-in `kgwas_2d` all-pairs executable (same)
 """
 import os
 import shutil
@@ -72,6 +73,11 @@ def build_kgwas_bin():
     _run(["g++", "-O3", "-std=c++17", "-fopenmp", "-I", INCLUDE, "-I", HOST, src,
           "-L", LIB, "-lkgwas_hip", "-lz", "-Wl,-rpath,$ORIGIN/../lib", "-o", tmp])
     os.replace(tmp, out)
+    src2 = os.path.join(HOST, "kgwas_2d_main.cpp")
+    out2 = os.path.join(BIN, "kgwas_2d")
+    _run(["g++", "-O3", "-std=c++17", "-I", INCLUDE, "-I", HOST, src2,
+          "-L", LIB, "-lkgwas_hip", "-lz", "-Wl,-rpath,$ORIGIN/../lib", "-o", out2 + ".tmp"])
+    os.replace(out2 + ".tmp", out2)
     return out
 
 
