@@ -46,11 +46,12 @@ def main():
                     help="only the mixed scans (for rocprofv3 counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="copy this many bytes device-to-device and exit (counter calibration)")
-    ap.add_argument("--legs", default="c2,c4,c5,pairs",
+    ap.add_argument("--legs", default="c2,c4,c5,pairs,ingest",
                     help="further workloads beside the headline: c2 (FP32 kernel, n=1024 p=20, "
                          "50 batches of 1e6), c4 (400 uneven shards), c5 (n=4096 p=64 8 traits, "
                          "streamed from page-locked host memory), pairs (all-pairs epistasis "
-                         "scan of 20,000 rows); '' for none")
+                         "scan of 20,000 rows), ingest (ASCII columns -> packed rows on the "
+                         "host); '' for none")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -492,7 +493,67 @@ def leg_pairs(args, synth, ds):
     return out
 
 
-LEGS = {"c2": leg_c2, "c4": leg_c4, "c5": leg_c5, "pairs": leg_pairs}
+def leg_ingest(args, synth, ds):
+    """Cols-mode ingest: the drop-in kgwas turns per-sample ASCII columns
+    (bits/col_%06d.txt, one byte per k-mer) into packed k-mer-major rows with SSE2 tile
+    transposes (KG_PACK_ONLY stops it there), beside the reference's loader restated
+    (load_matrix_rows, src/kgwas_utils.hpp:225-282: a float n x N matrix).  250,000
+    k-mers x 2048 accessions (512 MB of ASCII) from the page cache, 16 threads each."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    import oracle
+    from supergwas_amd import batchdir
+    N, threads = 250_000, 16
+    bits = ds.fill_bits(0, N)
+    root = tempfile.mkdtemp(prefix="kg_ingest_")
+    try:
+        paths = batchdir.write_inputs(root, ds, bits, rows_file=False)
+        exe = os.path.join(ROOT, "supergwas_amd", "bin", "kgwas")
+        out = os.path.join(root, "rows.bin")
+        env = dict(os.environ, KG_PACK_ONLY=out, LOAD_THREADS=str(threads))
+        cmd = [exe, paths["samples"], paths["pheno"], paths["batch"], paths["ubin"],
+               os.path.join(root, "o.tsv")]
+        walls, packs = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=300)
+            walls.append(time.perf_counter() - t0)
+            if r.returncode != 0:
+                raise RuntimeError("kgwas pack failed: " + r.stderr[-500:])
+            m = re.search(r"\[LOAD\] \d+/\d+  \(([0-9.e+-]+) s", r.stderr)
+            packs.append(float(m.group(1)))
+        rows = np.fromfile(out, np.uint64).reshape(N, -1)
+        mask = np.zeros(bits.shape[1], np.uint64)
+        for a in ds.idxs:
+            mask[a >> 6] |= np.uint64(1) << np.uint64(a & 63)
+        exact = bool(np.array_equal(rows, bits & mask))
+        del rows
+        tref = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            X = oracle.load_columns(os.path.join(paths["batch"], "bits"), ds.idxs, N, threads)
+            tref.append(time.perf_counter() - t0)
+            del X
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    ascii_gb = ds.n * N / 1e9
+    out = {"workload": f"{N} k-mers x {ds.n} phenotyped accessions, plain .txt columns "
+                       f"({ascii_gb * 1e3:.0f} MB ASCII, page cache)",
+           "threads": threads, "pack_s": min(packs), "pack_GBps": ascii_gb / min(packs),
+           "process_wall_s": min(walls), "rows_bit_exact": exact,
+           "cpu_baseline": {"value": ascii_gb / min(tref), "unit": "GB/s of ASCII",
+                            "cores": threads, "kind": "port",
+                            "sample": "load_matrix_rows restated (float n x N), same columns",
+                            "seconds": min(tref)}}
+    log(f"[bench] ingest: pack {min(packs):.3f} s ({ascii_gb / min(packs):.2f} GB/s), process "
+        f"{min(walls):.2f} s, bit-exact {exact}; reference-style float load {min(tref):.3f} s "
+        f"({ascii_gb / min(tref):.2f} GB/s)")
+    return out
+
+
+LEGS = {"c2": leg_c2, "c4": leg_c4, "c5": leg_c5, "pairs": leg_pairs, "ingest": leg_ingest}
 
 
 if __name__ == "__main__":

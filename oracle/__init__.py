@@ -58,6 +58,9 @@ def lib():
                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_int, _i64p, _i64p, _dp, _fp, _fp, _i64p, _i64p]
         L.cpu_cols_fp32.restype = ctypes.c_int64
+        L.cpu_load_columns.argtypes = [ctypes.c_char_p, _i32p, ctypes.c_int, ctypes.c_int64, _fp,
+                                       ctypes.c_int]
+        L.cpu_load_columns.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -170,3 +173,15 @@ def cols_fp32(X, y, U, corr_thr, log10_thresh, log10_thresh_2d, threads):
                                 _ptr(xx, _fp), _ptr(xy, _fp), _ptr(keep, _i64p), ctypes.byref(tb))
     return dict(n_passed=int(npass), hits_1d=h1.value, hits_2d=h2.value, t_stage=ts,
                 xxM=xx[:npass], xyM=xy[:npass], keep=keep[:npass], text_bytes=tb.value)
+
+
+def load_columns(bits_dir, idxs, N, threads):
+    """load_matrix_rows restated (src/kgwas_utils.hpp:225-282): X[n, N] float32 from
+    plain .txt columns."""
+    idxs = np.ascontiguousarray(idxs, dtype=np.int32)
+    X = np.empty((len(idxs), N), np.float32)
+    rc = lib().cpu_load_columns(bits_dir.encode(), _ptr(idxs, _i32p), len(idxs), int(N),
+                                _ptr(X, _fp), int(threads))
+    if rc != 0:
+        raise RuntimeError(f"column of sample {-1 - rc} missing or short")
+    return X

@@ -11,12 +11,18 @@
 //                          FP64 yyM
 //   src/kgwas.cpp:225-259  emit: FP64 compute_hit per k-mer, OpenMP over
 //                          1,000,000-k-mer chunks, "%g" text per survivor
+//   src/kgwas_utils.hpp:225-282  load_matrix_rows (plain .txt columns): the
+//                          ingest baseline beside the packed-row loader
 // The reference multiplies with OpenBLAS sgemm/sgemv (Eigen -DEIGEN_USE_BLAS);
 // this file uses a cache-blocked FP32 kernel with runtime ISA dispatch
 // (target_clones) in their place, so FP32 rounding matches the reference's
 // only up to summation order.
+#include <fcntl.h>
 #include <math.h>
 #include <omp.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -234,6 +240,43 @@ int64_t cpu_cols_fp32(const float* X_in, int n, int64_t N, const double* y, cons
     if (keep_out) keep_out[b] = (passed < N) ? keep[(size_t)b] : b;
   }
   return passed;
+}
+
+// load_matrix_rows restated for plain .txt columns (src/kgwas_utils.hpp:225-282):
+// sample k's column bits_dir/col_%06d.txt (accession idxs[k]) is mapped and
+// converted to X[k][i] = float(byte_i & 1), OpenMP over samples.  Returns 0, or
+// -1 - k when sample k's column is missing or shorter than N.
+int cpu_load_columns(const char* bits_dir, const int32_t* idxs, int n, int64_t N, float* X,
+                     int threads) {
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+  for (int k = 0; k < n; ++k) {
+    char base[64];
+    snprintf(base, sizeof(base), "/col_%06d.txt", idxs[k]);
+    const std::string path = std::string(bits_dir) + base;
+    const int fd = ::open(path.c_str(), O_RDONLY);
+    struct stat st {};
+    if (fd < 0 || fstat(fd, &st) != 0 || (int64_t)st.st_size < N) {
+      if (fd >= 0) ::close(fd);
+#pragma omp critical
+      bad = bad ? bad : -1 - k;
+      continue;
+    }
+    void* p = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+    ::close(fd);
+    if (p == MAP_FAILED) {
+#pragma omp critical
+      bad = bad ? bad : -1 - k;
+      continue;
+    }
+    madvise(p, (size_t)st.st_size, MADV_SEQUENTIAL);
+    const unsigned char* bytes = (const unsigned char*)p;
+    float* row = X + (size_t)k * (size_t)N;
+#pragma omp simd
+    for (int64_t i = 0; i < N; ++i) row[i] = (float)(bytes[i] & 1u);
+    munmap(p, (size_t)st.st_size);
+  }
+  return bad;
 }
 
 }  // extern "C"

@@ -22,8 +22,10 @@ def unpack_acc(bits, n_acc):
     return ((bits[:, a >> 6] >> (a & 63).astype(np.uint64)) & 1).astype(np.uint8)  # m x n_acc
 
 
-def write_inputs(root, ds, bits, gz_cols=False, rows_gz=False, missing_pheno_token=None):
-    """Writes everything under `root`; returns a dict of paths."""
+def write_inputs(root, ds, bits, gz_cols=False, rows_gz=False, missing_pheno_token=None,
+                 rows_file=True):
+    """Writes everything under `root` (the rows-mode file only if rows_file); returns a
+    dict of paths."""
     os.makedirs(root, exist_ok=True)
     ids = [f"acc{j:05d}" for j in range(ds.n_acc)]
     acc = os.path.join(root, "samples.txt")
@@ -56,11 +58,12 @@ def write_inputs(root, ds, bits, gz_cols=False, rows_gz=False, missing_pheno_tok
     with gzip.open(os.path.join(batch, "kmers.txt.gz"), "wt") as f:
         f.write("\n".join(names) + "\n")
     rows = os.path.join(root, "rows.tsv.gz" if rows_gz else "rows.tsv")
-    opener = gzip.open if rows_gz else open
-    with opener(rows, "wt") as f:
-        chars = (X + ord("0")).astype(np.uint8)
-        for i in range(m):
-            f.write(names[i] + "\t" + chars[i].tobytes().decode() + "\n")
+    if rows_file:
+        opener = gzip.open if rows_gz else open
+        with opener(rows, "wt") as f:
+            chars = (X + ord("0")).astype(np.uint8)
+            for i in range(m):
+                f.write(names[i] + "\t" + chars[i].tobytes().decode() + "\n")
     return dict(samples=acc, pheno=ph, ubin=ub, batch=batch, rows=rows, names=names)
 
 

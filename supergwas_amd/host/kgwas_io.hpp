@@ -23,6 +23,7 @@
 #include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
+#include <emmintrin.h>
 #include <zlib.h>
 
 #include <atomic>
@@ -293,10 +294,10 @@ inline void open_column(const string& bits_dir, int acc, size_t N, Column& col) 
       die("short column: " + txt + " (size " + std::to_string(sz) + " < N " + std::to_string(N) +
           ")");
     }
-    void* p = mmap(nullptr, sz, PROT_READ, MAP_PRIVATE, fd, 0);
+    // page tables filled here, by the thread that opens the column
+    void* p = mmap(nullptr, sz, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
     ::close(fd);
     if (p == MAP_FAILED) die("mmap: " + txt);
-    madvise(p, sz, MADV_SEQUENTIAL);
     col.data = (const unsigned char*)p;
     col.map_len = sz;
     return;
@@ -332,8 +333,43 @@ inline void open_column(const string& bits_dir, int acc, size_t N, Column& col) 
   die("missing column: " + txt + " (and " + gz + ")");
 }
 
+// 16 accessions x 16 k-mers of column bytes -> one 16-bit mask per k-mer (bit q
+// = byte & 1 of accession q): a 16 x 16 byte transpose by unpacks, then
+// movemask of each k-mer's 16 bytes shifted so that bit 0 is the sign bit.
+inline void pack_tile16(const unsigned char* const* col, size_t i, uint16_t* out,
+                        size_t out_stride) {
+  __m128i a[16], b[16];
+  for (int q = 0; q < 16; ++q) a[q] = _mm_loadu_si128((const __m128i*)(col[q] + i));
+  // a[q] byte k = (accession q, k-mer k).  Each stage interleaves pairs of
+  // vectors, doubling the run of accessions held together per k-mer.
+  for (int j = 0; j < 8; ++j) {  // 2 accessions x 8 k-mers per vector
+    b[j] = _mm_unpacklo_epi8(a[2 * j], a[2 * j + 1]);
+    b[j + 8] = _mm_unpackhi_epi8(a[2 * j], a[2 * j + 1]);
+  }
+  for (int j = 0; j < 4; ++j) {  // 4 accessions x 4 k-mers
+    a[j] = _mm_unpacklo_epi16(b[2 * j], b[2 * j + 1]);
+    a[j + 4] = _mm_unpackhi_epi16(b[2 * j], b[2 * j + 1]);
+    a[j + 8] = _mm_unpacklo_epi16(b[8 + 2 * j], b[8 + 2 * j + 1]);
+    a[j + 12] = _mm_unpackhi_epi16(b[8 + 2 * j], b[8 + 2 * j + 1]);
+  }
+  for (int h = 0; h < 4; ++h)    // 8 accessions x 2 k-mers
+    for (int j = 0; j < 2; ++j) {
+      b[4 * h + j] = _mm_unpacklo_epi32(a[4 * h + 2 * j], a[4 * h + 2 * j + 1]);
+      b[4 * h + j + 2] = _mm_unpackhi_epi32(a[4 * h + 2 * j], a[4 * h + 2 * j + 1]);
+    }
+  for (int h = 0; h < 8; ++h) {  // 16 accessions x 1 k-mer
+    a[2 * h] = _mm_unpacklo_epi64(b[2 * h], b[2 * h + 1]);
+    a[2 * h + 1] = _mm_unpackhi_epi64(b[2 * h], b[2 * h + 1]);
+  }
+  for (int k = 0; k < 16; ++k)
+    out[(size_t)k * out_stride] = (uint16_t)_mm_movemask_epi8(_mm_slli_epi64(a[k], 7));
+}
+
 // Packs the phenotyped samples' columns into k-mer-major rows:
-// rows[i * row_words + a/64] bit a%64 = byte_i(col of accession a) & 1.
+// rows[i * row_words + a/64] bit a%64 = byte_i(col of accession a) & 1 (0 for
+// accessions without a phenotype).  Tiles of 16 accessions x 16 k-mers are
+// transposed in SSE2 registers and stored as 16-bit row pieces; an accession
+// without a phenotype reads a zero column.
 inline vector<uint64_t> pack_batch(const string& bits_dir, const vector<int32_t>& idxs, size_t N,
                                    int64_t row_words, int threads) {
   Clock t;
@@ -349,21 +385,43 @@ inline vector<uint64_t> pack_batch(const string& bits_dir, const vector<int32_t>
       std::cerr << "\r[LOAD] " << d << "/" << n << std::flush;
     }
   }
+  const double t_open = t.sec();
+  // column of every accession up to the last phenotyped one, in groups of 16
+  const int n_acc = n ? idxs.back() + 1 : 0;
+  const int ngroups = (n_acc + 15) / 16;
+  const int64_t BK = 1024;  // k-mers per tile row (a multiple of 16)
+  vector<unsigned char> zeros((size_t)BK, 0);
+  vector<const unsigned char*> acc_col((size_t)ngroups * 16, nullptr);
+  for (int k = 0; k < n; ++k) acc_col[(size_t)idxs[k]] = cols[(size_t)k].data;
   vector<uint64_t> rows((size_t)N * (size_t)row_words, 0ull);
-  const int64_t BK = 4096;
+  const size_t stride16 = (size_t)row_words * 4;  // uint16 pieces per row
+  const int64_t nfull = (int64_t)(N / 16) * 16;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
-  for (int64_t i0 = 0; i0 < (int64_t)N; i0 += BK) {
-    const int64_t i1 = std::min<int64_t>(i0 + BK, (int64_t)N);
-    for (int k = 0; k < n; ++k) {
-      const int a = idxs[k];
-      const unsigned char* c = cols[k].data;
-      const uint64_t sh = (uint64_t)(a & 63);
-      uint64_t* base = rows.data() + (a >> 6);
-      for (int64_t i = i0; i < i1; ++i) base[i * row_words] |= (uint64_t)(c[i] & 1u) << sh;
+  for (int64_t i0 = 0; i0 < nfull; i0 += BK) {
+    const int64_t i1 = std::min<int64_t>(i0 + BK, nfull);
+    const unsigned char* cp[16];
+    for (int g = 0; g < ngroups; ++g) {
+      bool any = false;
+      for (int q = 0; q < 16; ++q) {
+        const unsigned char* c = acc_col[(size_t)g * 16 + q];
+        any |= c != nullptr;
+        cp[q] = c ? c + i0 : zeros.data();
+      }
+      if (!any) continue;
+      uint16_t* out = reinterpret_cast<uint16_t*>(rows.data()) + (size_t)i0 * stride16 + g;
+      for (int64_t i = 0; i < i1 - i0; i += 16)
+        pack_tile16(cp, (size_t)i, out + (size_t)i * stride16, stride16);
     }
   }
-  std::cerr << "\r[LOAD] " << n << "/" << n << "  (" << t.sec() << " s, packed " << N
-            << " k-mers x " << row_words << " words)\n";
+  for (int k = 0; k < n; ++k) {  // k-mers past the last full tile of 16
+    const int a = idxs[k];
+    const unsigned char* c = cols[k].data;
+    for (int64_t i = nfull; i < (int64_t)N; ++i)
+      rows[(size_t)i * row_words + (a >> 6)] |= (uint64_t)(c[i] & 1u) << (a & 63);
+  }
+  std::cerr << "\r[LOAD] " << n << "/" << n << "  (" << t.sec() << " s: open " << t_open
+            << " s, pack " << t.sec() - t_open << " s; " << N << " k-mers x " << row_words
+            << " words)\n";
   return rows;
 }
 

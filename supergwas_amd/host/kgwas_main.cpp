@@ -8,7 +8,9 @@
 // CORR_THRESHOLD, LOG10_THRESH_2D_INPUT), outputs and error behaviour
 // ("[ERROR] msg" + exit 1) follow src/kgwas.cpp:26-126, 168-307 (cols) and
 // 343-522 (rows).  Additional environment: KG_DEVICE (HIP device, default 0),
-// KG_PRECISION (mixed | fp32, default mixed), KG_BATCH_KMERS (rows-mode batch).
+// KG_PRECISION (mixed | fp32, default mixed), KG_BATCH_KMERS (rows-mode batch),
+// KG_PACK_ONLY (cols mode: write the packed rows, uint64 [N][row_words], to
+// this path and stop before the scan).
 //
 // Numerics: every reported k-mer's effect/T/p come from FP64 values computed
 // on the GPU and the reference's compute_hit on the host, so cols mode prints
@@ -164,6 +166,13 @@ void run_cols(const Config& cfg) {
   const int n_acc = (int)ov.accessions.size();
   const int64_t rw = row_words_for(n_acc);
   vector<uint64_t> rows = pack_batch(bits_dir, ov.idxs, N, rw, cfg.load_threads);
+  if (const char* dump = std::getenv("KG_PACK_ONLY")) {  // diagnostic: packed rows, no scan
+    std::ofstream out(dump, std::ios::binary);
+    out.write(reinterpret_cast<const char*>(rows.data()),
+              (std::streamsize)(sizeof(uint64_t) * rows.size()));
+    if (!out) die("write: " + string(dump));
+    return;
+  }
 
   Clock tscan;
   Scanner sc(n_acc, ov, u);

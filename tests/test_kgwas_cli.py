@@ -211,3 +211,30 @@ def test_parse_quirks(small, tmp_path):
     assert _overlap_n(p, tmp_path, "a\nb\n", "a 1\na 2\nb 3\n") == 2
     # accession order defines the overlap, not phenotype order
     assert _overlap_n(p, tmp_path, "x\nb\na\n", "a 1\nb 2\n") == 2
+
+
+@pytest.mark.parametrize("gz", [False, True])
+def test_packed_rows_bit_exact(small, tmp_path, gz):
+    """The SSE2 tile packer (16 accessions x 16 k-mers per transpose) against the
+    expected rows: the phenotyped accessions' bits, zero elsewhere, including the
+    k-mers past the last full tile of 16 (3000 = 187 x 16 + 8)."""
+    ds, bits, p, root = small
+    batch = p["batch"]
+    if gz:
+        batch = str(tmp_path / "batch")
+        shutil.copytree(p["batch"], batch)
+        for a in range(0, ds.n_acc, 3):
+            col = os.path.join(batch, "bits", f"col_{a:06d}.txt")
+            data = open(col, "rb").read()
+            os.remove(col)
+            with gzip.open(col + ".gz", "wb") as f:
+                f.write(data)
+    out = tmp_path / "rows.bin"
+    r = run(base_args(p, str(tmp_path / "o.tsv"), batch),
+            env={"KG_PACK_ONLY": str(out), "LOAD_THREADS": "3"})
+    assert r.returncode == 0, r.stderr
+    rows = np.fromfile(out, np.uint64).reshape(len(bits), -1)
+    mask = np.zeros(bits.shape[1], np.uint64)
+    for a in ds.idxs:
+        mask[a >> 6] |= np.uint64(1) << np.uint64(a & 63)
+    assert np.array_equal(rows, bits & mask)
