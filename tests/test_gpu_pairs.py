@@ -163,3 +163,30 @@ def test_pairs_argument_errors(built):
         with pytest.raises(KgError):
             k.scan_pairs(dev.ptr, 4, bits.shape[1], ScanParams(0.2, l10, 0.0))
         dev.free()
+
+
+def test_pairs_across_batches(built):
+    """25,000 rows = 3.1e8 pairs: the scan runs as two batches of rows i (at most ~2.7e8
+    pairs each), each with its own shard table and pair masks.  The result equals row-range
+    scans that each fit one batch, and the pairs of sampled rows i equal the oracle's."""
+    ds = synth.make_dataset(seed=44, n_acc=200, n_missing=10, npop=3, p=5, m=25000, n_causal=5,
+                            effect=0.5)
+    N = 25000
+    bits = ds.fill_bits(0, N)
+    l10 = 4.0
+    print(f"[pairs] batches N={N} pairs={N * (N - 1) // 2}", flush=True)
+    (whole,), tim = _scan_pairs(ds, bits, l10, ranges=[(0, N)])
+    parts, _ = _scan_pairs(ds, bits, l10, ranges=[(0, 3000), (3000, N)])
+    joined = np.concatenate(parts)
+    print(f"[pairs] batches hits={len(whole)} candidates={tim['candidates']}", flush=True)
+    assert tim["kmers"] == N * (N - 1) // 2
+    assert len(whole) > 100
+    for key in ("shard", "idx", "c", "T", "effect", "pval"):
+        assert np.array_equal(joined[key], whole[key]), key
+    for i in (0, 1, 2999, 3000, 12345, 24990, 24998):
+        ref = oracle.pairs_f64(bits, ds.idxs, ds.y, ds.U, l10, i, i + 1)
+        got = whole[whole["shard"] == i]
+        assert np.array_equal(got["idx"], ref["j"]), i
+        assert np.array_equal(got["c"], ref["c"]), i
+        if len(got):
+            assert np.max(np.abs(got["T"] - ref["T"]) / ref["T"]) <= REPORT_TOL
