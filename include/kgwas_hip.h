@@ -55,7 +55,8 @@ typedef struct kg_opts {
   int32_t precision;   /* KG_PREC_MIXED or KG_PREC_FP32 */
   int32_t limbs;       /* int8 limbs per weight column for KG_PREC_MIXED (2 or 3; 0 = 3) */
   int32_t screen;      /* KG_SCREEN_AUTO (0), KG_SCREEN_OFF or KG_SCREEN_ON */
-  int32_t reserved[4];
+  int32_t dedup;       /* 1: duplicate-row scans (kg_set_dedup); 0 (default): off */
+  int32_t reserved[3];
 } kg_opts;
 
 /* Thresholds of one scan (src/kgwas.cpp:28-30, 101-112, 230-232). */
@@ -178,6 +179,18 @@ int kg_last_timing(const kg_ctx* ctx, kg_timing* out);
 /* Changes the context's pre-filter screen mode (KG_SCREEN_*) for later scans
  * (no reference counterpart: the reference always filters first). */
 int kg_set_screen(kg_ctx* ctx, int32_t mode);
+
+/* Duplicate-row scans (the batches of src/convert_matrix.cpp:366-368 are sorted
+ * by bit pattern, so k-mers with equal rows are adjacent): when on, a screened
+ * scan compares every row with the previous one under the accession mask
+ * inside the pre-filter screen, runs the full kernel on the first row of each
+ * run of equal rows only, counts that row once per member of its run and
+ * reports every member of a reported run (same values, consecutive idx).
+ * Results are identical either way; it needs the screen (corr_threshold > 0,
+ * screen mode not KG_SCREEN_OFF) and keeps it on in KG_SCREEN_AUTO.  Runs are
+ * detected within each 64 (or 32) k-mer wave tile of the screen.  No reference
+ * counterpart (the reference computes every k-mer). */
+int kg_set_dedup(kg_ctx* ctx, int32_t on);
 
 /* Launch-shape autotuner (no reference counterpart; the reference has one
  * code path): times the int8 bulk scan of `sample` random rows with model 0

@@ -38,7 +38,7 @@ class KgError(RuntimeError):
 class _Opts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("limbs", ctypes.c_int32), ("screen", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("dedup", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class ScanParams(ctypes.Structure):
@@ -92,6 +92,7 @@ SIGNATURES = {
     "kg_stats_device": ([_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp], ctypes.c_int),
     "kg_last_timing": ([_vp, ctypes.POINTER(_Timing)], ctypes.c_int),
     "kg_set_screen": ([_vp, _i32], ctypes.c_int),
+    "kg_set_dedup": ([_vp, _i32], ctypes.c_int),
     "kg_dev_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
     "kg_dev_free": ([_vp, _vp], ctypes.c_int),
     "kg_host_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
@@ -155,9 +156,9 @@ class DeviceArray:
 class KgwasHip:
     """One device context (kg_create) with one null model (kg_set_model)."""
 
-    def __init__(self, device=0, precision="mixed", limbs=3, screen="auto"):
+    def __init__(self, device=0, precision="mixed", limbs=3, screen="auto", dedup=False):
         L = lib()
-        opts = _Opts(device, PRECISIONS[precision], limbs, SCREEN[screen])
+        opts = _Opts(device, PRECISIONS[precision], limbs, SCREEN[screen], 1 if dedup else 0)
         h = _vp()
         rc = L.kg_create(ctypes.byref(h), ctypes.byref(opts))
         self._h = h
@@ -319,6 +320,10 @@ class KgwasHip:
         self._check(lib().kg_memcpy(self._h, d.ptr, _p(arr), arr.nbytes, 0))
         d.shape, d.dtype = arr.shape, arr.dtype
         return d
+
+    def set_dedup(self, on):
+        """Duplicate-row scans on (True) or off for later scans (kg_set_dedup)."""
+        self._check(lib().kg_set_dedup(self._h, 1 if on else 0))
 
     def set_screen(self, mode):
         """Pre-filter screen mode for later scans: "auto", "off" or "on"."""

@@ -27,13 +27,13 @@ namespace kg {
 hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t s,
                           const ShardDesc* sd, int nshards, int64_t total_blocks,
                           const ScanOut& o, const int64_t* total_dev);
-int64_t screen_block_kmers(int t);
+int64_t screen_block_kmers(int t, bool dd);
 int screen_rows(int t);
 bool screen_count_row(int t);
 int screen_max_chunks();
-int screen_wgs_per_cu(int t, int nch);
+int screen_wgs_per_cu(int t, int nch, bool dd);
 hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
-                         int nshards, int64_t total_blocks);
+                         int nshards, int64_t total_blocks, bool dd);
 int screen_chunk_words();
 hipError_t launch_screen_compact(hipStream_t s, const ScrChunk* ch, int ncc, const ScreenDesc* ss,
                                  uint32_t* ccount, int64_t* P, ShardDesc* sd2, int nsh,
@@ -203,9 +203,11 @@ struct kg_ctx {
   // scans left before the screen is tried again
   int screen_mode = KG_SCREEN_AUTO;
   int screen_skip = 0;
+  int dedup = 0;  // screened scans check each row against the previous one
   DBuf surv, ssdesc, sdesc2, chdesc;  // survivor rows; graph-path screen tables
   DBuf bmap, scr_aux;                 // screen bitmaps; chunk counts and prefixes
   DBuf pmask;                         // all-pairs scan: mask AND row i, per shard
+  DBuf dmap, smult;                   // duplicate-row scans: dup bitmaps, survivor run lengths
   hipEvent_t evs = nullptr;   // end of the screen kernel
   int scr_occ_key = -1, scr_occ = 1;  // cached screen workgroups per CU
   // captured graph of the last batched scan
@@ -640,6 +642,11 @@ extern "C" int kg_create(kg_ctx** out, const kg_opts* opts) {
     ctx->precision = opts->precision;
     if (opts->limbs) ctx->limbs = opts->limbs;
     ctx->screen_mode = opts->screen;
+    ctx->dedup = opts->dedup;
+  }
+  if (ctx->dedup != 0 && ctx->dedup != 1) {
+    delete ctx;
+    return KG_ERR_ARG;
   }
   if (ctx->screen_mode < KG_SCREEN_AUTO || ctx->screen_mode > KG_SCREEN_ON) {
     delete ctx;
@@ -842,6 +849,7 @@ struct ScreenRun {
   int64_t sblocks = 0;          // screen blocks
   int sgrid = 0;
   int nch_max = 0;
+  bool dd = false;              // with the duplicate-row check
 };
 
 // Device part of a batched int8 scan: zero counters, [screen, fix-up,] bulk
@@ -855,7 +863,8 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
   KG_CK(ctx, launch_zero(ctx->stream, o.counters, (int)ctx->counters_len));
   KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (sr.on) {
-    KG_CK(ctx, launch_screen(t, sr.nch_max, sr.sgrid, ctx->stream, sr.d_ss, nsh, sr.sblocks));
+    KG_CK(ctx, launch_screen(t, sr.nch_max, sr.sgrid, ctx->stream, sr.d_ss, nsh, sr.sblocks,
+                             sr.dd));
     KG_CK(ctx, hipEventRecord(ctx->evs, ctx->stream));
     KG_CK(ctx, launch_screen_compact(ctx->stream, sr.d_ch, sr.ncc, sr.d_ss, sr.ccount, sr.P,
                                      sr.d_sd2, nsh, i8_block_kmers(NT, L, t, ctx->tiles),
@@ -921,7 +930,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     nch_max = std::max(nch_max, M.md.nch_i8);
     mtot += s.m;
   }
-  if (screen && ctx->screen_mode == KG_SCREEN_AUTO && ctx->screen_skip > 0) {
+  if (screen && ctx->screen_mode == KG_SCREEN_AUTO && ctx->screen_skip > 0 && !ctx->dedup) {
     --ctx->screen_skip;
     screen = false;
   }
@@ -976,15 +985,22 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   std::vector<ShardDesc> sd2;
   std::vector<ScrChunk> chunks;
   ScreenRun sr;
+  const bool dd = screen && ctx->dedup != 0;
   if (screen) {
-    const int64_t sbk = screen_block_kmers(t);
+    const int64_t sbk = screen_block_kmers(t, dd);
     const int64_t cw = screen_chunk_words();
     int64_t bm_words = 0;
     for (int s = 0; s < nsh; ++s) bm_words += (sh[s].m + sbk - 1) / sbk * (sbk / 32);
     KG_CK(ctx, ctx->surv.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(mtot, 1)));
     KG_CK(ctx, ctx->bmap.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(bm_words, 1)));
+    if (dd) {
+      KG_CK(ctx, ctx->dmap.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(bm_words, 1)));
+      KG_CK(ctx, ctx->smult.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(mtot, 1)));
+    }
     uint32_t* sv = ctx->surv.as<uint32_t>();
     uint32_t* bmw = ctx->bmap.as<uint32_t>();
+    uint32_t* dmw = dd ? ctx->dmap.as<uint32_t>() : nullptr;
+    uint32_t* mlw = dd ? ctx->smult.as<uint32_t>() : nullptr;
     ss.resize(nsh);
     sd2 = sd;
     int64_t off = 0, boff = 0;
@@ -1000,8 +1016,10 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       S.wts = M.swts.p;
       S.mask = M.mask.p;
       S.bmap = bmw + boff;
+      S.dmap = dd ? dmw + boff : nullptr;
       boff += (sh[s].m + sbk - 1) / sbk * (sbk / 32);
       S.list = sv + off;
+      S.mult = dd ? mlw + off : nullptr;
       S.nsurv = dc + base_surv + s;
       S.chunk0 = (int)chunks.size();
       const int64_t used_words = (sh[s].m + 31) / 32;
@@ -1028,11 +1046,13 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
                          : -1.0f;
       }
       sd2[s].rows = S.list;
+      sd2[s].mult = S.mult;
       sd2[s].m = 0;     // written on the device by the fix-up
       sd2[s].blk0 = 0;
       off += sh[s].m;
     }
     sr.on = true;
+    sr.dd = dd;
     sr.nch_max = nch_max;
     sr.d_total = reinterpret_cast<int64_t*>(dc + base_surv + nsh);
     sr.ncc = (int)chunks.size();
@@ -1040,9 +1060,9 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
                                    sizeof(int64_t) * (chunks.size() + 2)));
     sr.P = ctx->scr_aux.as<int64_t>();
     sr.ccount = reinterpret_cast<uint32_t*>(sr.P + chunks.size() + 2);
-    const int okey = t * 64 + nch_max;
+    const int okey = (t * 64 + nch_max) * 2 + (dd ? 1 : 0);
     if (ctx->scr_occ_key != okey) {
-      ctx->scr_occ = screen_wgs_per_cu(t, nch_max);
+      ctx->scr_occ = screen_wgs_per_cu(t, nch_max, dd);
       ctx->scr_occ_key = okey;
     }
     sr.sgrid = ctx->cus * ctx->scr_occ;
@@ -1254,7 +1274,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     // auto: when the screen plus the survivor scan took longer than the full
     // kernel would have over every k-mer (its per-survivor time scaled to all
     // k-mers), skip the screen for the next 8 scans, then measure again
-    if (ctx->screen_mode == KG_SCREEN_AUTO && !use_graph && ns > 0 &&
+    if (ctx->screen_mode == KG_SCREEN_AUTO && !use_graph && ns > 0 && !ctx->dedup &&
         ctx->timing.screen_ms + ctx->timing.bulk_ms >
             ctx->timing.bulk_ms * ((double)mtot / (double)ns))
       ctx->screen_skip = 8;
@@ -1280,10 +1300,15 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     const int c = (int)res[4 * i + 0];
     Decided d = decide(md, s.idx0 + cd.idx, cd.trait, c, res[4 * i + 1], res[4 * i + 2],
                        res[4 * i + 3], !pairs, prm, pairs);
-    if ((cd.flags & kPfAmb) && d.pass) n_passed[cd.shard * t + cd.trait] += 1;
+    // a duplicate-row scan decided row idx for the cd.mult equal rows idx, idx + 1, ...
+    const int mult = cd.mult > 1 ? cd.mult : 1;
+    if ((cd.flags & kPfAmb) && d.pass) n_passed[cd.shard * t + cd.trait] += mult;
     if (d.hit) {
       d.h.shard = cd.shard;
-      hits.push_back(d.h);
+      for (int q = 0; q < mult; ++q) {
+        hits.push_back(d.h);
+        ++d.h.idx;
+      }
     }
   }
   return KG_OK;
@@ -1751,6 +1776,13 @@ extern "C" int kg_set_screen(kg_ctx* ctx, int32_t mode) {
     return fail(ctx, KG_ERR_ARG, "screen mode must be KG_SCREEN_AUTO, _OFF or _ON");
   ctx->screen_mode = mode;
   ctx->screen_skip = 0;
+  return KG_OK;
+}
+
+extern "C" int kg_set_dedup(kg_ctx* ctx, int32_t on) {
+  if (!ctx) return KG_ERR_ARG;
+  if (on != 0 && on != 1) return fail(ctx, KG_ERR_ARG, "dedup must be 0 or 1");
+  ctx->dedup = on;
   return KG_OK;
 }
 

@@ -16,7 +16,7 @@ struct Cand {
   int32_t shard;
   float xx32;      // plain FP32 path: the reference-arithmetic xxM, xyM
   float xy32;
-  int32_t pad;
+  int32_t mult;    // rows idx .. idx + mult - 1 are this row (duplicate-row scans), else 1
 };
 constexpr uint32_t kPfPass = 1u;  // pre-filter decided pass
 constexpr uint32_t kPfAmb = 2u;   // pre-filter within the error bound of tau
@@ -86,6 +86,7 @@ struct ShardDesc {
   int id;                        // index of this shard in the caller's table
   int pad2;
   const uint32_t* rows;          // k-mer j of the scan is row rows[j] (null: row j)
+  const uint32_t* mult;          // k-mer j stands for mult[j] equal rows (null: 1 each)
   EpiI8 ep;
 };
 
@@ -107,6 +108,11 @@ struct ScreenDesc {
   uint32_t* bmap;                // kept k-mers: bit j % 32 of word j / 32
   uint32_t* list;                // survivors: rows within the shard, increasing
   unsigned long long* nsurv;     // survivor count (written by the compaction)
+  // duplicate-row scans (null otherwise): bit j of dmap set = row j equals row
+  // j - 1 under the accession mask (it is then not a survivor); mult[i] = the
+  // run of equal rows that survivor list[i] starts
+  uint32_t* dmap;
+  uint32_t* mult;
   int chunk0, chunk1;            // this shard's compaction chunks [chunk0, chunk1)
   int nchunks, t, n, pad;
   float wf[kMaxTraits];          // S~ = w_tr (254 A_0 + A_1), A_l = limb sums

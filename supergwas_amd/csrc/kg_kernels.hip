@@ -489,6 +489,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     const int64_t kidx = kbase + 16 * rme + col;
     const bool valid = primary && (kidx < D.m);
     const int64_t krow = (D.rows != nullptr && valid) ? (int64_t)gptr(D.rows)[kidx] : kidx;
+    const uint32_t kmult = (D.mult != nullptr && valid) ? gptr(D.mult)[kidx] : 1u;
     const double cd = (double)c, cn = cd * ep.inv_n;
     // |dz_k| <= c h_k  =>  |d |z|^2| <= 2c sum|z_k| h_k + c^2 sum h_k^2
     //                              <= 2c |z| |h| + c^2 |h|^2
@@ -527,7 +528,13 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
           out.st_xy[kidx * t + tr] = xyM;
           out.st_S[kidx * t + tr] = S;
         }
-        wpass[tr] += (uint32_t)__popcll(__ballot(pf_def));
+        if (D.mult == nullptr) {
+          wpass[tr] += (uint32_t)__popcll(__ballot(pf_def));
+        } else {  // each k-mer stands for kmult equal rows
+          uint32_t v = pf_def ? kmult : 0u;
+          for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+          wpass[tr] += v;
+        }
         Cand rec;
         rec.idx = krow;
         rec.trait = tr;
@@ -535,7 +542,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         rec.shard = D.id;
         rec.xx32 = 0.f;
         rec.xy32 = 0.f;
-        rec.pad = 0;
+        rec.mult = (int32_t)kmult;
         append_cand(fl != 0, rec, out);
       }
     }
@@ -592,9 +599,12 @@ __host__ __device__ constexpr int scr_rows(int t) {
 // chunks per load group keep a group at 8 KiB of rows per wave.
 __host__ __device__ constexpr int scr_tiles(int Q) { return Q == 4 ? 4 : 2; }
 // Waves per workgroup: 16 (the whole CU's registers at 128 VGPRs) for Q = 4,
-// else 8; one workgroup then owns whole CUs, so a screen launched beside
-// another kernel takes whole CUs and leaves whole CUs.
-__host__ __device__ constexpr int scr_waves(int Q) { return Q == 4 ? 16 : 8; }
+// else 8 (also with the duplicate-row check, which needs more registers); one
+// workgroup then owns whole CUs, so a screen launched beside another kernel
+// takes whole CUs and leaves whole CUs.
+__host__ __device__ constexpr int scr_waves(int Q, bool DD = false) {
+  return (Q == 4 && !DD) ? 16 : 8;
+}
 __host__ __device__ constexpr int scr_group_chunks(int R) { return 8 / R; }
 // LDS: the accession mask (64 B per chunk), then the weights (8 KiB per chunk).
 __host__ __device__ constexpr size_t scr_lds_bytes(int nch) {
@@ -623,20 +633,45 @@ __device__ __forceinline__ void scr_load(const ScreenDesc& D, int64_t w, int C0,
 
 // MFMA steps of one load group into the block's accumulators (and, without a
 // count row, the popcount of the phenotyped bits).
-template <int KG, int NA, int R, int DC, bool CR>
+// Row rotate right by one lane within each row of 16 lanes: lane (g, col)
+// receives lane (g, col - 1 mod 16)'s value.
+__device__ __forceinline__ uint32_t ror16(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x121, 0xF, 0xF, false);
+}
+
+// DD: ne[r] |= bits where this lane's slice of tile r's row differs (under the
+// accession mask) from the previous row's: row kbase + 16 r + col - 1 is lane
+// col - 1 of the same tile, or lane 15 of tile r - 1 when col = 0 (the wave's
+// first row has no previous row here and is never a duplicate).
+template <int KG, int NA, int R, int DC, bool CR, bool DD>
 __device__ __forceinline__ void scr_group(const v4u (&q)[DC][R], int C0, int nch, const v4i* wl,
                                           const v4u* mk_lds, int g, int lane, v4i (&acc)[R][NA],
-                                          uint32_t (&cnt)[R]) {
+                                          uint32_t (&cnt)[R], uint32_t (&ne)[R]) {
 #pragma unroll
   for (int u = 0; u < DC; ++u) {
     if (C0 + u < nch) {
       const int C = C0 + u;
-      if (!CR) {
+      if (!CR || DD) {
         const v4u mk = mk_lds[4 * C + g];
+        if (!CR) {
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-          cnt[r] += __popc(q[u][r].x & mk.x) + __popc(q[u][r].y & mk.y) +
-                    __popc(q[u][r].z & mk.z) + __popc(q[u][r].w & mk.w);
+          for (int r = 0; r < R; ++r)
+            cnt[r] += __popc(q[u][r].x & mk.x) + __popc(q[u][r].y & mk.y) +
+                      __popc(q[u][r].z & mk.z) + __popc(q[u][r].w & mk.w);
+        }
+        if (DD) {
+          const bool first_col = (lane & 15) == 0;
+          v4u rr[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            rr[r] = v4u{ror16(q[u][r].x), ror16(q[u][r].y), ror16(q[u][r].z), ror16(q[u][r].w)};
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const v4u pv = (r > 0 && first_col) ? rr[r > 0 ? r - 1 : 0] : rr[r];
+            ne[r] |= ((q[u][r].x ^ pv.x) & mk.x) | ((q[u][r].y ^ pv.y) & mk.y) |
+                     ((q[u][r].z ^ pv.z) & mk.z) | ((q[u][r].w ^ pv.w) & mk.w);
+          }
+        }
       }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
@@ -657,10 +692,10 @@ __device__ __forceinline__ void scr_group(const v4u (&q)[DC][R], int C0, int nch
 // rows (the same block's next chunks, or the next block's first) are loaded
 // into a second register set while the current group computes, then copied
 // into the current set (the copy is the only wait for them).
-template <int Q, bool CR>
-__global__ __launch_bounds__(64 * scr_waves(Q)) void screen_i8(const ScreenDesc* __restrict__ sd,
-                                                              int nshards, int64_t total_blocks) {
-  constexpr int W = scr_waves(Q);
+template <int Q, bool CR, bool DD>
+__global__ __launch_bounds__(64 * scr_waves(Q, DD)) void screen_i8(
+    const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blocks) {
+  constexpr int W = scr_waves(Q, DD);
   constexpr int NTH = 64 * W;
   constexpr int KG = 16 / Q;   // shifts sharing an accumulator
   constexpr int NA = 8 / KG;   // accumulators per tile
@@ -687,10 +722,11 @@ __global__ __launch_bounds__(64 * scr_waves(Q)) void screen_i8(const ScreenDesc*
   const void* staged = nullptr;
   int nch_staged = 0;
   v4i acc[R][NA];
-  uint32_t cnt[R];
+  uint32_t cnt[R], ne[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     cnt[r] = 0;
+    ne[r] = 0;
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[r][a] = v4i{0, 0, 0, 0};
   }
@@ -731,7 +767,7 @@ __global__ __launch_bounds__(64 * scr_waves(Q)) void screen_i8(const ScreenDesc*
       }
     }
     const v4i* const wl = scr_lds + 4 * nch_staged;
-    scr_group<KG, NA, R, DC, CR>(cq, C0, nch, wl, mk_lds, g, lane, acc, cnt);
+    scr_group<KG, NA, R, DC, CR, DD>(cq, C0, nch, wl, mk_lds, g, lane, acc, cnt, ne);
 
     if (last_group) {
       // exact row totals over the shifts (every row of accumulator a is a
@@ -776,6 +812,27 @@ __global__ __launch_bounds__(64 * scr_waves(Q)) void screen_i8(const ScreenDesc*
         }
       }
       keep = keep && valid && c > 0 && c < n;
+      if (DD) {
+        // dup bit 16 r + col: every lane group's slice of row kbase + 16 r + col
+        // equals the previous row's; rows past m are never duplicates
+        unsigned long long dm = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const unsigned long long b = __ballot(ne[r] != 0u);
+          const uint32_t d16 = (uint32_t)((b | (b >> 16) | (b >> 32) | (b >> 48)) & 0xFFFFull);
+          dm |= (unsigned long long)(~d16 & 0xFFFFu) << (16 * r);
+          ne[r] = 0;
+        }
+        dm &= ~1ull;
+        const int64_t nv = D.m - kbase;
+        if (nv < 16 * R) dm &= nv <= 0 ? 0ull : ((1ull << nv) - 1ull);
+        keep = keep && ((dm >> lane) & 1ull) == 0;
+        if (lane == 0) {
+          auto* dw = gptr_w(D.dmap) + (kbase >> 5);
+          dw[0] = (uint32_t)dm;
+          if (R == 4) dw[1] = (uint32_t)(dm >> 32);
+        }
+      }
       // lane l owns k-mer kbase + l (lanes < 16 R): the ballot is the bitmap
       const unsigned long long bal = __ballot(keep);
       if (lane == 0) {
@@ -892,12 +949,14 @@ __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch
   const int64_t a0 = min(C.w1, C.w0 + wave * per), a1 = min(C.w1, a0 + per);
   const int nw = (int)(a1 - a0);
   const uint32_t* __restrict__ bm = D.bmap;
-  uint32_t wv[8];
+  const bool dd = D.mult != nullptr;  // duplicate-row scan: write run lengths too
+  uint32_t wv[8], dv[8];
   uint32_t v = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int j = 64 * i + lane;
     wv[i] = (j < nw) ? bm[a0 + j] : 0u;
+    dv[i] = (dd && j < nw) ? D.dmap[a0 + j] : 0u;
     v += __popc(wv[i]);
   }
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -917,8 +976,27 @@ __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch
       const uint32_t word = h ? hi : lo;
       const bool set = ((word >> (lane & 31)) & 1u) != 0;
       const unsigned long long bal = __ballot(set);
-      if (set)
-        list[o + __popcll(bal & below)] = (uint32_t)(32 * (a0 + 64 * i + 2 * j + h) + (lane & 31));
+      // every lane reads the dup words (readlane needs the whole wave)
+      const uint32_t dlo = (uint32_t)__builtin_amdgcn_readlane((int)dv[i], 2 * j);
+      const uint32_t dhi = (uint32_t)__builtin_amdgcn_readlane((int)dv[i], 2 * j + 1);
+      if (set) {
+        const int64_t pos = o + __popcll(bal & below);
+        const int64_t wi = a0 + 64 * i + 2 * j + h;
+        list[pos] = (uint32_t)(32 * wi + (lane & 31));
+        if (dd) {
+          // the run of dup bits after this row: to the end of its word, then
+          // into the next (a run never crosses a wave's first row, whose dup
+          // bit is clear, so two words hold it)
+          const uint32_t dword = h ? dhi : dlo;
+          const int b = lane & 31;
+          uint32_t run = b < 31 ? (uint32_t)__builtin_ctz(~(dword >> (b + 1))) : 0u;
+          if (run == (uint32_t)(31 - b) && (wi + 1) * 32 < D.m) {
+            const uint32_t nx = D.dmap[wi + 1];
+            run += nx == 0xFFFFFFFFu ? 32u : (uint32_t)__builtin_ctz(~nx);
+          }
+          D.mult[pos] = 1u + run;
+        }
+      }
       o += __popcll(bal);
     }
   }
@@ -1060,7 +1138,7 @@ __global__ __launch_bounds__(kThreads, 2) void scan_f32(
         rec.trait = tr;
         rec.flags = kPfPass | kSigAmb;
         rec.shard = 0;
-        rec.pad = 0;
+        rec.mult = 1;
         rec.xx32 = xxM;
         rec.xy32 = xyM;
         append_cand(cand, rec, out);
@@ -1271,50 +1349,67 @@ hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t
 
 // Pre-filter screen over a shard table; nch_max = the largest model's 512-sample
 // chunks (sizes the LDS weight table).  grid = persistent workgroups.
-int64_t screen_block_kmers(int t) {
+int64_t screen_block_kmers(int t, bool dd) {
   const int Q = scr_rows(t);
-  return (int64_t)scr_waves(Q) * 16 * scr_tiles(Q);
+  return (int64_t)scr_waves(Q, dd) * 16 * scr_tiles(Q);
 }
 int screen_max_chunks() { return kScrMaxChunks; }
 int screen_rows(int t) { return scr_rows(t); }
 bool screen_count_row(int t) { return scr_count_row(t); }
 
-template <int Q, bool CR>
+template <int Q, bool CR, bool DD>
 static hipError_t launch_screen_q(dim3 grid, size_t shm, hipStream_t s, const ScreenDesc* sd,
                                   int nshards, int64_t total_blocks) {
-  hipLaunchKernelGGL((screen_i8<Q, CR>), grid, dim3(64 * scr_waves(Q)), shm, s, sd, nshards,
-                     total_blocks);
+  hipLaunchKernelGGL((screen_i8<Q, CR, DD>), grid, dim3(64 * scr_waves(Q, DD)), shm, s, sd,
+                     nshards, total_blocks);
   return hipGetLastError();
 }
 
-hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
-                         int nshards, int64_t total_blocks) {
-  if (nch_max > kScrMaxChunks || t < 1 || t > kMaxTraits) return hipErrorInvalidValue;
-  dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
-  const size_t shm = scr_lds_bytes(nch_max);
-  if (!scr_count_row(t)) return launch_screen_q<16, false>(g, shm, s, sd, nshards, total_blocks);
+template <bool DD>
+static hipError_t launch_screen_dd(int t, dim3 g, size_t shm, hipStream_t s, const ScreenDesc* sd,
+                                   int nshards, int64_t total_blocks) {
+  if (!scr_count_row(t))
+    return launch_screen_q<16, false, DD>(g, shm, s, sd, nshards, total_blocks);
   switch (scr_rows(t)) {
-    case 4: return launch_screen_q<4, true>(g, shm, s, sd, nshards, total_blocks);
-    case 8: return launch_screen_q<8, true>(g, shm, s, sd, nshards, total_blocks);
-    default: return launch_screen_q<16, true>(g, shm, s, sd, nshards, total_blocks);
+    case 4: return launch_screen_q<4, true, DD>(g, shm, s, sd, nshards, total_blocks);
+    case 8: return launch_screen_q<8, true, DD>(g, shm, s, sd, nshards, total_blocks);
+    default: return launch_screen_q<16, true, DD>(g, shm, s, sd, nshards, total_blocks);
   }
 }
 
-// Workgroups of the screen one CU holds at once for a model of nch chunks.
-int screen_wgs_per_cu(int t, int nch) {
+// dd: the duplicate-row check (every ScreenDesc then has dmap and mult).
+hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
+                         int nshards, int64_t total_blocks, bool dd) {
+  if (nch_max > kScrMaxChunks || t < 1 || t > kMaxTraits) return hipErrorInvalidValue;
+  dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
+  const size_t shm = scr_lds_bytes(nch_max);
+  return dd ? launch_screen_dd<true>(t, g, shm, s, sd, nshards, total_blocks)
+            : launch_screen_dd<false>(t, g, shm, s, sd, nshards, total_blocks);
+}
+
+template <bool DD>
+static int screen_occ(int t, size_t shm) {
   int nb = 0;
-  const size_t shm = scr_lds_bytes(nch);
   hipError_t e;
   if (!scr_count_row(t))
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, false>, 512, shm);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, false, DD>,
+                                                     64 * scr_waves(16, DD), shm);
   else if (scr_rows(t) == 4)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<4, true>, 1024, shm);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<4, true, DD>,
+                                                     64 * scr_waves(4, DD), shm);
   else if (scr_rows(t) == 8)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<8, true>, 512, shm);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<8, true, DD>,
+                                                     64 * scr_waves(8, DD), shm);
   else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, true>, 512, shm);
-  if (e != hipSuccess || nb < 1) nb = 1;
-  return nb;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, true, DD>,
+                                                     64 * scr_waves(16, DD), shm);
+  return (e != hipSuccess || nb < 1) ? 1 : nb;
+}
+
+// Workgroups of the screen one CU holds at once for a model of nch chunks.
+int screen_wgs_per_cu(int t, int nch, bool dd) {
+  const size_t shm = scr_lds_bytes(nch);
+  return dd ? screen_occ<true>(t, shm) : screen_occ<false>(t, shm);
 }
 
 int screen_chunk_words() { return kScrChunkWords; }
@@ -1380,8 +1475,11 @@ static hipError_t set_attr_nt() {
 hipError_t set_kernel_attributes() {
   hipError_t e = hipSuccess;
   const int scr = (int)scr_lds_bytes(kScrMaxChunks);
-  const void* scr_fns[4] = {(const void*)screen_i8<4, true>, (const void*)screen_i8<8, true>,
-                            (const void*)screen_i8<16, true>, (const void*)screen_i8<16, false>};
+  const void* scr_fns[8] = {
+      (const void*)screen_i8<4, true, false>,  (const void*)screen_i8<8, true, false>,
+      (const void*)screen_i8<16, true, false>, (const void*)screen_i8<16, false, false>,
+      (const void*)screen_i8<4, true, true>,   (const void*)screen_i8<8, true, true>,
+      (const void*)screen_i8<16, true, true>,  (const void*)screen_i8<16, false, true>};
   for (const void* f : scr_fns)
     if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, scr);
 #define KG_ATTR(X) \
