@@ -46,12 +46,12 @@ def main():
                     help="only the mixed scans (for rocprofv3 counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="copy this many bytes device-to-device and exit (counter calibration)")
-    ap.add_argument("--legs", default="c2,c4,c5,pairs,ingest",
+    ap.add_argument("--legs", default="c2,c4,c5,pairs,ingest,dedup",
                     help="further workloads beside the headline: c2 (FP32 kernel, n=1024 p=20, "
                          "50 batches of 1e6), c4 (400 uneven shards), c5 (n=4096 p=64 8 traits, "
                          "streamed from page-locked host memory), pairs (all-pairs epistasis "
                          "scan of 20,000 rows), ingest (ASCII columns -> packed rows on the "
-                         "host); '' for none")
+                         "host), dedup (sorted batches with runs of equal rows); '' for none")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -553,7 +553,51 @@ def leg_ingest(args, synth, ds):
     return out
 
 
-LEGS = {"c2": leg_c2, "c4": leg_c4, "c5": leg_c5, "pairs": leg_pairs, "ingest": leg_ingest}
+def leg_dedup(args, synth, ds):
+    """Duplicate-row scans on sorted batches (src/convert_matrix.cpp:366-368 sorts k-mers by
+    bit pattern, so equal rows are adjacent): 5e6 distinct rows of the n=2048, p=32 workload,
+    each repeated a geometric number of times (mean 2.9), about 1.4e7 rows in HBM; the screened
+    scan with and without the duplicate check, same hits and counts required."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    rng = np.random.default_rng(11)
+    mb = 5_000_000
+    lens = rng.geometric(0.35, mb)
+    rows = np.repeat(ds.fill_bits(0, mb), lens, axis=0)
+    m, rw = rows.shape
+    prm = ScanParams(0.2, 6.0, 15.0)
+    out = {"workload": f"{m} rows = {mb} distinct rows in runs (mean {m / mb:.2f}), n=2048, p=32"}
+    res = {}
+    for dd in (False, True):
+        k = KgwasHip(precision="mixed", screen="on", dedup=dd)
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        dev = k.to_device(rows)
+        k.scan_device(dev.ptr, m, rw, prm)
+        walls, dev_ms = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            hits, npass = k.scan_device(dev.ptr, m, rw, prm)
+            walls.append(time.perf_counter() - t0)
+            tm = k.timing()
+            dev_ms.append(tm["total_ms"])
+        res[dd] = (hits, npass)
+        out["dedup" if dd else "plain"] = {
+            "kmers_per_s": m / min(walls), "device_ms": min(dev_ms),
+            "screen_ms": tm["screen_ms"], "bulk_ms": tm["bulk_ms"],
+            "survivors": int(tm["survivors"]), "hits": int(len(hits)), "passed": int(npass[0])}
+        dev.free()
+        k.close()
+    (h0, p0), (h1, p1) = res[False], res[True]
+    out["identical"] = bool(np.array_equal(p0, p1) and len(h0) == len(h1) and
+                            all(np.array_equal(h0[f], h1[f]) for f in h0.dtype.names))
+    out["speedup"] = out["dedup"]["kmers_per_s"] / out["plain"]["kmers_per_s"]
+    log(f"[bench] dedup: {m} rows, plain {out['plain']['kmers_per_s']:.3e} k-mers/s "
+        f"({out['plain']['survivors']} survivors), dedup {out['dedup']['kmers_per_s']:.3e} "
+        f"({out['dedup']['survivors']} survivors), identical {out['identical']}")
+    return out
+
+
+LEGS = {"c2": leg_c2, "c4": leg_c4, "c5": leg_c5, "pairs": leg_pairs, "ingest": leg_ingest,
+        "dedup": leg_dedup}
 
 
 if __name__ == "__main__":

@@ -55,7 +55,10 @@ def test_bad_options_rejected():
     assert L.kg_create(ctypes.byref(h), ctypes.byref(opts)) == api.KG_ERR_ARG
     opts = api._Opts(0, 0, 3, 5)   # pre-filter screen mode out of range
     assert L.kg_create(ctypes.byref(h), ctypes.byref(opts)) == api.KG_ERR_ARG
+    opts = api._Opts(0, 0, 3, 0, 2)   # duplicate-row flag is 0 or 1
+    assert L.kg_create(ctypes.byref(h), ctypes.byref(opts)) == api.KG_ERR_ARG
     assert L.kg_set_screen(None, 0) == api.KG_ERR_ARG
+    assert L.kg_set_dedup(None, 1) == api.KG_ERR_ARG
 
 
 def test_option_and_timing_layouts():
@@ -63,6 +66,7 @@ def test_option_and_timing_layouts():
     assert ctypes.sizeof(api._Opts) == 32
     assert ctypes.sizeof(api._Timing) == 72
     assert api._Opts.screen.offset == 12 and api._Timing.survivors.offset == 64
+    assert api._Opts.dedup.offset == 16
 
 
 def test_null_context_calls():
