@@ -10,7 +10,8 @@
 // 343-522 (rows).  Additional environment: KG_DEVICE (HIP device, default 0),
 // KG_PRECISION (mixed | fp32, default mixed), KG_BATCH_KMERS (rows-mode batch),
 // KG_PACK_ONLY (cols mode: write the packed rows, uint64 [N][row_words], to
-// this path and stop before the scan).
+// this path and stop before the scan), KG_DEDUP (1: scan each run of equal
+// adjacent rows once, kg_set_dedup; same outputs).
 //
 // Numerics: every reported k-mer's effect/T/p come from FP64 values computed
 // on the GPU and the reference's compute_hit on the host, so cols mode prints
@@ -116,6 +117,11 @@ struct Scanner {
     rc = kg_set_model(ctx, n_acc, ov.idxs.data(), (int32_t)ov.idxs.size(),
                       u.U.empty() ? nullptr : u.U.data(), u.r, ov.y.data(), 1);
     if (rc != KG_OK) die(string("model set-up failed: ") + kg_last_error(ctx));
+    if (const char* dd = std::getenv("KG_DEDUP")) {  // batches sorted by bits: runs of equal rows
+      const string v = dd;
+      if (v != "0" && v != "1") die("KG_DEDUP must be 0 or 1, got: " + v);
+      if (kg_set_dedup(ctx, v == "1" ? 1 : 0) != KG_OK) die(kg_last_error(ctx));
+    }
   }
   ~Scanner() { kg_destroy(ctx); }
   vector<kg_hit> scan(const vector<uint64_t>& rows, int64_t m, int64_t rw,

@@ -238,3 +238,27 @@ def test_packed_rows_bit_exact(small, tmp_path, gz):
     for a in ds.idxs:
         mask[a >> 6] |= np.uint64(1) << np.uint64(a & 63)
     assert np.array_equal(rows, bits & mask)
+
+
+@pytest.mark.gpu
+def test_dedup_outputs_identical(small, tmp_path):
+    """KG_DEDUP=1 on a batch with runs of equal rows (as sorted batches have): the same
+    .tsv / .2d.tsv / .count text as the per-k-mer oracle."""
+    ds, bits, p, root = small
+    rng = np.random.default_rng(3)
+    lens = rng.geometric(0.4, 1200)
+    rows = np.ascontiguousarray(np.repeat(bits[:1200], lens, axis=0))
+    q = batchdir.write_inputs(str(tmp_path / "in"), ds, rows, rows_file=False)
+    out = str(tmp_path / "res.batch_000.tsv")
+    l10, l2d = 2.0, 3.0
+    r = run(base_args(q, out) + [str(l10), "--log10-thresh-2d", str(l2d)],
+            env={"CORR_THRESHOLD": "0.15", "KG_DEDUP": "1"})
+    assert r.returncode == 0, r.stderr
+    ref, t1, t2 = _expected(ds, rows, q["names"], 0.15, l10, l2d)
+    print(f"[cli] dedup rows={len(rows)} hits={t1.count(chr(10))} passed={ref['n_passed']}",
+          flush=True)
+    assert open(out).read() == t1
+    assert open(str(tmp_path / "res.batch_000.2d.tsv")).read() == t2
+    assert open(out + ".count").read() == f"res.batch_000\t{ref['n_passed']}\n"
+    r = run(base_args(q, out), env={"KG_DEDUP": "yes"})
+    assert r.returncode == 1 and "KG_DEDUP must be 0 or 1" in r.stderr
