@@ -956,7 +956,8 @@ __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch
   for (int i = 0; i < 8; ++i) {
     const int j = 64 * i + lane;
     wv[i] = (j < nw) ? bm[a0 + j] : 0u;
-    dv[i] = (dd && j < nw) ? D.dmap[a0 + j] : 0u;
+    dv[i] = 0u;
+    if (dd && j < nw) dv[i] = D.dmap[a0 + j];
     v += __popc(wv[i]);
   }
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -976,9 +977,12 @@ __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch
       const uint32_t word = h ? hi : lo;
       const bool set = ((word >> (lane & 31)) & 1u) != 0;
       const unsigned long long bal = __ballot(set);
-      // every lane reads the dup words (readlane needs the whole wave)
-      const uint32_t dlo = (uint32_t)__builtin_amdgcn_readlane((int)dv[i], 2 * j);
-      const uint32_t dhi = (uint32_t)__builtin_amdgcn_readlane((int)dv[i], 2 * j + 1);
+      // the dup words, by every lane of the wave (dd is uniform)
+      uint32_t dlo = 0, dhi = 0;
+      if (dd) {
+        dlo = (uint32_t)__builtin_amdgcn_readlane((int)dv[i], 2 * j);
+        dhi = (uint32_t)__builtin_amdgcn_readlane((int)dv[i], 2 * j + 1);
+      }
       if (set) {
         const int64_t pos = o + __popcll(bal & below);
         const int64_t wi = a0 + 64 * i + 2 * j + h;
