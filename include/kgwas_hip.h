@@ -188,8 +188,9 @@ int kg_set_screen(kg_ctx* ctx, int32_t mode);
  * reports every member of a reported run (same values, consecutive idx).
  * Results are identical either way; it needs the screen (corr_threshold > 0,
  * screen mode not KG_SCREEN_OFF) and keeps it on in KG_SCREEN_AUTO.  Runs are
- * detected within each 64 (or 32) k-mer wave tile of the screen.  No reference
- * counterpart (the reference computes every k-mer). */
+ * detected within each 64 (or 32) k-mer wave tile of the screen (and within a
+ * kg_scan chunk).  kg_scan_pairs does not use it.  No reference counterpart
+ * (the reference computes every k-mer). */
 int kg_set_dedup(kg_ctx* ctx, int32_t on);
 
 /* Launch-shape autotuner (no reference counterpart; the reference has one
