@@ -27,7 +27,7 @@ namespace kg {
 hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t s,
                           const ShardDesc* sd, int nshards, int64_t total_blocks,
                           const ScanOut& o, const int64_t* total_dev);
-int64_t screen_block_kmers(int t, bool dd);
+int64_t screen_block_kmers(int t);
 int screen_rows(int t);
 bool screen_count_row(int t);
 int screen_max_chunks();
@@ -987,7 +987,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   ScreenRun sr;
   const bool dd = screen && ctx->dedup != 0;
   if (screen) {
-    const int64_t sbk = screen_block_kmers(t, dd);
+    const int64_t sbk = screen_block_kmers(t);
     const int64_t cw = screen_chunk_words();
     int64_t bm_words = 0;
     for (int s = 0; s < nsh; ++s) bm_words += (sh[s].m + sbk - 1) / sbk * (sbk / 32);

@@ -599,12 +599,9 @@ __host__ __device__ constexpr int scr_rows(int t) {
 // chunks per load group keep a group at 8 KiB of rows per wave.
 __host__ __device__ constexpr int scr_tiles(int Q) { return Q == 4 ? 4 : 2; }
 // Waves per workgroup: 16 (the whole CU's registers at 128 VGPRs) for Q = 4,
-// else 8 (also with the duplicate-row check, which needs more registers); one
-// workgroup then owns whole CUs, so a screen launched beside another kernel
-// takes whole CUs and leaves whole CUs.
-__host__ __device__ constexpr int scr_waves(int Q, bool DD = false) {
-  return (Q == 4 && !DD) ? 16 : 8;
-}
+// else 8; one workgroup then owns whole CUs, so a screen launched beside
+// another kernel takes whole CUs and leaves whole CUs.
+__host__ __device__ constexpr int scr_waves(int Q) { return Q == 4 ? 16 : 8; }
 __host__ __device__ constexpr int scr_group_chunks(int R) { return 8 / R; }
 // LDS: the accession mask (64 B per chunk), then the weights (8 KiB per chunk).
 __host__ __device__ constexpr size_t scr_lds_bytes(int nch) {
@@ -660,16 +657,17 @@ __device__ __forceinline__ void scr_group(const v4u (&q)[DC][R], int C0, int nch
                       __popc(q[u][r].z & mk.z) + __popc(q[u][r].w & mk.w);
         }
         if (DD) {
+          // tile by tile, so that only the previous tile's rotation stays live
           const bool first_col = (lane & 15) == 0;
-          v4u rr[R];
-#pragma unroll
-          for (int r = 0; r < R; ++r)
-            rr[r] = v4u{ror16(q[u][r].x), ror16(q[u][r].y), ror16(q[u][r].z), ror16(q[u][r].w)};
+          v4u prev = v4u{0u, 0u, 0u, 0u};
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const v4u pv = (r > 0 && first_col) ? rr[r > 0 ? r - 1 : 0] : rr[r];
+            const v4u rr = v4u{ror16(q[u][r].x), ror16(q[u][r].y), ror16(q[u][r].z),
+                               ror16(q[u][r].w)};
+            const v4u pv = (r > 0 && first_col) ? prev : rr;
             ne[r] |= ((q[u][r].x ^ pv.x) & mk.x) | ((q[u][r].y ^ pv.y) & mk.y) |
                      ((q[u][r].z ^ pv.z) & mk.z) | ((q[u][r].w ^ pv.w) & mk.w);
+            prev = rr;
           }
         }
       }
@@ -693,9 +691,9 @@ __device__ __forceinline__ void scr_group(const v4u (&q)[DC][R], int C0, int nch
 // into a second register set while the current group computes, then copied
 // into the current set (the copy is the only wait for them).
 template <int Q, bool CR, bool DD>
-__global__ __launch_bounds__(64 * scr_waves(Q, DD)) void screen_i8(
+__global__ __launch_bounds__(64 * scr_waves(Q)) void screen_i8(
     const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blocks) {
-  constexpr int W = scr_waves(Q, DD);
+  constexpr int W = scr_waves(Q);
   constexpr int NTH = 64 * W;
   constexpr int KG = 16 / Q;   // shifts sharing an accumulator
   constexpr int NA = 8 / KG;   // accumulators per tile
@@ -1353,9 +1351,9 @@ hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t
 
 // Pre-filter screen over a shard table; nch_max = the largest model's 512-sample
 // chunks (sizes the LDS weight table).  grid = persistent workgroups.
-int64_t screen_block_kmers(int t, bool dd) {
+int64_t screen_block_kmers(int t) {
   const int Q = scr_rows(t);
-  return (int64_t)scr_waves(Q, dd) * 16 * scr_tiles(Q);
+  return (int64_t)scr_waves(Q) * 16 * scr_tiles(Q);
 }
 int screen_max_chunks() { return kScrMaxChunks; }
 int screen_rows(int t) { return scr_rows(t); }
@@ -1364,7 +1362,7 @@ bool screen_count_row(int t) { return scr_count_row(t); }
 template <int Q, bool CR, bool DD>
 static hipError_t launch_screen_q(dim3 grid, size_t shm, hipStream_t s, const ScreenDesc* sd,
                                   int nshards, int64_t total_blocks) {
-  hipLaunchKernelGGL((screen_i8<Q, CR, DD>), grid, dim3(64 * scr_waves(Q, DD)), shm, s, sd,
+  hipLaunchKernelGGL((screen_i8<Q, CR, DD>), grid, dim3(64 * scr_waves(Q)), shm, s, sd,
                      nshards, total_blocks);
   return hipGetLastError();
 }
@@ -1397,16 +1395,16 @@ static int screen_occ(int t, size_t shm) {
   hipError_t e;
   if (!scr_count_row(t))
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, false, DD>,
-                                                     64 * scr_waves(16, DD), shm);
+                                                     64 * scr_waves(16), shm);
   else if (scr_rows(t) == 4)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<4, true, DD>,
-                                                     64 * scr_waves(4, DD), shm);
+                                                     64 * scr_waves(4), shm);
   else if (scr_rows(t) == 8)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<8, true, DD>,
-                                                     64 * scr_waves(8, DD), shm);
+                                                     64 * scr_waves(8), shm);
   else
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, true, DD>,
-                                                     64 * scr_waves(16, DD), shm);
+                                                     64 * scr_waves(16), shm);
   return (e != hipSuccess || nb < 1) ? 1 : nb;
 }
 
