@@ -53,6 +53,8 @@ def main():
                          "scan of 20,000 rows), ingest (ASCII columns -> packed rows on the "
                          "host), dedup (sorted batches with runs of equal rows); '' for none")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--screen", default="auto", choices=("auto", "on", "off"),
+                    help="pre-filter screen of the headline scans (auto: screen while it pays)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -82,7 +84,7 @@ def main():
     log(f"[bench] dataset n={n} p={p} rw={rw} ({time.time() - t0:.1f}s)")
     prm = ScanParams(0.2, 6.0, 15.0)
 
-    mixed = KgwasHip(precision="mixed", limbs=args.limbs)
+    mixed = KgwasHip(precision="mixed", limbs=args.limbs, screen=args.screen)
     dev = mixed.alloc(args.m * rw * 8)
     mixed.set_model(n_acc, ds.idxs, ds.U, ds.y)
     t1 = time.time()

@@ -46,8 +46,12 @@ hipError_t launch_scan_f32(int NT, int nch, int grid, hipStream_t s, const uint6
                            int64_t m, int64_t rw, int64_t row0, const float* wts,
                            const uint64_t* mask, const EpiF32& ep, const ScanOut& o);
 hipError_t launch_refine(hipStream_t s, const Cand* cand, const unsigned long long* count,
-                         int64_t cap, const RefDesc* rd, double* res, int blocks);
+                         int64_t cap, const RefDesc* rd, double* res, int blocks,
+                         const RefHost& hb);
 hipError_t launch_zero(hipStream_t s, unsigned long long* p, int n);
+size_t setup_max_bytes();
+hipError_t launch_setup(hipStream_t s, const void* src, size_t bytes, void* dst,
+                        unsigned long long* zero, int nzero);
 hipError_t launch_pair_masks(hipStream_t s, const uint64_t* bits, int64_t rw,
                             const uint64_t* mask, int64_t W, int64_t i0, int64_t cnt,
                             uint64_t* out);
@@ -85,9 +89,11 @@ struct DBuf {
   }
 };
 
-// Page-locked host buffer (bounce buffer for pageable rows).
+// Page-locked host buffer (bounce buffer for pageable rows; with
+// hipHostMallocCoherent, a buffer kernels write into directly through `dev`).
 struct HBuf {
   void* p = nullptr;
+  void* dev = nullptr;  // device-visible address of p
   size_t bytes = 0;
   HBuf() = default;
   HBuf(const HBuf&) = delete;
@@ -95,14 +101,24 @@ struct HBuf {
   ~HBuf() {
     if (p) (void)hipHostFree(p);
   }
-  hipError_t ensure(size_t b) {
+  hipError_t ensure(size_t b, unsigned flags = hipHostMallocDefault) {
     if (b <= bytes && p) return hipSuccess;
     if (p) (void)hipHostFree(p);
-    p = nullptr;
+    p = dev = nullptr;
     bytes = 0;
-    hipError_t e = hipHostMalloc(&p, b ? b : 16, hipHostMallocDefault);
-    if (e == hipSuccess) bytes = b;
+    hipError_t e = hipHostMalloc(&p, b ? b : 16, flags);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&dev, p, 0);
+    if (e == hipSuccess) {
+      bytes = b;
+    } else if (p) {
+      (void)hipHostFree(p);
+      p = dev = nullptr;
+    }
     return e;
+  }
+  template <class T>
+  T* dev_as() const {
+    return reinterpret_cast<T*>(dev);
   }
 };
 
@@ -184,8 +200,9 @@ struct kg_ctx {
   // device copy (one asynchronous upload per scan)
   HBuf hdesc;
   DBuf ddesc;
-  HBuf hcnt;  // page-locked counter read-back
-  HBuf hcand, hres;  // page-locked read-back of the first kEagerCands candidates
+  // page-locked read-back, written by refine_f64: the counters, and the first
+  // kEagerCands candidates with their FP64 results
+  HBuf hcnt, hcand, hres;
   hipEvent_t cev[2] = {nullptr, nullptr};
   hipEvent_t ctev[4] = {nullptr, nullptr, nullptr, nullptr};
   HBuf bounce[2];
@@ -852,15 +869,15 @@ struct ScreenRun {
   bool dd = false;              // with the duplicate-row check
 };
 
-// Device part of a batched int8 scan: zero counters, [screen, fix-up,] bulk
-// kernel(s), refine.
+// Device part of a batched int8 scan: zero counters (unless the table upload
+// did), [screen, fix-up,] bulk kernel(s), refine (with the host read-back).
 static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks, bool per_launch,
                       const std::vector<int64_t>& nblk, const ScanOut& o, int ref_blocks,
                       const ShardDesc* d_sd, const ShardDesc* d_sd1, const RefDesc* d_rd,
-                      const ScreenRun& sr) {
+                      const ScreenRun& sr, bool zeroed, const RefHost& hb) {
   const int L = ctx->limbs;
   const int grid = ctx->cus * ctx->grid_mult;
-  KG_CK(ctx, launch_zero(ctx->stream, o.counters, (int)ctx->counters_len));
+  if (!zeroed) KG_CK(ctx, launch_zero(ctx->stream, o.counters, (int)ctx->counters_len));
   KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (sr.on) {
     KG_CK(ctx, launch_screen(t, sr.nch_max, sr.sgrid, ctx->stream, sr.d_ss, nsh, sr.sblocks,
@@ -883,7 +900,7 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
   }
   KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
   KG_CK(ctx, launch_refine(ctx->stream, o.cand, o.counters, o.cap, d_rd,
-                           ctx->res.as<double>(), ref_blocks));
+                           ctx->res.as<double>(), ref_blocks, hb));
   KG_CK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   return KG_OK;
 }
@@ -1090,13 +1107,25 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     o.st_S = ctx->st_S.as<double>();
   }
   const bool use_graph = (flags & kGraph) != 0 && !stats;
-  const int ref_blocks = ctx->cus * 4;
+  const int ref_blocks = ctx->cus * 2;
   std::vector<unsigned long long> cnt(ctx->counters_len);
-  KG_CK(ctx, ctx->hcnt.ensure(sizeof(unsigned long long) * cnt.size()));
+  KG_CK(ctx, ctx->hcnt.ensure(sizeof(unsigned long long) * cnt.size(), hipHostMallocCoherent));
   for (int attempt = 0; attempt < 3; ++attempt) {
     o.cand = ctx->cand.as<Cand>();
     o.counters = dc;
     o.cap = ctx->cand_cap;
+    // the counters and the first candidates come back through page-locked
+    // memory that refine_f64 writes (one synchronisation for the usual scan;
+    // more candidates are copied after it)
+    const int64_t eager = std::min<int64_t>(ctx->cand_cap, kEagerCands);
+    KG_CK(ctx, ctx->hcand.ensure(sizeof(Cand) * eager, hipHostMallocCoherent));
+    KG_CK(ctx, ctx->hres.ensure(sizeof(double) * 4 * eager, hipHostMallocCoherent));
+    RefHost hb{};
+    hb.cand = ctx->hcand.dev_as<Cand>();
+    hb.res = ctx->hres.dev_as<double>();
+    hb.cnt = ctx->hcnt.dev_as<unsigned long long>();
+    hb.eager = eager;
+    hb.ncnt = (int)cnt.size();
     // graph key: everything the captured launches depend on
     std::vector<char> key;
     if (use_graph) {
@@ -1129,11 +1158,14 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       kp += sizeof(o);
       const int64_t extra[2] = {(int64_t)flags, (int64_t)ctx->grid_mult * 64 + ctx->tiles};
       memcpy(kp, extra, 16);
+      key.insert(key.end(), reinterpret_cast<const char*>(&hb),
+                 reinterpret_cast<const char*>(&hb) + sizeof(hb));
     }
     const bool replay = use_graph && ctx->gexec && key == ctx->graph_key;
     const ShardDesc* d_sd = ctx->sdesc.as<ShardDesc>();
     const ShardDesc* d_sd1 = ctx->sdesc1.as<ShardDesc>();
     const RefDesc* d_rd = ctx->rdesc.as<RefDesc>();
+    bool zeroed = false;  // counters zeroed by the table upload
     if (use_graph) {
       if (!replay) {
         KG_CK(ctx, hipMemcpy(ctx->sdesc.p, sd.data(), sizeof(ShardDesc) * nsh, hipMemcpyHostToDevice));
@@ -1161,8 +1193,10 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     } else {
       // descriptor tables: packed into one page-locked staging buffer and
       // copied with one asynchronous copy on the scan's stream (the previous
-      // scan ended with a synchronisation, so both buffers are free)
-      auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+      // scan ended with a synchronisation, so both buffers are free); tables
+      // that fit a kernel argument (a single shard's) go up by value in the
+      // launch that also zeroes the counters
+      auto al = [](size_t b) { return (b + 15) & ~(size_t)15; };
       const size_t b0 = sizeof(ShardDesc) * nsh, b2 = sizeof(RefDesc) * nsh;
       const size_t b3 = screen ? sizeof(ScreenDesc) * nsh : 0, b4 = screen ? b0 : 0;
       const size_t b5 = screen ? sizeof(ScrChunk) * chunks.size() : 0;
@@ -1179,7 +1213,12 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
         memcpy(hp + o4, sd2.data(), b4);
         if (b5) memcpy(hp + o5, chunks.data(), b5);
       }
-      KG_CK(ctx, hipMemcpyAsync(ctx->ddesc.p, hp, tot, hipMemcpyHostToDevice, ctx->stream));
+      if (tot <= setup_max_bytes() && ctx->counters_len <= (size_t)INT32_MAX) {
+        KG_CK(ctx, launch_setup(ctx->stream, hp, tot, ctx->ddesc.p, dc, (int)ctx->counters_len));
+        zeroed = true;
+      } else {
+        KG_CK(ctx, hipMemcpyAsync(ctx->ddesc.p, hp, tot, hipMemcpyHostToDevice, ctx->stream));
+      }
       char* dp = static_cast<char*>(ctx->ddesc.p);
       d_sd = reinterpret_cast<const ShardDesc*>(dp);
       d_sd1 = reinterpret_cast<const ShardDesc*>(dp + o1);
@@ -1196,7 +1235,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
         drop_graph(ctx);
         KG_CK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_i8(ctx, NT, t, nsh, total_blocks, per_launch, nblk, o, ref_blocks, d_sd,
-                            d_sd1, d_rd, sr);
+                            d_sd1, d_rd, sr, false, hb);
         hipGraph_t g = nullptr;
         hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
         if (rc != KG_OK) {
@@ -1212,20 +1251,9 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       KG_CK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     } else {
       int rc = enqueue_i8(ctx, NT, t, nsh, total_blocks, per_launch, nblk, o, ref_blocks, d_sd,
-                          d_sd1, d_rd, sr);
+                          d_sd1, d_rd, sr, zeroed, hb);
       if (rc != KG_OK) return rc;
     }
-    // the first candidates come back with the counters (one synchronisation
-    // for the usual scan; more candidates are copied after it)
-    const int64_t eager = std::min<int64_t>(ctx->cand_cap, kEagerCands);
-    KG_CK(ctx, ctx->hcand.ensure(sizeof(Cand) * eager));
-    KG_CK(ctx, ctx->hres.ensure(sizeof(double) * 4 * eager));
-    KG_CK(ctx, hipMemcpyAsync(ctx->hcand.p, o.cand, sizeof(Cand) * eager, hipMemcpyDeviceToHost,
-                              ctx->stream));
-    KG_CK(ctx, hipMemcpyAsync(ctx->hres.p, ctx->res.p, sizeof(double) * 4 * eager,
-                              hipMemcpyDeviceToHost, ctx->stream));
-    KG_CK(ctx, hipMemcpyAsync(ctx->hcnt.p, dc, sizeof(unsigned long long) * cnt.size(),
-                              hipMemcpyDeviceToHost, ctx->stream));
     if (overlap && attempt == 0) {
       const int orc = (*overlap)();
       if (orc != KG_OK) {

@@ -139,6 +139,17 @@ struct RefDesc {
   int nwords, p, t, n;
 };
 
+// Read-back of a scan written by refine_f64 straight into page-locked host
+// memory (device-visible pointers): the first `eager` candidate records with
+// their FP64 results, and the `ncnt` counters.  Null pointers: nothing.
+struct RefHost {
+  Cand* cand;               // [eager]
+  double* res;              // [4 * eager]
+  unsigned long long* cnt;  // [ncnt]
+  int64_t eager;
+  int ncnt, pad;
+};
+
 // Epilogue constants of the plain FP32 kernel.
 struct EpiF32 {
   int n, p, t, NU;

@@ -19,6 +19,8 @@
 // contraction is the plain sum over accessions.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "kg_internal.h"
 #include "synth_core.h"
 
@@ -40,7 +42,8 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gptr_w(T* p) {
   return (__attribute__((address_space(1))) T*)p;
 }
 
-constexpr int kWaves = 4;        // waves per workgroup (FP32 scan, refinement)
+constexpr int kWaves = 4;        // waves per workgroup (FP32 scan)
+constexpr int kRefWaves = 16;    // waves per candidate (refine_f64)
 constexpr int kThreads = 64 * kWaves;
 // The int8 scan runs one 8-wave workgroup per CU (two waves per SIMD): two
 // 4-wave workgroups per CU measured ~40% slower on the MFMA pipe.
@@ -1154,21 +1157,28 @@ __global__ __launch_bounds__(kThreads, 2) void scan_f32(
 }
 
 // ----------------------------------------------------------------------------
-// Exact FP64 statistic of each candidate.  One workgroup per candidate: wave
-// v sums the columns over the set, phenotyped bits of the v-th quarter of the
-// row's words in accession order (lane b owns base column b and b + 64), and
-// wave 0 adds the quarters in order, so the result is deterministic.  The
+// Exact FP64 statistic of each candidate.  One workgroup of kRefWaves waves
+// per candidate: wave v sums the columns over the set, phenotyped bits of the
+// v-th part of the row's words in accession order (lane b owns base column b
+// and b + 64), and wave 0 adds the parts in order, so the result is
+// deterministic.  A candidate's time is a chain of dependent column loads, 8
+// set bits per round, so more waves per candidate mean fewer rounds.  The
 // candidate count is read on the device (so the launch can sit in a captured
 // graph).  Columns of RefDesc::B64: [0,p) rotated centred U, [p, p+t) centred
-// residuals.  res[i] = {c, xxM, xyM, S}.
-__global__ __launch_bounds__(kThreads) void refine_f64(const Cand* __restrict__ cand,
-                                                       const unsigned long long* __restrict__ count,
-                                                       int64_t cap,
-                                                       const RefDesc* __restrict__ rd,
-                                                       double* __restrict__ res) {
-  __shared__ double pa[kWaves][2][64];
-  __shared__ int pc[kWaves];
+// residuals.  res[i] = {c, xxM, xyM, S}.  The first hb.eager candidates and
+// results, and the counters, are also written to hb's page-locked host
+// buffers, which the host reads after its one synchronisation.
+__global__ __launch_bounds__(64 * kRefWaves) void refine_f64(const Cand* __restrict__ cand,
+                                                             const unsigned long long* __restrict__ count,
+                                                             int64_t cap,
+                                                             const RefDesc* __restrict__ rd,
+                                                             double* __restrict__ res, RefHost hb) {
+  __shared__ double pa[kRefWaves][2][64];
+  __shared__ int pc[kRefWaves];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the counters are final: the kernels that count ran before this one
+  if (blockIdx.x == 0 && hb.cnt)
+    for (int i = threadIdx.x; i < hb.ncnt; i += blockDim.x) hb.cnt[i] = count[i];
   int64_t ncand = (int64_t)count[0];
   if (ncand > cap) ncand = cap;
   for (int64_t i = blockIdx.x; i < ncand; i += gridDim.x) {
@@ -1176,7 +1186,7 @@ __global__ __launch_bounds__(kThreads) void refine_f64(const Cand* __restrict__ 
     const RefDesc& R = rd[cd.shard];
     const uint64_t* row = R.bits + cd.idx * R.rw;
     const int nbf = R.p + R.t, p = R.p;
-    const int per = (R.nwords + kWaves - 1) / kWaves;
+    const int per = (R.nwords + kRefWaves - 1) / kRefWaves;
     const int wb = min(R.nwords, wave * per), we = min(R.nwords, wb + per);
     double a0 = 0.0, a1 = 0.0;
     int c = 0;
@@ -1221,7 +1231,7 @@ __global__ __launch_bounds__(kThreads) void refine_f64(const Cand* __restrict__ 
       a0 = pa[0][0][lane];
       a1 = pa[0][1][lane];
       c = pc[0];
-      for (int v = 1; v < kWaves; ++v) {
+      for (int v = 1; v < kRefWaves; ++v) {
         a0 += pa[v][0][lane];
         a1 += pa[v][1][lane];
         c += pc[v];
@@ -1248,10 +1258,12 @@ __global__ __launch_bounds__(kThreads) void refine_f64(const Cand* __restrict__ 
       if (lane == 0) {
         const double cdv = (double)c, cn = cdv / (double)R.n;
         const double xxM = cdv - cdv * cdv * R.base_coef - 2.0 * cn * R.sigma * z0 - zz;
-        res[4 * i + 0] = cdv;
-        res[4 * i + 1] = xxM;
-        res[4 * i + 2] = sv + cdv * R.rbar[tr];
-        res[4 * i + 3] = sv + zb;
+        const double r4[4] = {cdv, xxM, sv + cdv * R.rbar[tr], sv + zb};
+        for (int k = 0; k < 4; ++k) res[4 * i + k] = r4[k];
+        if (i < hb.eager) {
+          for (int k = 0; k < 4; ++k) hb.res[4 * i + k] = r4[k];
+          hb.cand[i] = cd;
+        }
       }
     }
     __syncthreads();  // the partials are rewritten for the next candidate
@@ -1457,9 +1469,44 @@ hipError_t launch_scan_f32(int NT, int nch, int grid, hipStream_t s, const uint6
 }
 
 hipError_t launch_refine(hipStream_t s, const Cand* cand, const unsigned long long* count,
-                         int64_t cap, const RefDesc* rd, double* res, int blocks) {
-  hipLaunchKernelGGL(refine_f64, dim3((unsigned)blocks), dim3(kThreads), 0, s, cand, count, cap,
-                     rd, res);
+                         int64_t cap, const RefDesc* rd, double* res, int blocks,
+                         const RefHost& hb) {
+  hipLaunchKernelGGL(refine_f64, dim3((unsigned)blocks), dim3(64 * kRefWaves), 0, s, cand, count,
+                     cap, rd, res, hb);
+  return hipGetLastError();
+}
+
+// Descriptor tables passed by value: one small launch writes them to device
+// memory and zeroes the scan's counters (in place of a host-to-device copy and
+// a separate zeroing launch).
+constexpr int kSetupWords = 240;
+struct SetupArgs {
+  uint64_t* dst;
+  unsigned long long* zero;
+  int nwords, nzero;
+  uint64_t w[kSetupWords];
+};
+
+__global__ void setup_tables(SetupArgs a) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
+  for (int i = i0; i < a.nwords; i += st) a.dst[i] = a.w[i];
+  for (int i = i0; i < a.nzero; i += st) a.zero[i] = 0ull;
+}
+
+size_t setup_max_bytes() { return sizeof(uint64_t) * kSetupWords; }
+
+hipError_t launch_setup(hipStream_t s, const void* src, size_t bytes, void* dst,
+                        unsigned long long* zero, int nzero) {
+  if (bytes > setup_max_bytes() || bytes % 8 != 0 || nzero < 0) return hipErrorInvalidValue;
+  SetupArgs a{};
+  a.dst = static_cast<uint64_t*>(dst);
+  a.zero = zero;
+  a.nwords = (int)(bytes / 8);
+  a.nzero = nzero;
+  if (bytes) memcpy(a.w, src, bytes);
+  const int work = a.nwords > nzero ? a.nwords : nzero;
+  const int blocks = work <= 256 ? 1 : (work + 255) / 256 < 64 ? (work + 255) / 256 : 64;
+  hipLaunchKernelGGL(setup_tables, dim3((unsigned)blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -177,6 +177,36 @@ def test_edge_cases(built):
         assert npass[0] == 0 and len(h) == 0
 
 
+def test_many_candidates_read_back(built):
+    """tau = 0 and p <= 1 make every polymorphic k-mer a reported candidate.  90,000 of them are
+    more than the 4,096 returned in place through page-locked memory (the rest are copied after
+    the synchronisation), and more than the initial candidate capacity of 65,536 (the scan then
+    reruns with a larger one).  Every k-mer comes back with the oracle's count and FP64
+    statistic, and a later, smaller scan on the grown buffers reports the same values."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    ds = dataset("small_n190_p5", **CASES["small_n190_p5"][0])
+    m = 90000
+    bits = ds.fill_bits(0, m)
+    ref = oracle.scan_f64(bits, ds.idxs, ds.y, ds.U, 0.0)
+    poly = (ref["c"] > 0) & (ref["c"] < ds.n)
+    print(f"[case] many candidates m={m} polymorphic={int(poly.sum())}", flush=True)
+    with KgwasHip() as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        h, npass = k.scan(bits, ScanParams(0.0, 0.0, 0.0))
+        tim = k.timing()
+        h2, _ = k.scan(bits[:5000], ScanParams(0.0, 0.0, 0.0))
+    print(f"[case] many candidates: candidates={tim['candidates']} hits={len(h)}", flush=True)
+    assert tim["candidates"] == poly.sum() > 65536
+    assert npass[0] == poly.sum()
+    assert np.array_equal(h["idx"], np.nonzero(poly)[0])
+    assert np.array_equal(h["c"], ref["c"][poly])
+    dT = np.abs(h["T"] - ref["T"][poly])
+    assert np.all(dT <= REPORT_TOL * ref["T"][poly] + 1e-10), dT.max()
+    first = h["idx"] < 5000
+    assert np.array_equal(h2["idx"], h["idx"][first])
+    assert np.array_equal(h2["T"], h["T"][first])
+
+
 def test_multi_trait_equals_single_trait_runs(built):
     from supergwas_amd.api import KgwasHip, ScanParams
     ds = synth.make_dataset(seed=7, n_acc=512, n_missing=20, npop=4, p=8, m=4000, n_causal=10,
