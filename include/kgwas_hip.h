@@ -197,7 +197,8 @@ int kg_set_dedup(kg_ctx* ctx, int32_t on);
  * code path): times the int8 bulk scan of `sample` random rows with model 0
  * for each candidate k-mer tiles per wave (k-mers per workgroup) and
  * persistent workgroups per CU, keeps the fastest for later scans on ctx and
- * returns it.  A no-op for KG_PREC_FP32. */
+ * returns it.  The default shape is timed first and is replaced only by one
+ * that is more than 2% faster.  A no-op for KG_PREC_FP32. */
 int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t* grid_mult);
 
 /* Device memory on the context's device for callers without their own

@@ -1757,7 +1757,9 @@ extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t*
           }
           if (rep > 0) tmin = std::min(tmin, ctx->timing.bulk_ms);
         }
-        if (tmin < best) {
+        // a shape replaces the earlier (default first) one only when it is
+        // faster by more than the run-to-run spread of these short scans
+        if (tmin < best * 0.98) {
           best = tmin;
           best_R = R;
           best_gm = gm;
