@@ -220,6 +220,13 @@ struct kg_ctx {
   // scans left before the screen is tried again
   int screen_mode = KG_SCREEN_AUTO;
   int screen_skip = 0;
+  // auto mode: device time of the last unscreened scan that could have been
+  // screened, and its k-mer count; scans of up to kProbeKmers k-mers measure
+  // it now and then (fixed costs decide there, so scaling the survivor scan
+  // to all k-mers misjudges them)
+  double full_ms = 0.0;
+  int64_t full_m = 0;
+  int probe_tick = 0;
   int dedup = 0;  // screened scans check each row against the previous one
   DBuf surv, ssdesc, sdesc2, chdesc;  // survivor rows; graph-path screen tables
   DBuf bmap, scr_aux;                 // screen bitmaps; chunk counts and prefixes
@@ -779,6 +786,8 @@ extern "C" int kg_set_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int
   drop_graph(ctx);
   ctx->models.clear();
   ctx->screen_skip = 0;
+  ctx->full_m = 0;
+  ctx->probe_tick = 0;
   return add_model(ctx, n_acc, idxs, n, U, p, Y, t, nullptr);
 }
 
@@ -907,6 +916,7 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
 
 enum { kPerLaunch = 1, kGraph = 2, kPairs = 4 };
 constexpr int64_t kEagerCands = 4096;  // candidates read back with the counters
+constexpr int64_t kProbeKmers = int64_t(1) << 22;  // auto screen: scans that measure unscreened
 
 // Scans every shard (int8 path), appending hits (idx offset by shard idx0,
 // shard index set) and adding pre-filter passes to n_passed[s * t + tr].
@@ -947,9 +957,20 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     nch_max = std::max(nch_max, M.md.nch_i8);
     mtot += s.m;
   }
-  if (screen && ctx->screen_mode == KG_SCREEN_AUTO && ctx->screen_skip > 0 && !ctx->dedup) {
-    --ctx->screen_skip;
-    screen = false;
+  // auto mode: unscreened while a measurement said the screen does not pay;
+  // small scans also run unscreened once per 32 (or at a new size) to measure
+  // the full kernel directly
+  const bool auto_scr = screen && ctx->screen_mode == KG_SCREEN_AUTO && !ctx->dedup &&
+                        (flags & kGraph) == 0;
+  if (auto_scr) {
+    const bool same_size = ctx->full_m > 0 && mtot <= 2 * ctx->full_m && 2 * mtot >= ctx->full_m;
+    if (ctx->screen_skip > 0) {
+      --ctx->screen_skip;
+      screen = false;
+    } else if (mtot <= kProbeKmers && (!same_size || ++ctx->probe_tick >= 32)) {
+      ctx->probe_tick = 0;
+      screen = false;
+    }
   }
   const size_t base_surv = 1 + (size_t)nsh * t;  // counters: [cand][passes][survivors][total]
   ctx->counters_len = base_surv + (screen ? (size_t)nsh + 1 : 0);
@@ -1300,12 +1321,18 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     for (int s = 0; s < nsh; ++s) ns += (int64_t)cnt[base_surv + s];
     ctx->timing.survivors = ns;
     // auto: when the screen plus the survivor scan took longer than the full
-    // kernel would have over every k-mer (its per-survivor time scaled to all
-    // k-mers), skip the screen for the next 8 scans, then measure again
-    if (ctx->screen_mode == KG_SCREEN_AUTO && !use_graph && ns > 0 && !ctx->dedup &&
-        ctx->timing.screen_ms + ctx->timing.bulk_ms >
-            ctx->timing.bulk_ms * ((double)mtot / (double)ns))
-      ctx->screen_skip = 8;
+    // kernel over every k-mer (measured on an unscreened scan of about this
+    // size, else its per-survivor time scaled to all k-mers), skip the screen
+    // for the next 8 scans, then measure again
+    if (auto_scr && ns > 0) {
+      double full = ctx->timing.bulk_ms * ((double)mtot / (double)ns);
+      if (ctx->full_m > 0 && mtot <= 2 * ctx->full_m && 2 * mtot >= ctx->full_m)
+        full = ctx->full_ms * (double)mtot / (double)ctx->full_m;
+      if (ctx->timing.screen_ms + ctx->timing.bulk_ms > full) ctx->screen_skip = 8;
+    }
+  } else if (auto_scr) {
+    ctx->full_ms = ctx->timing.bulk_ms;
+    ctx->full_m = mtot;
   }
   for (int s = 0; s < nsh; ++s)
     for (int tr = 0; tr < t; ++tr) n_passed[s * t + tr] += (int64_t)cnt[1 + (size_t)s * t + tr];
@@ -1806,6 +1833,8 @@ extern "C" int kg_set_screen(kg_ctx* ctx, int32_t mode) {
     return fail(ctx, KG_ERR_ARG, "screen mode must be KG_SCREEN_AUTO, _OFF or _ON");
   ctx->screen_mode = mode;
   ctx->screen_skip = 0;
+  ctx->full_m = 0;
+  ctx->probe_tick = 0;
   return KG_OK;
 }
 

@@ -492,6 +492,33 @@ def test_screen_at_the_threshold(built):
         dev.free()
 
 
+def test_screen_auto_small_scans(built):
+    """Auto mode on scans of up to 2^22 k-mers: the first scan at a new size runs unscreened to
+    measure the full kernel, the next one is screened, and later ones follow whichever was
+    faster (with a re-measurement every 32 scans).  Every scan reports what the forced modes
+    report."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    kw, m, corr, l10, l2d = CASES["n2048_p32"]
+    ds = dataset("n2048_p32", **kw)
+    bits = ds.fill_bits(0, m)
+    prm = ScanParams(corr, l10, l2d)
+    with KgwasHip(screen="on") as kon:
+        kon.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        h_on, n_on = kon.scan(bits, prm)
+    with KgwasHip() as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        dev = k.to_device(bits)
+        surv = []
+        for _ in range(12):
+            h, n = k.scan_device(dev.ptr, m, bits.shape[1], prm)
+            surv.append(k.timing()["survivors"])
+            assert np.array_equal(n, n_on)
+            assert np.array_equal(h["idx"], h_on["idx"]) and np.array_equal(h["T"], h_on["T"])
+        dev.free()
+    print(f"[screen auto] m={m} survivors per scan {surv}", flush=True)
+    assert surv[0] == m and surv[1] < m
+
+
 def test_screen_models_of_different_width(built):
     """One shard table whose models have different accession counts (1100, 600
     and 2600: 3, 2 and 6 chunks of 512), interleaved, so the screen re-stages a
