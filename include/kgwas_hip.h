@@ -42,8 +42,10 @@ extern "C" {
  * kernel runs on the rest.  Counts and hits are identical either way.
  * AUTO screens whenever corr_threshold > 0 and keeps screening while the
  * screen plus the survivor scan take less time than the full kernel over every
- * k-mer would (measured per scan; after a loss it skips 8 scans); OFF never
- * screens; ON always screens where the model allows (n_acc <= 8192). */
+ * k-mer would (measured per scan; after a loss it skips 8 scans; scans of up
+ * to 2^22 k-mers run unscreened at a new size and once per 32 scans to measure
+ * the full kernel); OFF never screens; ON always screens where the model
+ * allows (n_acc <= 8192). */
 #define KG_SCREEN_AUTO 0
 #define KG_SCREEN_OFF 1
 #define KG_SCREEN_ON 2
