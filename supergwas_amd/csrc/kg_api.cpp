@@ -1400,8 +1400,9 @@ static int scan_f32_shard(kg_ctx* ctx, const HShard& s, int sidx, const kg_scan_
   ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;
   ep.all_sig = p_thr >= 1.0 ? 1 : 0;
   ep.Tstar_lo = (Ts > Tmax) ? INFINITY : Ts * (1.0 - 1e-9);
-  unsigned long long cnt[1 + kMaxTraits];
   const int grid = ctx->cus * ctx->grid_mult_f32;
+  KG_CK(ctx, ctx->hcnt.ensure(sizeof(unsigned long long) * (1 + kMaxTraits)));
+  unsigned long long* cnt = static_cast<unsigned long long*>(ctx->hcnt.p);
   for (int attempt = 0; attempt < 3; ++attempt) {
     o.cand = ctx->cand.as<Cand>();
     o.counters = ctx->counters.as<unsigned long long>();
@@ -1413,7 +1414,8 @@ static int scan_f32_shard(kg_ctx* ctx, const HShard& s, int sidx, const kg_scan_
                                M.wf32.as<float>(),
                                M.mask.as<uint64_t>(), ep, o));
     KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    KG_CK(ctx, hipMemcpyAsync(cnt, o.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    KG_CK(ctx, hipMemcpyAsync(cnt, o.counters, sizeof(unsigned long long) * (1 + kMaxTraits),
+                              hipMemcpyDeviceToHost, ctx->stream));
     if (overlap && attempt == 0) {
       const int orc = (*overlap)();
       if (orc != KG_OK) {
@@ -1439,27 +1441,14 @@ static int scan_f32_shard(kg_ctx* ctx, const HShard& s, int sidx, const kg_scan_
   std::vector<Cand> cands(nc);
   if (nc)
     KG_CK(ctx, hipMemcpy(cands.data(), ctx->cand.p, sizeof(Cand) * nc, hipMemcpyDeviceToHost));
-  const size_t h0 = hits.size();
   for (int64_t i = 0; i < nc; ++i) {
     const Cand& cd = cands[i];
-    Decided d = decide(md, s.idx0 + cd.idx, cd.trait, -1, (double)cd.xx32, (double)cd.xy32, 0.0,
+    Decided d = decide(md, s.idx0 + cd.idx, cd.trait, cd.c, (double)cd.xx32, (double)cd.xy32, 0.0,
                        false, prm);
     if (d.hit) {
       d.h.shard = sidx;
       hits.push_back(d.h);
     }
-  }
-  // allele counts of reported k-mers from their bits (the FP32 records carry
-  // only xxM/xyM)
-  std::vector<uint64_t> row(s.rw);
-  for (size_t i = h0; i < hits.size(); ++i) {
-    kg_hit& h = hits[i];
-    KG_CK(ctx, hipMemcpy(row.data(), s.bits + (h.idx - s.idx0) * s.rw, s.rw * 8,
-                         hipMemcpyDeviceToHost));
-    int c = 0;
-    for (size_t w = 0; w < md.maskw.size() && (int64_t)w < s.rw; ++w)
-      c += __builtin_popcountll(row[w] & md.maskw[w]);
-    h.c = c;
   }
   return KG_OK;
 }

@@ -17,6 +17,8 @@ struct Cand {
   float xx32;      // plain FP32 path: the reference-arithmetic xxM, xyM
   float xy32;
   int32_t mult;    // rows idx .. idx + mult - 1 are this row (duplicate-row scans), else 1
+  int32_t c;       // allele count over the (shard-)masked accessions
+  int32_t pad;
 };
 constexpr uint32_t kPfPass = 1u;  // pre-filter decided pass
 constexpr uint32_t kPfAmb = 2u;   // pre-filter within the error bound of tau

@@ -546,6 +546,8 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         rec.xx32 = 0.f;
         rec.xy32 = 0.f;
         rec.mult = (int32_t)kmult;
+        rec.c = c;
+        rec.pad = 0;
         append_cand(fl != 0, rec, out);
       }
     }
@@ -1146,6 +1148,8 @@ __global__ __launch_bounds__(kThreads, 2) void scan_f32(
         rec.mult = 1;
         rec.xx32 = xxM;
         rec.xy32 = xyM;
+        rec.c = c;
+        rec.pad = 0;
         append_cand(cand, rec, out);
       }
       __syncthreads();
