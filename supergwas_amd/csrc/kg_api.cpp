@@ -95,21 +95,23 @@ struct HBuf {
   void* p = nullptr;
   void* dev = nullptr;  // device-visible address of p
   size_t bytes = 0;
+  unsigned flags = hipHostMallocDefault;  // of the current allocation
   HBuf() = default;
   HBuf(const HBuf&) = delete;
   HBuf& operator=(const HBuf&) = delete;
   ~HBuf() {
     if (p) (void)hipHostFree(p);
   }
-  hipError_t ensure(size_t b, unsigned flags = hipHostMallocDefault) {
-    if (b <= bytes && p) return hipSuccess;
+  hipError_t ensure(size_t b, unsigned want = hipHostMallocDefault) {
+    if (b <= bytes && p && want == flags) return hipSuccess;
     if (p) (void)hipHostFree(p);
     p = dev = nullptr;
     bytes = 0;
-    hipError_t e = hipHostMalloc(&p, b ? b : 16, flags);
+    hipError_t e = hipHostMalloc(&p, b ? b : 16, want);
     if (e == hipSuccess) e = hipHostGetDevicePointer(&dev, p, 0);
     if (e == hipSuccess) {
       bytes = b;
+      flags = want;
     } else if (p) {
       (void)hipHostFree(p);
       p = dev = nullptr;
@@ -1401,7 +1403,8 @@ static int scan_f32_shard(kg_ctx* ctx, const HShard& s, int sidx, const kg_scan_
   ep.all_sig = p_thr >= 1.0 ? 1 : 0;
   ep.Tstar_lo = (Ts > Tmax) ? INFINITY : Ts * (1.0 - 1e-9);
   const int grid = ctx->cus * ctx->grid_mult_f32;
-  KG_CK(ctx, ctx->hcnt.ensure(sizeof(unsigned long long) * (1 + kMaxTraits)));
+  KG_CK(ctx,
+        ctx->hcnt.ensure(sizeof(unsigned long long) * (1 + kMaxTraits), hipHostMallocCoherent));
   unsigned long long* cnt = static_cast<unsigned long long*>(ctx->hcnt.p);
   for (int attempt = 0; attempt < 3; ++attempt) {
     o.cand = ctx->cand.as<Cand>();

@@ -1172,11 +1172,9 @@ __global__ __launch_bounds__(kThreads, 2) void scan_f32(
 // residuals.  res[i] = {c, xxM, xyM, S}.  The first hb.eager candidates and
 // results, and the counters, are also written to hb's page-locked host
 // buffers, which the host reads after its one synchronisation.
-__global__ __launch_bounds__(64 * kRefWaves) void refine_f64(const Cand* __restrict__ cand,
-                                                             const unsigned long long* __restrict__ count,
-                                                             int64_t cap,
-                                                             const RefDesc* __restrict__ rd,
-                                                             double* __restrict__ res, RefHost hb) {
+__global__ __launch_bounds__(64 * kRefWaves) void refine_f64(
+    const Cand* __restrict__ cand, const unsigned long long* __restrict__ count, int64_t cap,
+    const RefDesc* __restrict__ rd, double* __restrict__ res, RefHost hb) {
   __shared__ double pa[kRefWaves][2][64];
   __shared__ int pc[kRefWaves];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
