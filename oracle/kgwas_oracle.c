@@ -52,6 +52,16 @@ double or_chi2_sf_k1(double T) {
   return erfc(sqrt(0.5 * T));
 }
 
+// Row-major copy of the column-major n x p U: the per-k-mer loops add U's
+// row j for every set bit j, and a contiguous row keeps those loads in cache
+// (the sums and their order are unchanged).
+static double* rows_of(const double* U, int n, int p) {
+  double* Ur = (double*)malloc(sizeof(double) * (size_t)n * (size_t)(p > 0 ? p : 1));
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < p; ++k) Ur[(size_t)j * p + k] = U[(size_t)k * n + j];
+  return Ur;
+}
+
 // yyM = y'y - |U'y|^2 (src/kgwas.cpp:391-392, src/kgwas_utils.hpp:151-152).
 double or_yyM(const double* y, int n, const double* U, int p) {
   double yy = 0.0, uu = 0.0;
@@ -94,6 +104,7 @@ int64_t or_scan_f64(const uint64_t* bits, int64_t m, int64_t row_words, const in
   const double yyM = or_yyM(y, n, U, p);
   const double eps = 1e-12;
   int64_t passed = 0;
+  double* Ur = rows_of(U, n, p);
 
 #pragma omp parallel
   {
@@ -130,7 +141,8 @@ int64_t or_scan_f64(const uint64_t* bits, int64_t m, int64_t row_words, const in
         if (x[j] != 0.0) {
           xx += x[j] * x[j];
           xy += x[j] * y[j];
-          for (int k = 0; k < p; ++k) Ux[k] += U[(size_t)k * n + j];
+          const double* Uj = Ur + (size_t)j * p;
+          for (int k = 0; k < p; ++k) Ux[k] += Uj[k];
         }
       }
       double uu = 0.0, uy = 0.0;
@@ -155,6 +167,7 @@ int64_t or_scan_f64(const uint64_t* bits, int64_t m, int64_t row_words, const in
   }
   free(yc);
   free(Uy);
+  free(Ur);
   return passed;
 }
 
@@ -181,6 +194,7 @@ int64_t or_pairs_f64(const uint64_t* bits, int64_t N, int64_t row_words, const i
     Uy[k] = s;
   }
   const double yyM = or_yyM(y, n, U, p);
+  double* Ur = rows_of(U, n, p);
   // x_i over the phenotyped accessions, one byte per sample
   uint8_t* X = (uint8_t*)malloc((size_t)(N > 0 ? N : 1) * (size_t)n);
   for (int64_t r = 0; r < N; ++r)
@@ -209,7 +223,8 @@ int64_t or_pairs_f64(const uint64_t* bits, int64_t N, int64_t row_words, const i
         if (xi[j] & xj[j]) {
           ++c12;
           b3 += y[j];
-          for (int k = 0; k < p; ++k) t12[k] += U[(size_t)k * n + j];
+          const double* Uj = Ur + (size_t)j * p;
+          for (int k = 0; k < p; ++k) t12[k] += Uj[k];
         }
       }
       if (cj == 0 || c12 == 0) continue;
@@ -243,6 +258,7 @@ int64_t or_pairs_f64(const uint64_t* bits, int64_t N, int64_t row_words, const i
   }
   free(X);
   free(Uy);
+  free(Ur);
   return found;
 }
 
