@@ -94,6 +94,8 @@ typedef struct kg_timing {
   double wall_ms;       /* kg_scan: host wall time of the whole call */
   double screen_ms;     /* pre-filter screen kernel (0 if unscreened) */
   int64_t survivors;    /* k-mers the screen passed to the full kernel (= kmers if unscreened) */
+  int64_t stat_refined; /* kg_stats_device (KG_PREC_MIXED): k-mer/trait values recomputed in
+                           FP64 because their bulk error estimate exceeded the budget */
 } kg_timing;
 
 /* Device selection, stream creation.  Replaces the OpenBLAS/OpenMP thread
@@ -170,9 +172,13 @@ int kg_scan_pairs(kg_ctx* ctx, const uint64_t* d_bits, int64_t N, int64_t row_wo
                   int64_t i0, int64_t i1, const kg_scan_params* prm, kg_hit* out, int64_t cap,
                   int64_t* n_out);
 
-/* Full-statistic mode (validation and checksums): the bulk kernel's values for
+/* Full-statistic mode (validation and checksums): the statistic's inputs for
  * every k-mer, written to host arrays: c[m], xxM[m], xyM[m*t], S[m*t]
- * (S = x'(y - mean y), the pre-filter numerator), k-mer-major. */
+ * (S = x'(y - mean y), the pre-filter numerator), k-mer-major.  KG_PREC_FP32:
+ * the FP32 kernel's values.  KG_PREC_MIXED: the bulk kernel's values, except
+ * for the k-mers whose estimated error of T exceeds 4e-6 relative (k-mers
+ * almost in the span of U, where x'x - |U'x|^2 cancels): those are recomputed
+ * in FP64 on the device (kg_timing.stat_refined counts them). */
 int kg_stats_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t row_words,
                     int32_t* c, double* xxM, double* xyM, double* S);
 

@@ -65,7 +65,7 @@ class _Timing(ctypes.Structure):
                 ("total_ms", ctypes.c_double), ("candidates", ctypes.c_int64),
                 ("kmers", ctypes.c_int64), ("copy_ms", ctypes.c_double),
                 ("wall_ms", ctypes.c_double), ("screen_ms", ctypes.c_double),
-                ("survivors", ctypes.c_int64)]
+                ("survivors", ctypes.c_int64), ("stat_refined", ctypes.c_int64)]
 
 
 _lib = None
@@ -334,7 +334,7 @@ class KgwasHip:
         self._check(lib().kg_last_timing(self._h, ctypes.byref(t)))
         return dict(bulk_ms=t.bulk_ms, refine_ms=t.refine_ms, total_ms=t.total_ms,
                     candidates=t.candidates, kmers=t.kmers, copy_ms=t.copy_ms, wall_ms=t.wall_ms,
-                    screen_ms=t.screen_ms, survivors=t.survivors)
+                    screen_ms=t.screen_ms, survivors=t.survivors, stat_refined=t.stat_refined)
 
     def autotune(self, sample=2_000_000):
         """Pick k-mer tiles per wave and workgroups per CU by timing `sample` random

@@ -171,6 +171,13 @@ struct Model {
   int nch_i8 = 0, nch_f32 = 0;
 };
 
+// Full-statistic mode: a k-mer/trait whose bulk values were recomputed in FP64.
+struct StatPatch {
+  int64_t idx;
+  int trait, c;
+  double xxM, xyM, S;
+};
+
 // A null model with its device-side weights and constants.
 struct ModelDev {
   Model md;
@@ -230,6 +237,7 @@ struct kg_ctx {
   int64_t full_m = 0;
   int probe_tick = 0;
   int dedup = 0;  // screened scans check each row against the previous one
+  std::vector<StatPatch> stat_patch;  // FP64 values of the last full-statistic scan
   DBuf surv, ssdesc, sdesc2, chdesc;  // survivor rows; graph-path screen tables
   DBuf bmap, scr_aux;                 // screen bitmaps; chunk counts and prefixes
   DBuf pmask;                         // all-pairs scan: mask AND row i, per shard
@@ -367,11 +375,16 @@ static void prepare_model(Model& md) {
 }
 
 // Quantise columns to `L` balanced int8 limbs: v = scale * sum_l limb_l / (127 * 254^l).
+// *h bounds |e_j| (e_j = col_j - v_j); with E != nullptr, *E = sum_j e_j and
+// *hc bounds |e_j - E / n| (the mean-corrected remainder), both with the FP64
+// rounding of the remainders and of their sum allowed for.
 static void quantize(const double* col, int n, int L, std::vector<int8_t>& limbs, double* w,
-                     double* h) {
+                     double* h, double* E = nullptr, double* hc = nullptr) {
   double scale = 0.0;
   for (int j = 0; j < n; ++j) scale = std::max(scale, std::fabs(col[j]));
   limbs.assign((size_t)n * L, 0);
+  if (E) *E = 0.0;
+  if (hc) *hc = 0.0;
   if (scale == 0.0) {
     for (int l = 0; l < L; ++l) w[l] = 0.0;
     *h = 0.0;
@@ -397,6 +410,23 @@ static void quantize(const double* col, int n, int L, std::vector<int8_t>& limbs
   // exact representable remainder bound: half of the last limb's step, plus
   // FP64 rounding of the residual recursion
   *h = 0.5 * w[L - 1] + 1e-15 * scale;
+  if (E) {
+    std::vector<double> e(n);
+    double sum = 0.0;
+    for (int j = 0; j < n; ++j) {
+      double v = 0.0;
+      for (int l = L - 1; l >= 0; --l) v += w[l] * (double)limbs[(size_t)j * L + l];
+      e[j] = col[j] - v;
+      sum += e[j];
+    }
+    const double mean = sum / n;
+    double mx = 0.0;
+    for (int j = 0; j < n; ++j) mx = std::max(mx, std::fabs(e[j] - mean));
+    *E = sum;
+    // e_j carries FP64 rounding of v and of the subtraction (< 1e-15 scale),
+    // the sum n such roundings of terms below *h
+    *hc = mx + 2e-15 * scale + 4e-16 * (double)n * (*h);
+  }
 }
 
 static int build_i8(kg_ctx* ctx, ModelDev& M) {
@@ -432,10 +462,11 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
     return 16 * (sl >> 2) + 4 * (b & 3) + (sl & 3);
   };
   std::vector<int8_t> lim;
-  std::vector<double> colw((size_t)nb * L), colh(nb);
+  std::vector<double> colw((size_t)nb * L), colh(nb), colE(nb);
   std::vector<int8_t> ucol((size_t)n * NU, 0);  // [sample][u-row]
   for (int b = 0; b < nb; ++b) {
-    quantize(src_col(b), n, L, lim, &colw[(size_t)b * L], &colh[b]);
+    double h0 = 0.0;
+    quantize(src_col(b), n, L, lim, &colw[(size_t)b * L], &h0, &colE[b], &colh[b]);
     for (int l = 0; l < L; ++l) {
       const int u = urow(b, l);
       for (int j = 0; j < n; ++j) ucol[(size_t)j * NU + u] = lim[(size_t)j * L + l];
@@ -469,26 +500,32 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
     for (int k = 0; k < p; ++k) B[(size_t)a * nbf + k] = md.Uc[(size_t)k * n + j];
     for (int tr = 0; tr < t; ++tr) B[(size_t)a * nbf + p + tr] = md.rc[(size_t)tr * n + j];
   }
-  // column records {w0, w1, w2, k_0..k_7, 0}; zero past nb
+  // column records {w0, w1, w2, k_0..k_7, E, h^2, 0}; zero past nb
   std::vector<double> rec((size_t)ncol * kColRec, 0.0);
-  double sum_h2 = 0.0;
-  std::vector<double> sum_hbp(kMaxTraits, 0.0);
+  double sum_h2 = 0.0, sum_E2 = 0.0;
+  std::vector<double> sum_hbp(kMaxTraits, 0.0), sum_Ek(kMaxTraits, 0.0);
   for (int b = 0; b < nb; ++b) {
     double* r = &rec[(size_t)b * kColRec];
     for (int l = 0; l < L; ++l) r[l] = colw[(size_t)b * L + l];
+    r[11] = colE[b];
+    r[12] = colh[b] * colh[b];
     if (b < t) {
       r[3 + b] = 1.0;
+      sum_Ek[b] += colE[b];
     } else {
       const int k = zc[b - t];
       sum_h2 += colh[b] * colh[b];
+      sum_E2 += colE[b] * colE[b];
       for (int tr = 0; tr < t; ++tr) {
         const double bpv = md.bprime[(size_t)tr * p + k];
         r[3 + tr] = bpv;
         sum_hbp[tr] += colh[b] * std::fabs(bpv);
+        sum_Ek[tr] += colE[b] * bpv;
       }
     }
   }
-  // per-trait records {mean r, |y - ybar|, yyM, b'_0 (dropped d column), h of r_c}
+  // per-trait records {mean r, |y - ybar|, yyM, b'_0 (dropped d column), h of r_c,
+  // E of r_c, sum_b E_b k_tr}
   std::vector<double> trc((size_t)t * kTrcRec, 0.0);
   for (int tr = 0; tr < t; ++tr) {
     double* r = &trc[(size_t)tr * kTrcRec];
@@ -497,6 +534,8 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
     r[2] = md.yyM[tr];
     r[3] = (md.rotated && md.drop_d) ? md.bprime[(size_t)tr * p + 0] : 0.0;
     r[4] = colh[tr];
+    r[5] = colE[tr];
+    r[6] = sum_Ek[tr];
   }
   auto up = [&](DBuf& d, const void* src, size_t bytes) -> hipError_t {
     hipError_t e = d.ensure(bytes);
@@ -507,7 +546,10 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
   KG_CK(ctx, up(M.mask, md.maskw.data(), md.maskw.size() * 8));
   KG_CK(ctx, up(M.rbar, md.rbar.data(), t * 8));
   KG_CK(ctx, up(M.B64, B.data(), B.size() * 8));
-  KG_CK(ctx, up(M.bpfull, md.bprime.data(), std::max<size_t>(md.bprime.size(), 1) * 8));
+  // b' has t * p entries: none when U has no columns (one zero keeps the upload valid)
+  std::vector<double> bpf(md.bprime);
+  if (bpf.empty()) bpf.push_back(0.0);
+  KG_CK(ctx, up(M.bpfull, bpf.data(), bpf.size() * 8));
   KG_CK(ctx, up(M.colrec, rec.data(), rec.size() * 8));
   KG_CK(ctx, up(M.trc, trc.data(), trc.size() * 8));
   EpiI8& ep = M.epi;
@@ -526,6 +568,9 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
   ep.trc = M.trc.as<double>();
   ep.sum_h2 = sum_h2 * (1.0 + 1e-12);
   for (int tr = 0; tr < kMaxTraits; ++tr) ep.sum_hbp[tr] = sum_hbp[tr] * (1.0 + 1e-12);
+  ep.sum_E2 = sum_E2;
+  ep.E_d = ep.has_d ? colE[t] : 0.0;
+  ep.stat_est = 0;
   return KG_OK;
 }
 
@@ -1000,6 +1045,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     D.ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;
     D.ep.all_sig = p_thr >= 1.0 ? 1 : 0;
     D.ep.pairs = pairs ? 1 : 0;
+    D.ep.stat_est = stats ? 1 : 0;
     const double Tmax = (double)md.n * std::log(1.0 / (1.0 - (1 - 1e-15)));
     D.ep.r2star = (Ts > Tmax) ? INFINITY : -std::expm1(-Ts / (double)md.n);
     sd1[s] = D;
@@ -1130,6 +1176,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     o.st_S = ctx->st_S.as<double>();
   }
   const bool use_graph = (flags & kGraph) != 0 && !stats;
+  ctx->stat_patch.clear();
   const int ref_blocks = ctx->cus * 2;
   std::vector<unsigned long long> cnt(ctx->counters_len);
   KG_CK(ctx, ctx->hcnt.ensure(sizeof(unsigned long long) * cnt.size(), hipHostMallocCoherent));
@@ -1318,6 +1365,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   ctx->timing.candidates = nc;
   ctx->timing.kmers = mtot;
   ctx->timing.survivors = mtot;
+  ctx->timing.stat_refined = 0;
   if (screen) {
     int64_t ns = 0;
     for (int s = 0; s < nsh; ++s) ns += (int64_t)cnt[base_surv + s];
@@ -1355,6 +1403,9 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     const HShard& s = sh[cd.shard];
     const Model& md = ctx->models[s.model]->md;
     const int c = (int)res[4 * i + 0];
+    if (stats && (cd.flags & kStatRef))
+      ctx->stat_patch.push_back(
+          StatPatch{s.idx0 + cd.idx, cd.trait, c, res[4 * i + 1], res[4 * i + 2], res[4 * i + 3]});
     Decided d = decide(md, s.idx0 + cd.idx, cd.trait, c, res[4 * i + 1], res[4 * i + 2],
                        res[4 * i + 3], !pairs, prm, pairs);
     // a duplicate-row scan decided row idx for the cd.mult equal rows idx, idx + 1, ...
@@ -1816,6 +1867,15 @@ extern "C" int kg_stats_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m,
   KG_CK(ctx, hipMemcpy(xxM, ctx->st_xx.p, sizeof(double) * m, hipMemcpyDeviceToHost));
   KG_CK(ctx, hipMemcpy(xyM, ctx->st_xy.p, sizeof(double) * m * t, hipMemcpyDeviceToHost));
   KG_CK(ctx, hipMemcpy(S, ctx->st_S.p, sizeof(double) * m * t, hipMemcpyDeviceToHost));
+  // the mixed path's k-mers whose bulk values risk more than the error budget
+  // were recomputed in FP64 (refine_f64)
+  for (const StatPatch& q : ctx->stat_patch) {
+    xxM[q.idx] = q.xxM;
+    xyM[q.idx * t + q.trait] = q.xyM;
+    S[q.idx * t + q.trait] = q.S;
+  }
+  ctx->timing.stat_refined = (int64_t)ctx->stat_patch.size();
+  ctx->stat_patch.clear();
   return KG_OK;
 }
 

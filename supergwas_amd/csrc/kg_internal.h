@@ -24,6 +24,9 @@ constexpr uint32_t kPfPass = 1u;  // pre-filter decided pass
 constexpr uint32_t kPfAmb = 2u;   // pre-filter within the error bound of tau
 constexpr uint32_t kSigDef = 4u;  // 1D threshold decided pass
 constexpr uint32_t kSigAmb = 8u;  // 1D threshold within the error bound
+// full-statistic mode: the statistic's estimated quantisation error exceeds the
+// bulk budget, so the k-mer's values are recomputed in FP64
+constexpr uint32_t kStatRef = 16u;
 
 // Per-scan outputs of the bulk kernels.
 struct ScanOut {
@@ -62,15 +65,26 @@ struct EpiI8 {
   double r2star;    // 1D threshold in r^2
   double h_d;       // quantisation bound per set bit of column d (if kept)
   double inv_n;     // 1 / n
-  const double* colrec; // [ncol][kColRec] {w0, w1, w2, k_0..k_7, 0}: staged in LDS
-  const double* trc;    // [t][kTrcRec] {mean r, |y - ybar|, yyM, b'_0, h of r_c}
-  double sum_h2;        // sum over z columns of h^2   (|z|^2 bound: 2c |z| |h| + c^2 |h|^2)
+  const double* colrec; // [ncol][kColRec] {w0, w1, w2, k_0..k_7, E, h^2, 0}: staged in LDS
+  const double* trc;    // [t][kTrcRec] {mean r, |y - ybar|, yyM, b'_0, h of r_c, E of r_c,
+                        //               sum_b E_b k_tr (the mean correction of S)}
+  double sum_h2;        // sum over z columns of h^2   (|z|^2 bound: 2c' |z| |h| + c'^2 |h|^2)
   double sum_hbp[8];    // per trait: sum over z columns of h |b'|
+  double sum_E2;        // sum over z columns of E^2
+  double E_d;           // E of column d (if kept)
+  int stat_est;         // full-statistic mode: flag k-mers whose estimated error is large
+  int pad_s;
 };
+// Mean correction.  A column v is quantised to v~ = v - e (e = the per-sample
+// remainder below the last limb, |e_j| <= h0).  With E = sum_j e_j over the
+// phenotyped samples, the kernel adds (c / n) E to x'v~, so its error is
+// sum_{j in x} (e_j - E / n) = -sum_{j not in x} (e_j - E / n): at most
+// c' h with c' = min(c, n - c) and h = max_j |e_j - E / n|.
+//
 // Column record: limb weights w_l, then per trait k_tr = b'_tr of a structure
 // column, 1 for the trait's own residual column and 0 otherwise, so that
-// sum_b z_b k_tr = z'b' + x'r_c = S.
-constexpr int kColRec = 12;
+// sum_b z_b k_tr = z'b' + x'r_c = S; then E and h^2 of the column.
+constexpr int kColRec = 14;
 constexpr int kTrcRec = 8;
 constexpr int kMaxCols = 88;      // column records the int8 kernel stages (NT <= 16, L = 3)
 

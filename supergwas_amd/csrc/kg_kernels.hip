@@ -141,6 +141,31 @@ __device__ __forceinline__ uint32_t classify_i8(int c, double xxM, double bxx, d
 }
 
 // ----------------------------------------------------------------------------
+// Full-statistic mode: does the bulk statistic of this k-mer risk more than the
+// budget of relative error (|dT| / max(T, 1) > 4e-6)?  The mean-corrected
+// remainders behave as a sample of c of the n per-sample remainders, so
+// dz_k has standard deviation h_k sqrt(c (n - c) / (3 n)) at most; the
+// estimate takes kappa = 8 of those (the certified bounds, linear in c, would
+// flag most k-mers).  Such k-mers get FP64 values from refine_f64.
+__device__ __forceinline__ bool stat_needs_fp64(int c, double xx, double xy, double zh2,
+                                                double h_r, double yy, const EpiI8& ep) {
+  constexpr double kappa = 8.0, budget = 4e-6;
+  const double cd = (double)c, nn = (double)ep.n;
+  const double v3 = cd * (nn - cd) / (3.0 * nn);
+  if (!(v3 > 0.0) || !(yy > 1e-12)) return false;  // c = 0, c = n: T = 0 either way
+  const double exx = 2.0 * kappa * sqrt(zh2 * v3) + kappa * kappa * ep.sum_h2 * v3;
+  const double exy = kappa * h_r * sqrt(v3);
+  const double lo = xx - exx;
+  if (!(lo > 1e-12)) return xx + exx > 1e-12;       // compute_hit's xxM > eps may flip
+  const double r2 = xy * xy / (xx * yy);
+  const double dr2 = (2.0 * fabs(xy) * exy + exy * exy) / (lo * yy) + r2 * exx / lo;
+  const double q = 1.0 - r2 - dr2;
+  if (!(q > 1e-15)) return true;                     // near the r^2 clamp
+  const double T = -nn * log1p(-fmin(r2, 1.0 - 1e-15));
+  return nn * dr2 / q > budget * fmax(T, 1.0);
+}
+
+// ----------------------------------------------------------------------------
 // Bulk int8-limb scan over a table of shards.  Workgroup = 8 waves (one per
 // CU); global block `w` belongs to the shard s with sd[s].blk0 <= w <
 // sd[s+1].blk0, and wave v of that block owns the shard's k-mers
@@ -429,10 +454,11 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     const int t = ep.t, n = ep.n;
     const int rme = g % R;
     const bool primary = g < R;
-    double pzz[R], pz0[R], pS[R][TT], psv[R][TT];
+    const bool est = ep.stat_est != 0;  // full-statistic mode: error estimate per k-mer
+    double pzz[R], pz0[R], pzE[R], pzh[R], pS[R][TT], psv[R][TT];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      pzz[r] = pz0[r] = 0.0;
+      pzz[r] = pz0[r] = pzE[r] = pzh[r] = 0.0;
 #pragma unroll
       for (int tr = 0; tr < TT; ++tr) pS[r][tr] = psv[r][tr] = 0.0;
     }
@@ -440,6 +466,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     for (int j = 0; j < J; ++j) {
       const double* cr = colrec_lds + (4 * j + g) * kColRec;
       const double w0 = cr[0], w1 = cr[1], w2 = cr[2];
+      const double cE = cr[11], ch2 = cr[12];
       double kc[TT];
 #pragma unroll
       for (int tr = 0; tr < TT; ++tr) kc[tr] = cr[3 + tr];
@@ -463,19 +490,27 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         if (j < JSP) {  // may be a residual column (b < t) or column d (b == t)
           const double zs = (b < t) ? 0.0 : z;
           pzz[r] = fma(zs, zs, pzz[r]);
+          pzE[r] = fma(zs, cE, pzE[r]);
+          if (est) pzh[r] = fma(zs * zs, ch2, pzh[r]);
           if (b == t) pz0[r] = z;
 #pragma unroll
           for (int tr = 0; tr < TT; ++tr) psv[r][tr] += (b == tr) ? z : 0.0;
         } else {
           pzz[r] = fma(z, z, pzz[r]);
+          pzE[r] = fma(z, cE, pzE[r]);
+          if (est) pzh[r] = fma(z * z, ch2, pzh[r]);
         }
 #pragma unroll
         for (int tr = 0; tr < TT; ++tr) pS[r][tr] = fma(z, kc[tr], pS[r][tr]);
       }
     }
     const int c = (int)group_scatter<R>(cnt);
-    const double kzz = group_scatter<R>(pzz);
-    const double kz0 = group_scatter<R>(pz0);
+    const double cd = (double)c, cn = cd * ep.inv_n;
+    // mean correction (kg_internal.h): z_k + (c/n) E_k for every column
+    const double kzE = group_scatter<R>(pzE);
+    const double kzz = group_scatter<R>(pzz) + cn * fma(cn, ep.sum_E2, 2.0 * kzE);
+    const double kz0 = group_scatter<R>(pz0) + cn * ep.E_d;
+    const double kzh = est ? group_scatter<R>(pzh) : 0.0;
     double kS[TT], ksv[TT];
 #pragma unroll
     for (int tr = 0; tr < TT; ++tr) {
@@ -493,14 +528,14 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     const bool valid = primary && (kidx < D.m);
     const int64_t krow = (D.rows != nullptr && valid) ? (int64_t)gptr(D.rows)[kidx] : kidx;
     const uint32_t kmult = (D.mult != nullptr && valid) ? gptr(D.mult)[kidx] : 1u;
-    const double cd = (double)c, cn = cd * ep.inv_n;
-    // |dz_k| <= c h_k  =>  |d |z|^2| <= 2c sum|z_k| h_k + c^2 sum h_k^2
-    //                              <= 2c |z| |h| + c^2 |h|^2
-    const double bzz = 2.0 * cd * sqrt(kzz * ep.sum_h2) + cd * cd * ep.sum_h2;
+    // |dz_k| <= c' h_k (c' = min(c, n - c))  =>
+    //   |d |z|^2| <= 2c' sum|z_k| h_k + c'^2 sum h_k^2 <= 2c' |z| |h| + c'^2 |h|^2
+    const double cm = fmin(cd, (double)ep.n - cd);
+    const double bzz = 2.0 * cm * sqrt(kzz * ep.sum_h2) + cm * cm * ep.sum_h2;
     double cross = 0.0, bcross = 0.0;
     if (ep.has_d) {
       cross = 2.0 * cn * ep.sigma * kz0;
-      bcross = 2.0 * cn * fabs(ep.sigma) * cd * ep.h_d;
+      bcross = 2.0 * cn * fabs(ep.sigma) * cm * ep.h_d;
     }
     const double xxM = cd - cd * cd * ep.base_coef - cross - kzz;
     double bxx = bzz + bcross + 1e-11 * (cd + kzz + fabs(cross) + 1.0);
@@ -516,10 +551,10 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     for (int tr = 0; tr < TT; ++tr) {
       if (tr < t) {
         const double* tc = trc_lds + tr * kTrcRec;
-        const double S = kS[tr], sv = ksv[tr];
+        const double S = kS[tr] + cn * tc[6], sv = ksv[tr] + cn * tc[5];
         const double rbar = tc[0];
-        const double es = cd * tc[4];
-        double bS = cd * ep.sum_hbp[tr] + es;
+        const double es = cm * tc[4];
+        double bS = cm * ep.sum_hbp[tr] + es;
         if (ep.dmax_drop > 0.0) bS += cd * ep.dmax_drop * fabs(tc[3]);
         bS += 1e-12 * (fabs(S) + fabs(sv) + 1.0);
         const double xyM = sv + cd * rbar;
@@ -530,6 +565,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         if (out.st_c != nullptr && valid) {
           out.st_xy[kidx * t + tr] = xyM;
           out.st_S[kidx * t + tr] = S;
+          if (est && stat_needs_fp64(c, xxM, xyM, kzh, tc[4], tc[2], ep)) fl |= kStatRef;
         }
         if (D.mult == nullptr) {
           wpass[tr] += (uint32_t)__popcll(__ballot(pf_def));

@@ -187,7 +187,15 @@ void run_cols(const Config& cfg) {
   vector<kg_hit> hits = sc.scan(rows, (int64_t)N, rw, prm, &passed);
   std::cerr << "[INFO] CORR_THRESHOLD=" << cfg.corr << "  total kmers: " << N
             << ", passed correlation filter: " << passed << "\n";
-  std::cerr << "[INFO] GPU scan (" << tscan.sec() << " s)\n";
+  // the reference's stage lines (src/kgwas.cpp:217, 164): device time of the
+  // pre-filter screen, then of the projections and the FP64 refinement
+  kg_timing tm{};
+  if (kg_last_timing(sc.ctx, &tm) != KG_OK) die(string("timing: ") + kg_last_error(sc.ctx));
+  std::cerr << "[INFO] Correlation filter (" << tm.screen_ms * 1e-3 << " s)\n";
+  std::cerr << "[INFO] LA (" << (tm.bulk_ms + tm.refine_ms) * 1e-3
+            << " s, BLAS_THREADS=" << cfg.blas_threads << ")\n";
+  std::cerr << "[INFO] GPU scan (" << tscan.sec() << " s, copies " << tm.copy_ms * 1e-3
+            << " s)\n";
 
   // emit in k-mer order (src/kgwas.cpp:239-266)
   Clock temit;
@@ -217,7 +225,8 @@ void run_cols(const Config& cfg) {
     if (!out) die("write out failed");
   }
   std::cerr << "[INFO] wrote 1D hits at -log10(p) >= " << cfg.log10_thresh << " ("
-            << temit.sec() << " s, " << hits.size() << " hits)\n";
+            << temit.sec() << " s, EMIT_THREADS=" << cfg.emit_threads << ", " << hits.size()
+            << " hits)\n";
   if (do_2d) {
     const string p2 = path_2d(cfg.out);
     std::ofstream out2(p2, std::ios::binary);

@@ -62,10 +62,11 @@ def test_bad_options_rejected():
 
 
 def test_option_and_timing_layouts():
-    # kg_opts: 8 int32; kg_timing: 3 doubles, 2 int64, 3 doubles, 1 int64
+    # kg_opts: 8 int32; kg_timing: 3 doubles, 2 int64, 3 doubles, 2 int64
     assert ctypes.sizeof(api._Opts) == 32
-    assert ctypes.sizeof(api._Timing) == 72
+    assert ctypes.sizeof(api._Timing) == 80
     assert api._Opts.screen.offset == 12 and api._Timing.survivors.offset == 64
+    assert api._Timing.stat_refined.offset == 72
     assert api._Opts.dedup.offset == 16
 
 

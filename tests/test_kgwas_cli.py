@@ -173,6 +173,10 @@ def test_end_to_end_outputs(small, tmp_path, mode):
     assert open(out).read() == t1
     assert open(str(tmp_path / "res.batch_000.2d.tsv")).read() == t2
     assert open(out + ".count").read() == f"res.batch_000\t{ref['n_passed']}\n"
+    if mode.startswith("cols"):  # the reference's stage lines (src/kgwas.cpp:164,199,217)
+        for line in ("passed correlation filter: ", "[INFO] Correlation filter (",
+                     "[INFO] LA (", "[INFO] Count written to "):
+            assert line in r.stderr, line
 
 
 @pytest.mark.gpu
@@ -262,3 +266,47 @@ def test_dedup_outputs_identical(small, tmp_path):
     assert open(out + ".count").read() == f"res.batch_000\t{ref['n_passed']}\n"
     r = run(base_args(q, out), env={"KG_DEDUP": "yes"})
     assert r.returncode == 1 and "KG_DEDUP must be 0 or 1" in r.stderr
+
+
+def _write_ubin(path, U):
+    with open(path, "wb") as f:
+        f.write(np.array(U.shape, np.int32).tobytes())
+        f.write(np.asfortranarray(U, np.float64).tobytes(order="F"))
+
+
+@pytest.mark.gpu
+def test_too_many_structure_columns(small, tmp_path):
+    """p + t above the int8 kernel's 84 base columns (at 3 limbs) ends with [ERROR] + exit 1."""
+    ds, bits, p, root = small
+    rng = np.random.default_rng(5)
+    U, _, _ = np.linalg.svd(rng.standard_normal((ds.n, 85)), full_matrices=False)
+    ub = tmp_path / "U85.bin"
+    _write_ubin(ub, U)
+    r = run([p["samples"], p["pheno"], p["batch"], str(ub), str(tmp_path / "o.tsv")])
+    print(r.stderr[-600:])
+    assert r.returncode == 1
+    assert "[ERROR] model set-up failed: p + t exceeds the int8 kernel's weight columns" in r.stderr
+
+
+@pytest.mark.gpu
+def test_too_many_samples(tmp_path, built):
+    """More than 60000 phenotyped samples (the int8 kernel's exact-sum limit) ends with
+    [ERROR] + exit 1; the inputs are 60001 accessions with 2 k-mers."""
+    n = 60001
+    ids = [f"s{j}" for j in range(n)]
+    (tmp_path / "acc.txt").write_text("\n".join(ids) + "\n")
+    rng = np.random.default_rng(6)
+    y = rng.standard_normal(n)
+    (tmp_path / "ph.txt").write_text("".join(f"{a} {v!r}\n" for a, v in zip(ids, y.tolist())))
+    _write_ubin(tmp_path / "U.bin", np.full((n, 1), 1.0 / np.sqrt(n)))
+    bd = tmp_path / "batch_000" / "bits"
+    bd.mkdir(parents=True)
+    for j in range(n):
+        (bd / f"col_{j:06d}.txt").write_bytes(b"01\n" if j % 2 else b"10\n")
+    with gzip.open(tmp_path / "batch_000" / "kmers.txt.gz", "wt") as f:
+        f.write("K0\nK1\n")
+    r = run([str(tmp_path / "acc.txt"), str(tmp_path / "ph.txt"), str(tmp_path / "batch_000"),
+             str(tmp_path / "U.bin"), str(tmp_path / "o.tsv")])
+    print(r.stderr[-600:])
+    assert r.returncode == 1
+    assert "[ERROR] model set-up failed: more than 60000 phenotyped samples" in r.stderr
