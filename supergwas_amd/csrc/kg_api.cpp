@@ -27,13 +27,13 @@ namespace kg {
 hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t s,
                           const ShardDesc* sd, int nshards, int64_t total_blocks,
                           const ScanOut& o, const int64_t* total_dev);
-int64_t screen_block_kmers(int t);
+int64_t screen_block_kmers(int t, bool w8);
 int screen_rows(int t);
 bool screen_count_row(int t);
 int screen_max_chunks();
-int screen_wgs_per_cu(int t, int nch, bool dd);
+int screen_wgs_per_cu(int t, int nch, bool dd, bool w8);
 hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
-                         int nshards, int64_t total_blocks, bool dd);
+                         int nshards, int64_t total_blocks, bool dd, bool w8);
 int screen_chunk_words();
 hipError_t launch_screen_compact(hipStream_t s, const ScrChunk* ch, int ncc, const ScreenDesc* ss,
                                  uint32_t* ccount, int64_t* P, ShardDesc* sd2, int nsh,
@@ -225,6 +225,7 @@ struct kg_ctx {
   int grid_mult = 1;  // int8 scan: persistent workgroups per CU (KG_GRID_MULT)
   int tiles = 0;      // int8 scan: k-mer tiles per wave (0 = the kernel's default)
   int grid_mult_f32 = 8;  // FP32 scan: workgroups per CU
+  bool scr_w8 = false;    // screen: the 8-wave workgroup shape (kg_autotune may pick it)
   // pre-filter screen (int8 path): mode, and in auto mode the unscreened
   // scans left before the screen is tried again
   int screen_mode = KG_SCREEN_AUTO;
@@ -923,6 +924,7 @@ struct ScreenRun {
   int sgrid = 0;
   int nch_max = 0;
   bool dd = false;              // with the duplicate-row check
+  bool w8 = false;              // the screen's 8-wave shape
 };
 
 // Device part of a batched int8 scan: zero counters (unless the table upload
@@ -937,7 +939,7 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
   KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (sr.on) {
     KG_CK(ctx, launch_screen(t, sr.nch_max, sr.sgrid, ctx->stream, sr.d_ss, nsh, sr.sblocks,
-                             sr.dd));
+                             sr.dd, sr.w8));
     KG_CK(ctx, hipEventRecord(ctx->evs, ctx->stream));
     KG_CK(ctx, launch_screen_compact(ctx->stream, sr.d_ch, sr.ncc, sr.d_ss, sr.ccount, sr.P,
                                      sr.d_sd2, nsh, i8_block_kmers(NT, L, t, ctx->tiles),
@@ -964,6 +966,49 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
 enum { kPerLaunch = 1, kGraph = 2, kPairs = 4 };
 constexpr int64_t kEagerCands = 4096;  // candidates read back with the counters
 constexpr int64_t kProbeKmers = int64_t(1) << 22;  // auto screen: scans that measure unscreened
+
+// Descriptors of shard s (index `sid`) for the int8 bulk kernel and the FP64
+// refinement; its pre-filter passes are counted at `pass`, its blocks start at
+// global block blk0.
+static void fill_descs(const kg_ctx* ctx, const HShard& sh, int sid, const kg_scan_params& prm,
+                       double Ts, double p_thr, bool pairs, bool stats, unsigned long long* pass,
+                       int64_t blk0, ShardDesc& D, RefDesc& R) {
+  const ModelDev& M = *ctx->models[sh.model];
+  const Model& md = M.md;
+  D.bits = sh.bits;
+  D.m = sh.m;
+  D.rw = sh.rw;
+  D.blk0 = blk0;
+  D.wts = M.wts.p;
+  D.mask = sh.mask ? static_cast<const void*>(sh.mask) : M.mask.p;
+  D.nchunks = md.nch_i8;
+  D.pad = 0;
+  D.pass = pass;
+  D.id = sid;
+  D.pad2 = 0;
+  D.rows = nullptr;
+  D.mult = nullptr;
+  D.ep = M.epi;
+  D.ep.tau = prm.corr_threshold;
+  D.ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;
+  D.ep.all_sig = p_thr >= 1.0 ? 1 : 0;
+  D.ep.pairs = pairs ? 1 : 0;
+  D.ep.stat_est = stats ? 1 : 0;
+  const double Tmax = (double)md.n * std::log(1.0 / (1.0 - (1 - 1e-15)));
+  D.ep.r2star = (Ts > Tmax) ? INFINITY : -std::expm1(-Ts / (double)md.n);
+  R.bits = sh.bits;
+  R.rw = sh.rw;
+  R.maskw = sh.mask ? sh.mask : M.mask.as<uint64_t>();
+  R.B64 = M.B64.as<double>();
+  R.bpfull = M.bpfull.as<double>();
+  R.rbar = M.rbar.as<double>();
+  R.sigma = md.sigma;
+  R.base_coef = md.base_coef;
+  R.nwords = (md.n_acc + 63) / 64;
+  R.p = md.p;
+  R.t = md.t;
+  R.n = md.n;
+}
 
 // Scans every shard (int8 path), appending hits (idx offset by shard idx0,
 // shard index set) and adding pre-filter passes to n_passed[s * t + tr].
@@ -1024,45 +1069,12 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   KG_CK(ctx, ctx->counters.ensure(sizeof(unsigned long long) * ctx->counters_len));
   unsigned long long* dc = ctx->counters.as<unsigned long long>();
   for (int s = 0; s < nsh; ++s) {
-    const ModelDev& M = *ctx->models[sh[s].model];
-    const Model& md = M.md;
-    ShardDesc& D = sd[s];
-    D.bits = sh[s].bits;
-    D.m = sh[s].m;
-    D.rw = sh[s].rw;
-    D.blk0 = total_blocks;
+    fill_descs(ctx, sh[s], s, prm, Ts, p_thr, pairs, stats, dc + 1 + (size_t)s * t, total_blocks,
+               sd[s], rd[s]);
     nblk[s] = (sh[s].m + bk - 1) / bk;
     total_blocks += nblk[s];
-    D.wts = M.wts.p;
-    D.mask = sh[s].mask ? static_cast<const void*>(sh[s].mask) : M.mask.p;
-    D.nchunks = md.nch_i8;
-    D.pad = 0;
-    D.pass = dc + 1 + (size_t)s * t;
-    D.id = s;
-    D.pad2 = 0;
-    D.ep = M.epi;
-    D.ep.tau = prm.corr_threshold;
-    D.ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;
-    D.ep.all_sig = p_thr >= 1.0 ? 1 : 0;
-    D.ep.pairs = pairs ? 1 : 0;
-    D.ep.stat_est = stats ? 1 : 0;
-    const double Tmax = (double)md.n * std::log(1.0 / (1.0 - (1 - 1e-15)));
-    D.ep.r2star = (Ts > Tmax) ? INFINITY : -std::expm1(-Ts / (double)md.n);
-    sd1[s] = D;
+    sd1[s] = sd[s];
     sd1[s].blk0 = 0;
-    RefDesc& R = rd[s];
-    R.bits = sh[s].bits;
-    R.rw = sh[s].rw;
-    R.maskw = sh[s].mask ? sh[s].mask : M.mask.as<uint64_t>();
-    R.B64 = M.B64.as<double>();
-    R.bpfull = M.bpfull.as<double>();
-    R.rbar = M.rbar.as<double>();
-    R.sigma = md.sigma;
-    R.base_coef = md.base_coef;
-    R.nwords = (md.n_acc + 63) / 64;
-    R.p = md.p;
-    R.t = md.t;
-    R.n = md.n;
   }
   // screen tables: survivors of shard s go to surv[off_s, off_s + m_s); its
   // bitmap covers its screen blocks; compaction chunks of at most
@@ -1073,7 +1085,8 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   ScreenRun sr;
   const bool dd = screen && ctx->dedup != 0;
   if (screen) {
-    const int64_t sbk = screen_block_kmers(t);
+    sr.w8 = ctx->scr_w8;
+    const int64_t sbk = screen_block_kmers(t, sr.w8);
     const int64_t cw = screen_chunk_words();
     int64_t bm_words = 0;
     for (int s = 0; s < nsh; ++s) bm_words += (sh[s].m + sbk - 1) / sbk * (sbk / 32);
@@ -1146,9 +1159,9 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
                                    sizeof(int64_t) * (chunks.size() + 2)));
     sr.P = ctx->scr_aux.as<int64_t>();
     sr.ccount = reinterpret_cast<uint32_t*>(sr.P + chunks.size() + 2);
-    const int okey = (t * 64 + nch_max) * 2 + (dd ? 1 : 0);
+    const int okey = ((t * 64 + nch_max) * 2 + (dd ? 1 : 0)) * 2 + (sr.w8 ? 1 : 0);
     if (ctx->scr_occ_key != okey) {
-      ctx->scr_occ = screen_wgs_per_cu(t, nch_max, dd);
+      ctx->scr_occ = screen_wgs_per_cu(t, nch_max, dd, sr.w8);
       ctx->scr_occ_key = okey;
     }
     sr.sgrid = ctx->cus * ctx->scr_occ;
@@ -1226,7 +1239,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       kp += sizeof(RefDesc) * nsh;
       memcpy(kp, &o, sizeof(o));
       kp += sizeof(o);
-      const int64_t extra[2] = {(int64_t)flags, (int64_t)ctx->grid_mult * 64 + ctx->tiles};
+      const int64_t extra[2] = {(int64_t)flags, ((int64_t)ctx->grid_mult * 64 + ctx->tiles) * 2 + (ctx->scr_w8 ? 1 : 0)};
       memcpy(kp, extra, 16);
       key.insert(key.end(), reinterpret_cast<const char*>(&hb),
                  reinterpret_cast<const char*>(&hb) + sizeof(hb));

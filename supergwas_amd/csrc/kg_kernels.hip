@@ -731,10 +731,12 @@ __device__ __forceinline__ void scr_group(const v4u (&q)[DC][R], int C0, int nch
 // rows (the same block's next chunks, or the next block's first) are loaded
 // into a second register set while the current group computes, then copied
 // into the current set (the copy is the only wait for them).
-template <int Q, bool CR, bool DD>
-__global__ __launch_bounds__(64 * scr_waves(Q)) void screen_i8(
-    const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blocks) {
-  constexpr int W = scr_waves(Q);
+//
+// W = waves per workgroup; MINB = 2 caps the registers at 128 per wave (four
+// waves per SIMD), so that two 8-wave workgroups share a CU.
+template <int Q, bool CR, bool DD, int W, int MINB>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MINB == 2 ? 4 : 1)))
+void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blocks) {
   constexpr int NTH = 64 * W;
   constexpr int KG = 16 / Q;   // shifts sharing an accumulator
   constexpr int NA = 8 / KG;   // accumulators per tile
@@ -1401,67 +1403,56 @@ hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t
 
 // Pre-filter screen over a shard table; nch_max = the largest model's 512-sample
 // chunks (sizes the LDS weight table).  grid = persistent workgroups.
-int64_t screen_block_kmers(int t) {
+// w8: the 8-wave shape (at most 128 registers per wave, two workgroups per
+// CU) in place of the default (16 waves for the two-accumulator form).
+__host__ __device__ constexpr int scr_waves_w8(int Q, bool w8) { return w8 ? 8 : scr_waves(Q); }
+int64_t screen_block_kmers(int t, bool w8) {
   const int Q = scr_rows(t);
-  return (int64_t)scr_waves(Q) * 16 * scr_tiles(Q);
+  return (int64_t)scr_waves_w8(Q, w8) * 16 * scr_tiles(Q);
 }
 int screen_max_chunks() { return kScrMaxChunks; }
 int screen_rows(int t) { return scr_rows(t); }
 bool screen_count_row(int t) { return scr_count_row(t); }
 
-template <int Q, bool CR, bool DD>
-static hipError_t launch_screen_q(dim3 grid, size_t shm, hipStream_t s, const ScreenDesc* sd,
-                                  int nshards, int64_t total_blocks) {
-  hipLaunchKernelGGL((screen_i8<Q, CR, DD>), grid, dim3(64 * scr_waves(Q)), shm, s, sd,
-                     nshards, total_blocks);
-  return hipGetLastError();
+template <int Q, bool CR, bool DD, bool W8>
+static const void* screen_fn() {
+  return (const void*)screen_i8<Q, CR, DD, scr_waves_w8(Q, W8), W8 ? 2 : 1>;
 }
 
-template <bool DD>
-static hipError_t launch_screen_dd(int t, dim3 g, size_t shm, hipStream_t s, const ScreenDesc* sd,
-                                   int nshards, int64_t total_blocks) {
-  if (!scr_count_row(t))
-    return launch_screen_q<16, false, DD>(g, shm, s, sd, nshards, total_blocks);
+template <bool DD, bool W8>
+static const void* screen_kernel_t(int t) {
+  if (!scr_count_row(t)) return screen_fn<16, false, DD, W8>();
   switch (scr_rows(t)) {
-    case 4: return launch_screen_q<4, true, DD>(g, shm, s, sd, nshards, total_blocks);
-    case 8: return launch_screen_q<8, true, DD>(g, shm, s, sd, nshards, total_blocks);
-    default: return launch_screen_q<16, true, DD>(g, shm, s, sd, nshards, total_blocks);
+    case 4: return screen_fn<4, true, DD, W8>();
+    case 8: return screen_fn<8, true, DD, W8>();
+    default: return screen_fn<16, true, DD, W8>();
   }
+}
+
+// The screen instantiation for (t, dd, w8).
+static const void* screen_kernel(int t, bool dd, bool w8) {
+  if (dd) return w8 ? screen_kernel_t<true, true>(t) : screen_kernel_t<true, false>(t);
+  return w8 ? screen_kernel_t<false, true>(t) : screen_kernel_t<false, false>(t);
 }
 
 // dd: the duplicate-row check (every ScreenDesc then has dmap and mult).
 hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
-                         int nshards, int64_t total_blocks, bool dd) {
+                         int nshards, int64_t total_blocks, bool dd, bool w8) {
   if (nch_max > kScrMaxChunks || t < 1 || t > kMaxTraits) return hipErrorInvalidValue;
+  const void* fn = screen_kernel(t, dd, w8);
   dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
   const size_t shm = scr_lds_bytes(nch_max);
-  return dd ? launch_screen_dd<true>(t, g, shm, s, sd, nshards, total_blocks)
-            : launch_screen_dd<false>(t, g, shm, s, sd, nshards, total_blocks);
-}
-
-template <bool DD>
-static int screen_occ(int t, size_t shm) {
-  int nb = 0;
-  hipError_t e;
-  if (!scr_count_row(t))
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, false, DD>,
-                                                     64 * scr_waves(16), shm);
-  else if (scr_rows(t) == 4)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<4, true, DD>,
-                                                     64 * scr_waves(4), shm);
-  else if (scr_rows(t) == 8)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<8, true, DD>,
-                                                     64 * scr_waves(8), shm);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, screen_i8<16, true, DD>,
-                                                     64 * scr_waves(16), shm);
-  return (e != hipSuccess || nb < 1) ? 1 : nb;
+  void* args[] = {(void*)&sd, (void*)&nshards, (void*)&total_blocks};
+  const dim3 blk(64 * scr_waves_w8(scr_rows(t), w8));
+  return hipLaunchKernel(fn, g, blk, args, shm, s);
 }
 
 // Workgroups of the screen one CU holds at once for a model of nch chunks.
-int screen_wgs_per_cu(int t, int nch, bool dd) {
-  const size_t shm = scr_lds_bytes(nch);
-  return dd ? screen_occ<true>(t, shm) : screen_occ<false>(t, shm);
+int screen_wgs_per_cu(int t, int nch, bool dd, bool w8) {
+  int nb = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, screen_kernel(t, dd, w8), 64 * scr_waves_w8(scr_rows(t), w8), scr_lds_bytes(nch));
+  return (e != hipSuccess || nb < 1) ? 1 : nb;
 }
 
 int screen_chunk_words() { return kScrChunkWords; }
@@ -1562,13 +1553,11 @@ static hipError_t set_attr_nt() {
 hipError_t set_kernel_attributes() {
   hipError_t e = hipSuccess;
   const int scr = (int)scr_lds_bytes(kScrMaxChunks);
-  const void* scr_fns[8] = {
-      (const void*)screen_i8<4, true, false>,  (const void*)screen_i8<8, true, false>,
-      (const void*)screen_i8<16, true, false>, (const void*)screen_i8<16, false, false>,
-      (const void*)screen_i8<4, true, true>,   (const void*)screen_i8<8, true, true>,
-      (const void*)screen_i8<16, true, true>,  (const void*)screen_i8<16, false, true>};
-  for (const void* f : scr_fns)
-    if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, scr);
+  for (int t = 1; t <= kMaxTraits; ++t)
+    for (int v = 0; v < 4; ++v)
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute(screen_kernel(t, (v & 1) != 0, (v & 2) != 0),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, scr);
 #define KG_ATTR(X) \
   if (e == hipSuccess) e = set_attr_nt<X>();
   KG_ATTR(1) KG_ATTR(2) KG_ATTR(3) KG_ATTR(4) KG_ATTR(5) KG_ATTR(6) KG_ATTR(7) KG_ATTR(8)
