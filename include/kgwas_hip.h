@@ -28,6 +28,7 @@ extern "C" {
 #define KG_ERR_DEVICE 2    /* HIP runtime error (no device, launch or copy failure) */
 #define KG_ERR_STATE 3     /* call order (e.g. scan before kg_set_model) */
 #define KG_ERR_CAPACITY 4  /* more hits than `cap`; *n_out holds the required size */
+#define KG_ERR_SCRATCH 5   /* kg_scan_device_noalloc: more candidates than its fixed scratch holds */
 
 #define KG_PREC_MIXED 0    /* bits x int8-limb weights, exact int32 MFMA accumulation,
                               FP64 epilogue, FP64 refinement of every reported k-mer */
@@ -183,6 +184,21 @@ int kg_stats_device(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t row_
                     int32_t* c, double* xxM, double* xyM, double* S);
 
 int kg_last_timing(const kg_ctx* ctx, kg_timing* out);
+
+/* Allocation-free variant of kg_scan_device (no reference counterpart; for
+ * callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and errors
+ * as kg_scan_device for KG_PREC_MIXED, scanned without the pre-filter screen;
+ * it allocates no device or host memory: counters, descriptors and up to
+ * 65,536 candidate k-mers (those the bulk kernel cannot decide alone or
+ * reports) with their FP64 values live in a fixed 5.3 MB array that each GPU
+ * holds for the life of the process, and in fixed host staging.  A scan with
+ * more candidates returns KG_ERR_SCRATCH (kg_scan_device handles any number).
+ * Calls share that array and staging, so the library serialises them (one
+ * lock per process). */
+int kg_scan_device_noalloc(kg_ctx* ctx, const uint64_t* d_bits, int64_t m, int64_t row_words,
+                           const kg_scan_params* prm, kg_hit* out, int64_t cap, int64_t* n_out,
+                           int64_t* n_passed);
 
 /* Changes the context's pre-filter screen mode (KG_SCREEN_*) for later scans
  * (no reference counterpart: the reference always filters first). */

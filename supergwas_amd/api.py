@@ -17,7 +17,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libkgwas_hip.so")
 
-KG_OK, KG_ERR_ARG, KG_ERR_DEVICE, KG_ERR_STATE, KG_ERR_CAPACITY = 0, 1, 2, 3, 4
+KG_OK, KG_ERR_ARG, KG_ERR_DEVICE, KG_ERR_STATE, KG_ERR_CAPACITY, KG_ERR_SCRATCH = 0, 1, 2, 3, 4, 5
 PRECISIONS = {"mixed": 0, "fp32": 1}
 # pre-filter screen of mixed scans (KG_SCREEN_*)
 SCREEN = {"auto": 0, "off": 1, "on": 2}
@@ -83,6 +83,8 @@ SIGNATURES = {
                  ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "kg_scan_device": ([_vp, _vp, _i64, _i64, ctypes.POINTER(ScanParams), _vp, _i64,
                         ctypes.POINTER(_i64), _vp], ctypes.c_int),
+    "kg_scan_device_noalloc": ([_vp, _vp, _i64, _i64, ctypes.POINTER(ScanParams), _vp, _i64,
+                                ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "kg_add_model": ([_vp, _i32, _vp, _i32, _vp, _i32, _vp, _i32, ctypes.POINTER(_i32)],
                      ctypes.c_int),
     "kg_scan_shards": ([_vp, _vp, _i32, ctypes.POINTER(ScanParams), ctypes.c_uint32, _vp, _i64,
@@ -272,6 +274,10 @@ class KgwasHip:
     def scan_device(self, dev_ptr, m, row_words, params=None, cap=1 << 16):
         """Rows already in device memory at dev_ptr (e.g. a torch tensor's data_ptr())."""
         return self._scan(lib().kg_scan_device, dev_ptr, m, row_words, params, cap)
+
+    def scan_device_noalloc(self, dev_ptr, m, row_words, params=None, cap=1 << 16):
+        """kg_scan_device without any allocation (fixed per-GPU scratch; mixed precision)."""
+        return self._scan(lib().kg_scan_device_noalloc, dev_ptr, m, row_words, params, cap)
 
     def scan_pairs(self, dev_ptr, n_rows, row_words, params=None, i0=0, i1=None, cap=1 << 16):
         """All-pairs scan of device rows (src/kgwas_2d.cpp:258-401): hits with

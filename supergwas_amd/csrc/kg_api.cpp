@@ -17,6 +17,7 @@
 #include <functional>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -49,6 +50,7 @@ hipError_t launch_refine(hipStream_t s, const Cand* cand, const unsigned long lo
                          int64_t cap, const RefDesc* rd, double* res, int blocks,
                          const RefHost& hb);
 hipError_t launch_zero(hipStream_t s, unsigned long long* p, int n);
+hipError_t noalloc_scratch(int device, void** ptr);
 size_t setup_max_bytes();
 hipError_t launch_setup(hipStream_t s, const void* src, size_t bytes, void* dst,
                         unsigned long long* zero, int nzero);
@@ -226,6 +228,7 @@ struct kg_ctx {
   int tiles = 0;      // int8 scan: k-mer tiles per wave (0 = the kernel's default)
   int grid_mult_f32 = 8;  // FP32 scan: workgroups per CU
   bool scr_w8 = false;    // screen: the 8-wave workgroup shape (kg_autotune may pick it)
+  void* scratch = nullptr;  // device address of this GPU's allocation-free scratch array
   // pre-filter screen (int8 path): mode, and in auto mode the unscreened
   // scans left before the screen is tried again
   int screen_mode = KG_SCREEN_AUTO;
@@ -751,6 +754,7 @@ extern "C" int kg_create(kg_ctx** out, const kg_opts* opts) {
   for (auto& e : ctx->cev) KG_CK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (auto& e : ctx->ctev) KG_CK(ctx, hipEventCreate(&e));
   KG_CK(ctx, set_kernel_attributes());
+  KG_CK(ctx, noalloc_scratch(ctx->device, &ctx->scratch));
   if (const char* v = std::getenv("KG_GRID_MULT")) {
     const int g = atoi(v);
     if (g >= 1 && g <= 64) ctx->grid_mult = g;
@@ -1676,6 +1680,126 @@ extern "C" int kg_scan(kg_ctx* ctx, const uint64_t* bits, int64_t m, int64_t row
   if (n_passed)
     for (int tr = 0; tr < t; ++tr) n_passed[tr] = np[tr];
   return finish_hits(ctx, hits, out, cap, n_out);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free scan: kg_scan_device on the GPU's scratch array (counters,
+// descriptors, candidates, FP64 results) and on host staging arrays of fixed
+// size, with no device or host allocation.  Unscreened, single shard, model 0.
+// The calls of a process are serialised by one lock: they share the host
+// staging, and calls on one GPU share its scratch array (which keeps the last
+// call's contents; the counters are cleared by each call's first kernel).
+namespace {
+std::mutex g_noalloc_mu;
+Cand g_na_cand[kNoAllocCands];
+double g_na_res[4 * kNoAllocCands];
+kg_hit g_na_hits[kNoAllocCands];
+unsigned long long g_na_cnt[16];
+}  // namespace
+
+extern "C" int kg_scan_device_noalloc(kg_ctx* ctx, const uint64_t* d_bits, int64_t m,
+                                      int64_t row_words, const kg_scan_params* prm, kg_hit* out,
+                                      int64_t cap, int64_t* n_out, int64_t* n_passed) {
+  int rc = check_common(ctx, prm);
+  if (rc) return rc;
+  if (ctx->precision != KG_PREC_MIXED)
+    return fail(ctx, KG_ERR_ARG, "kg_scan_device_noalloc needs KG_PREC_MIXED");
+  const HShard sh{d_bits, m, row_words, 0, 0};
+  rc = check_shard(ctx, sh);
+  if (rc) return rc;
+  if (!ctx->scratch) return fail(ctx, KG_ERR_STATE, "no scratch array on this device");
+  const ModelDev& M = *ctx->models[0];
+  const Model& md = M.md;
+  const int t = md.t;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  KG_CK(ctx, hipSetDevice(ctx->device));
+  if (n_out) *n_out = 0;
+  int64_t np[kMaxTraits] = {};
+  int64_t nh = 0;
+  if (m > 0) {
+    char* base = static_cast<char*>(ctx->scratch);
+    auto* dc = reinterpret_cast<unsigned long long*>(base + kNoAllocCounterOff);
+    auto* d_cand = reinterpret_cast<Cand*>(base + kNoAllocCandOff);
+    auto* d_res = reinterpret_cast<double*>(base + kNoAllocResOff);
+    const double p_thr = std::pow(10.0, -prm->log10_thresh);
+    const double Ts = t_star(p_thr);
+    // descriptors go up as the argument of the kernel that also clears the counters
+    struct {
+      ShardDesc D;
+      RefDesc R;
+    } desc{};
+    fill_descs(ctx, sh, 0, *prm, Ts, p_thr, false, false, dc + 1, 0, desc.D, desc.R);
+    static_assert(sizeof(desc) % 8 == 0, "descriptor upload in words");
+    auto* d_desc = base + kNoAllocDescOff;
+    const int64_t bk = i8_block_kmers(M.NT, ctx->limbs, t, ctx->tiles);
+    const int64_t blocks = (m + bk - 1) / bk;
+    ScanOut o{};
+    o.cand = d_cand;
+    o.counters = dc;
+    o.cap = kNoAllocCands;
+    KG_CK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    KG_CK(ctx, launch_setup(ctx->stream, &desc, sizeof(desc), d_desc, dc, 1 + t));
+    KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    KG_CK(ctx, launch_scan_i8(M.NT, ctx->limbs, t, ctx->tiles, ctx->cus * ctx->grid_mult,
+                              ctx->stream, reinterpret_cast<const ShardDesc*>(d_desc), 1,
+                              blocks, o, nullptr));
+    KG_CK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    KG_CK(ctx, launch_refine(ctx->stream, d_cand, dc, kNoAllocCands,
+                             reinterpret_cast<const RefDesc*>(d_desc + sizeof(ShardDesc)), d_res,
+                             ctx->cus * 2, RefHost{}));
+    KG_CK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+    KG_CK(ctx, hipMemcpyAsync(g_na_cnt, dc, sizeof(unsigned long long) * (1 + t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    KG_CK(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t nc = (int64_t)g_na_cnt[0];
+    if (nc > kNoAllocCands)
+      return fail(ctx, KG_ERR_SCRATCH,
+                  "kg_scan_device_noalloc: " + std::to_string(nc) + " candidates exceed its " +
+                      std::to_string(kNoAllocCands) + " (use kg_scan_device)");
+    if (nc > 0) {
+      KG_CK(ctx, hipMemcpyAsync(g_na_cand, d_cand, sizeof(Cand) * nc, hipMemcpyDeviceToHost,
+                                ctx->stream));
+      KG_CK(ctx, hipMemcpyAsync(g_na_res, d_res, sizeof(double) * 4 * nc, hipMemcpyDeviceToHost,
+                                ctx->stream));
+      KG_CK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    for (int tr = 0; tr < t; ++tr) np[tr] = (int64_t)g_na_cnt[1 + tr];
+    for (int64_t i = 0; i < nc; ++i) {
+      const Cand& cd = g_na_cand[i];
+      Decided d = decide(md, cd.idx, cd.trait, (int)g_na_res[4 * i], g_na_res[4 * i + 1],
+                         g_na_res[4 * i + 2], g_na_res[4 * i + 3], true, *prm);
+      if ((cd.flags & kPfAmb) && d.pass) ++np[cd.trait];
+      if (d.hit) {
+        d.h.shard = 0;
+        g_na_hits[nh++] = d.h;
+      }
+    }
+    std::sort(g_na_hits, g_na_hits + nh, [](const kg_hit& a, const kg_hit& b) {
+      return a.idx != b.idx ? a.idx < b.idx : a.trait < b.trait;
+    });
+    float ms = 0.f;
+    KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+    ctx->timing = kg_timing{};
+    ctx->timing.bulk_ms = ms;
+    KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+    ctx->timing.refine_ms = ms;
+    KG_CK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+    ctx->timing.total_ms = ms;
+    ctx->timing.candidates = nc;
+    ctx->timing.kmers = m;
+    ctx->timing.survivors = m;
+  }
+  if (n_passed)
+    for (int tr = 0; tr < t; ++tr) n_passed[tr] = np[tr];
+  if (n_out) *n_out = nh;
+  if (nh > cap)
+    return fail(ctx, KG_ERR_CAPACITY,
+                "hit capacity " + std::to_string(cap) + " < " + std::to_string(nh));
+  if (nh > 0) {
+    if (!out) return fail(ctx, KG_ERR_ARG, "null hit buffer");
+    memcpy(out, g_na_hits, sizeof(kg_hit) * (size_t)nh);
+  }
+  return KG_OK;
 }
 
 extern "C" int kg_host_alloc(kg_ctx* ctx, size_t bytes, void** h_ptr) {

@@ -166,6 +166,19 @@ struct RefHost {
   int ncnt, pad;
 };
 
+// Allocation-free scan (kg_scan_device_noalloc): one scratch array per GPU
+// holding the counters, the shard and refinement descriptors, and up to
+// kNoAllocCands candidate records with their FP64 results.
+constexpr int64_t kNoAllocCands = 65536;
+constexpr size_t kNoAllocCounterOff = 0;          // 16 u64 counters
+constexpr size_t kNoAllocDescOff = 256;           // ShardDesc, then RefDesc
+constexpr size_t kNoAllocCandOff = 4096;          // Cand[kNoAllocCands]
+constexpr size_t kNoAllocResOff = kNoAllocCandOff + sizeof(Cand) * (size_t)kNoAllocCands;
+constexpr size_t kNoAllocScratchBytes = kNoAllocResOff + 4 * sizeof(double) * (size_t)kNoAllocCands;
+constexpr int kNoAllocMaxDevices = 64;
+static_assert(kNoAllocDescOff + sizeof(ShardDesc) + sizeof(RefDesc) <= kNoAllocCandOff,
+              "descriptor slot of the allocation-free scratch");
+
 // Epilogue constants of the plain FP32 kernel.
 struct EpiF32 {
   int n, p, t, NU;

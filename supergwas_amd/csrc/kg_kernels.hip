@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 
 #include "kg_internal.h"
 #include "synth_core.h"
@@ -1574,6 +1575,31 @@ hipError_t launch_pair_masks(hipStream_t s, const uint64_t* bits, int64_t rw,
   hipLaunchKernelGGL(pair_masks, dim3((unsigned)blocks), dim3(256), 0, s, bits, rw, mask, W, i0,
                      cnt, out);
   return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// Scratch of the allocation-free scan (kg_scan_device_noalloc): counters,
+// descriptor tables, candidate records and their FP64 results, in a
+// file-scope device array that each GPU holds for the life of the process.
+// The host never passes `scratch` itself to a kernel: its host-side name is a
+// stand-in, not a device address.  noalloc_scratch() looks the address up with
+// hipGetSymbolAddress once per device (with that device current) and every
+// host use goes through that address.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(int device, void** ptr) {
+  static std::mutex mu;
+  static void* addr[kNoAllocMaxDevices] = {};
+  if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
 }
 
 hipError_t launch_zero(hipStream_t s, unsigned long long* p, int n) {
