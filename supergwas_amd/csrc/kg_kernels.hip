@@ -456,53 +456,49 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     const int rme = g % R;
     const bool primary = g < R;
     const bool est = ep.stat_est != 0;  // full-statistic mode: error estimate per k-mer
-    double pzz[R], pz0[R], pzE[R], pzh[R], pS[R][TT], psv[R][TT];
+    // this lane's column values z (FP64) of every tile, after which the limb
+    // sums are dead: the per-trait passes below then need R registers of
+    // partial sums each, not R x t
+    double zc[R][J];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      pzz[r] = pz0[r] = pzE[r] = pzh[r] = 0.0;
-#pragma unroll
-      for (int tr = 0; tr < TT; ++tr) pS[r][tr] = psv[r][tr] = 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const double* cr = colrec_lds + (4 * j + g) * kColRec;
+    for (int jc = 0; jc < J; ++jc) {
+      const double* cr = colrec_lds + (4 * jc + g) * kColRec;
       const double w0 = cr[0], w1 = cr[1], w2 = cr[2];
-      const double cE = cr[11], ch2 = cr[12];
-      double kc[TT];
-#pragma unroll
-      for (int tr = 0; tr < TT; ++tr) kc[tr] = cr[3 + tr];
-      const int b = 4 * j + g;
+      const int sl = L * jc;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int sl = L * j;
         const int a0 = acc[r][sl >> 2][sl & 3];
-        double z;
         if (L == 3) {
           // limbs 1 and 2 merged exactly: |a| <= 127 n < 2^23 for n <= 60000, so
           // the 24-bit multiply-add is exact
           const int a12 = __mul24(acc[r][(sl + 1) >> 2][(sl + 1) & 3], 254) +
                           acc[r][(sl + 2) >> 2][(sl + 2) & 3];
-          z = fma(w2, (double)a12, w0 * (double)a0);
+          zc[r][jc] = fma(w2, (double)a12, w0 * (double)a0);
         } else if (L == 2) {
-          z = fma(w1, (double)acc[r][(sl + 1) >> 2][(sl + 1) & 3], w0 * (double)a0);
+          zc[r][jc] = fma(w1, (double)acc[r][(sl + 1) >> 2][(sl + 1) & 3], w0 * (double)a0);
         } else {
-          z = w0 * (double)a0;
+          zc[r][jc] = w0 * (double)a0;
         }
-        if (j < JSP) {  // may be a residual column (b < t) or column d (b == t)
-          const double zs = (b < t) ? 0.0 : z;
-          pzz[r] = fma(zs, zs, pzz[r]);
-          pzE[r] = fma(zs, cE, pzE[r]);
-          if (est) pzh[r] = fma(zs * zs, ch2, pzh[r]);
-          if (b == t) pz0[r] = z;
+      }
+    }
+    // trait-independent sums over the structure columns (b >= t)
+    double pzz[R], pz0[R], pzE[R], pzh[R];
 #pragma unroll
-          for (int tr = 0; tr < TT; ++tr) psv[r][tr] += (b == tr) ? z : 0.0;
-        } else {
-          pzz[r] = fma(z, z, pzz[r]);
-          pzE[r] = fma(z, cE, pzE[r]);
-          if (est) pzh[r] = fma(z * z, ch2, pzh[r]);
-        }
+    for (int r = 0; r < R; ++r) pzz[r] = pz0[r] = pzE[r] = pzh[r] = 0.0;
 #pragma unroll
-        for (int tr = 0; tr < TT; ++tr) pS[r][tr] = fma(z, kc[tr], pS[r][tr]);
+    for (int jc = 0; jc < J; ++jc) {
+      const double* cr = colrec_lds + (4 * jc + g) * kColRec;
+      const double cE = cr[11], ch2 = cr[12];
+      const int b = 4 * jc + g;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double z = zc[r][jc];
+        // columns j < JSP may be a residual column (b < t) or column d (b == t)
+        const double zs = (jc < JSP && b < t) ? 0.0 : z;
+        pzz[r] = fma(zs, zs, pzz[r]);
+        pzE[r] = fma(zs, cE, pzE[r]);
+        if (est) pzh[r] = fma(zs * zs, ch2, pzh[r]);
+        if (jc < JSP && b == t) pz0[r] = z;
       }
     }
     const int c = (int)group_scatter<R>(cnt);
@@ -512,18 +508,6 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     const double kzz = group_scatter<R>(pzz) + cn * fma(cn, ep.sum_E2, 2.0 * kzE);
     const double kz0 = group_scatter<R>(pz0) + cn * ep.E_d;
     const double kzh = est ? group_scatter<R>(pzh) : 0.0;
-    double kS[TT], ksv[TT];
-#pragma unroll
-    for (int tr = 0; tr < TT; ++tr) {
-      double vS[R], vsv[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        vS[r] = pS[r][tr];
-        vsv[r] = psv[r][tr];
-      }
-      kS[tr] = group_scatter<R>(vS);
-      ksv[tr] = group_scatter<R>(vsv);
-    }
 
     const int64_t kidx = kbase + 16 * rme + col;
     const bool valid = primary && (kidx < D.m);
@@ -551,8 +535,23 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 #pragma unroll
     for (int tr = 0; tr < TT; ++tr) {
       if (tr < t) {
+        // S = z'b' + x'r_c (column weights k_tr) and x'r_c of this trait
+        double vS[R], vsv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) vS[r] = vsv[r] = 0.0;
+#pragma unroll
+        for (int jc = 0; jc < J; ++jc) {
+          const double kc = colrec_lds[(4 * jc + g) * kColRec + 3 + tr];
+          const bool own = jc < JSP && 4 * jc + g == tr;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            vS[r] = fma(zc[r][jc], kc, vS[r]);
+            if (own) vsv[r] = zc[r][jc];
+          }
+        }
         const double* tc = trc_lds + tr * kTrcRec;
-        const double S = kS[tr] + cn * tc[6], sv = ksv[tr] + cn * tc[5];
+        const double S = group_scatter<R>(vS) + cn * tc[6];
+        const double sv = group_scatter<R>(vsv) + cn * tc[5];
         const double rbar = tc[0];
         const double es = cm * tc[4];
         double bS = cm * ep.sum_hbp[tr] + es;
