@@ -30,6 +30,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """CPU model and online logical CPUs of this host (for the CPU baseline records)."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,6 +58,9 @@ def main():
     ap.add_argument("--limbs", type=int, default=3)
     ap.add_argument("--profile-only", action="store_true",
                     help="only the mixed scans (for rocprofv3 counter runs)")
+    ap.add_argument("--profile-c5", action="store_true",
+                    help="only HBM-resident scans of the n=4096, p=64, 8-trait workload (--m rows, "
+                         "--steps timed scans; for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="copy this many bytes device-to-device and exit (counter calibration)")
     ap.add_argument("--legs", default="c2,c4,c5,pairs,ingest,dedup",
@@ -53,6 +70,8 @@ def main():
                          "scan of 20,000 rows), ingest (ASCII columns -> packed rows on the "
                          "host), dedup (sorted batches with runs of equal rows); '' for none")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--c5-stream-kmers", type=int, default=1_000_000_000,
+                    help="k-mers the c5 leg streams from page-locked host memory")
     ap.add_argument("--screen", default="auto", choices=("auto", "on", "off"),
                     help="pre-filter screen of the headline scans (auto: screen while it pays)")
     args = ap.parse_args()
@@ -75,6 +94,10 @@ def main():
         log(f"[bench] copied {args.copy_calibration} bytes device-to-device x3")
         return
 
+    if args.profile_c5:
+        profile_c5(args, synth)
+        return
+
     cfg = dict(synth.CONFIGS["mixed"])
     cfg["m"] = args.m
     t0 = time.time()
@@ -95,7 +118,7 @@ def main():
     if not args.no_autotune:
         t_at = time.time()
         tune = mixed.autotune(2_000_000)
-        log(f"[bench] autotune: {tune[0]} tiles/wave, {tune[1]} workgroups/CU ({time.time() - t_at:.1f}s)")
+        log(f"[bench] autotune: {tune} ({time.time() - t_at:.1f}s)")
 
     # ---- mixed precision: warmup + timed steps ----
     for _ in range(args.warmup):
@@ -190,6 +213,7 @@ def main():
         inside = res["keep"] < es   # error on the part of the sample the oracle covers
         err_c = rel_err_T(Tc_k[inside], ref["T"][res["keep"][inside]])
         cpu = {"value": cs / cel, "unit": "k-mer tests/s", "cores": threads, "kind": "port",
+               **host_cpu(),
                "sample": f"{cs} k-mers of the same workload (n=2048, p=32), in-memory float "
                          f"matrix; pre-filter {res['t_stage'][0]:.2f}s + LA "
                          f"{res['t_stage'][1]:.2f}s + emit {res['t_stage'][2]:.2f}s",
@@ -262,7 +286,7 @@ def main():
         "cpu_baseline": cpu,
     }
     if tune is not None:
-        line["config"]["autotune"] = {"tiles_per_wave": tune[0], "workgroups_per_cu": tune[1]}
+        line["config"]["autotune"] = tune
     dev.free()
     legs = [x for x in args.legs.split(",") if x]
     if legs:
@@ -272,6 +296,28 @@ def main():
         line["legs"][name] = LEGS[name](args, synth, ds)
         log(f"[bench] leg {name} done ({time.time() - t_leg:.1f}s)")
     print(json.dumps(line), flush=True)
+
+
+def profile_c5(args, synth):
+    """HBM-resident scans of the n=4096, p=64, 8-trait workload only (the c5 leg's
+    kernel-only part), for profiler runs: --warmup untimed and --steps timed scans of --m rows."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    ds = synth.make_dataset(seed=5, n_struct=1024, **dict(synth.CONFIGS["multitrait"], m=args.m))
+    rw = synth.row_words_for(ds.n_acc)
+    k = KgwasHip(precision="mixed", screen=args.screen)
+    k.set_model(ds.n_acc, ds.idxs, ds.U, ds.Y)
+    dev = k.alloc(args.m * rw * 8)
+    k.synth_bits(dev.ptr, ds, 0, args.m, rw)
+    prm = ScanParams(0.2, 6.0, 0.0)
+    for _ in range(args.warmup):
+        k.scan_device(dev.ptr, args.m, rw, prm)
+    for _ in range(args.steps):
+        h, npass = k.scan_device(dev.ptr, args.m, rw, prm)
+        tm = k.timing()
+        log(f"[bench] c5 profile: m={args.m} screen {tm['screen_ms']:.3f} ms, full {tm['bulk_ms']:.3f} "
+            f"ms, refine {tm['refine_ms']:.3f} ms, survivors {tm['survivors']}, hits {len(h)}")
+    dev.free()
+    k.close()
 
 
 def leg_c2(args, synth, ds_c3):
@@ -314,6 +360,29 @@ def leg_c2(args, synth, ds_c3):
         k.close()
     out["checksum_rel_diff_T"] = abs(out["fp32"]["checksum_sum_T"] - out["mixed"]["checksum_sum_T"]) / \
         max(abs(out["mixed"]["checksum_sum_T"]), 1.0)
+    # both paths' statistics against the FP64 oracle on the first batch of 1e6
+    bits = ds.fill_bits(0, bm)
+    ref = oracle.scan_f64(bits, ds.idxs, ds.y, ds.U, 0.2)
+    sT64 = float(ref["T"].sum())
+    out["fp64_first_batch"] = {"kmers": bm, "sum_T": sT64, "sum_c": int(ref["c"].astype(np.int64).sum()),
+                               "hits_p_lt_1e-5": int(np.sum(ref["passed"] & (ref["p"] <= 1e-5)))}
+    for prec in ("fp32", "mixed"):
+        k = KgwasHip(precision=prec)
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        dbits = k.to_device(bits)
+        st = k.stats_device(dbits.ptr, bm, rw)
+        dbits.free()
+        T = T_from(st["xxM"], st["xyM"][:, 0], yy, ds.n)
+        h, _ = k.scan(bits, prm)
+        exp = np.nonzero(ref["passed"] & (ref["p"] <= 1e-5))[0]
+        out[prec]["first_batch_vs_fp64"] = {
+            "sum_T_rel_diff": abs(float(T.sum()) - sT64) / sT64,
+            "stat_max_rel_err": float(np.max(np.abs(T - ref["T"]) / np.maximum(ref["T"], 1.0))),
+            "counts_bit_exact": bool(np.array_equal(st["c"], ref["c"])),
+            "hit_flips": int(len(np.setxor1d(exp, h["idx"])))}
+        k.close()
+    log(f"[bench] c2 vs FP64 (first batch): fp32 {out['fp32']['first_batch_vs_fp64']}, "
+        f"mixed {out['mixed']['first_batch_vs_fp64']}")
     out["checksum_c_equal"] = out["fp32"]["checksum_sum_c"] == out["mixed"]["checksum_sum_c"]
     return out
 
@@ -328,7 +397,7 @@ def leg_c4(args, synth, ds):
     total = int(sizes.sum())
     rw = synth.row_words_for(ds.n_acc)
     k = KgwasHip(precision="mixed")
-    mids = []
+    mids, models = [], []
     t0 = time.time()
     for s in range(len(sizes)):
         keep = np.ones(ds.n, bool)
@@ -337,6 +406,7 @@ def leg_c4(args, synth, ds):
         idxs = ds.idxs[keep]
         U, _, _ = np.linalg.svd(ds.U[keep], full_matrices=False)
         y = ds.y[keep]
+        models.append((idxs, U, y))
         if s == 0:
             k.set_model(ds.n_acc, idxs, U, y)
             mids.append(0)
@@ -351,7 +421,7 @@ def leg_c4(args, synth, ds):
     prm = ScanParams(0.2, 6.0, 0.0)
     out = {"workload": f"{len(sizes)} shards log-uniform in [1e3, 2e6], total {total} k-mers, "
                        "n=2048, p=32, per-shard masks 0-15% missing",
-           "autotune": {"tiles_per_wave": tune[0], "workgroups_per_cu": tune[1]}}
+           "autotune": tune}
     ref = None
     runs = (("persistent", 0, "auto"), ("graph", SHARDS_GRAPH, "auto"),
             ("persistent_unscreened", 0, "off"), ("per_shard", SHARDS_PER_LAUNCH, "off"))
@@ -375,6 +445,22 @@ def leg_c4(args, synth, ds):
                      "hits": int(len(hits)), "matches_persistent": same}
         log(f"[bench] c4 {name}: {total / min(ts):.3e} k-mers/s (device {min(ks):.1f} ms,"
             f" wall {1e3 * min(ts):.1f} ms), hits {len(hits)}, same {same}")
+    # the persistent launch's per-shard counts and hits against the FP64 oracle on the
+    # eight smallest shards (each with its own mask, U and phenotype)
+    import oracle
+    h0, n0 = ref
+    checked, count_eq, hit_eq = 0, 0, 0
+    for s in np.argsort(sizes)[:8]:
+        m, o = int(sizes[s]), int(off[s])
+        bits = dev.to_host(np.uint64, (m, rw), nbytes=m * rw * 8, offset=o * rw * 8)
+        idxs, U, y = models[s]
+        r = oracle.scan_f64(bits, idxs, y, U, 0.2)
+        exp = np.nonzero(r["passed"] & (r["p"] <= 1e-6))[0]
+        checked += 1
+        count_eq += int(n0[s, 0] == r["n_passed"])
+        hit_eq += int(np.array_equal(h0[h0["shard"] == s]["idx"], exp))
+    out["oracle_check"] = {"shards": checked, "counts_equal": count_eq, "hits_equal": hit_eq}
+    log(f"[bench] c4 oracle check on {checked} shards: counts equal {count_eq}, hits equal {hit_eq}")
     out["speedup_persistent_vs_per_shard"] = (out["per_shard"]["wall_ms"] /
                                               out["persistent_unscreened"]["wall_ms"])
     out["speedup_screen"] = out["persistent_unscreened"]["wall_ms"] / out["persistent"]["wall_ms"]
@@ -385,13 +471,18 @@ def leg_c4(args, synth, ds):
 
 def leg_c5(args, synth, ds_c3):
     """n=4096, p=64, 8 traits (correlation 0.2, 100 causal each) in one pass: kernel-only rate on
-    HBM-resident rows and end-to-end rate streaming from page-locked host memory (64 MiB chunks
-    on a copy stream)."""
+    HBM-resident rows, every trait's statistic against the FP64 oracle on a sample, and the
+    end-to-end rate of a 1e9-k-mer stream from page-locked host memory (64 MiB chunks on a copy
+    stream): kg_scan over a 4 GB page-locked ring of 8e6 rows, 125 passes, every pass copying
+    its rows over the host link again."""
+    import oracle
     from supergwas_amd.api import KgwasHip, ScanParams
+    from tests.helpers import T_from, rel_err_T
     cfg = dict(synth.CONFIGS["multitrait"])
     cfg["m"] = 10_000_000
     ds = synth.make_dataset(seed=5, n_struct=1024, **cfg)
     rw = synth.row_words_for(ds.n_acc)
+    t = ds.Y.shape[1]
     k = KgwasHip(precision="mixed")
     k.set_model(ds.n_acc, ds.idxs, ds.U, ds.Y)
     prm = ScanParams(0.2, 6.0, 0.0)
@@ -407,31 +498,56 @@ def leg_c5(args, synth, ds_c3):
         bulk.append(tm["bulk_ms"] + tm["screen_ms"])
     surv = tm["survivors"]
     kb = min(bulk) / 1e3
+    # statistic of every trait against FP64 on a sample of the same rows
+    es = min(args.err_sample // 2, mk)
+    sample = dev.to_host(np.uint64, (es, rw), nbytes=es * rw * 8)
+    st = k.stats_device(dev.ptr, es, rw)
+    refined = k.timing()["stat_refined"]
+    hs, nps = k.scan_device(dev.ptr, es, rw, prm)
+    err, flips, counts_exact = 0.0, 0, True
+    for tr in range(t):
+        ref = oracle.scan_f64(sample, ds.idxs, ds.Y[:, tr], ds.U, 0.2)
+        counts_exact &= bool(np.array_equal(st["c"], ref["c"]))
+        Tb = T_from(st["xxM"], st["xyM"][:, tr], oracle.yyM(ds.Y[:, tr], ds.U), ds.n)
+        err = max(err, float(rel_err_T(Tb, ref["T"]).max()))
+        exp = np.nonzero(ref["passed"] & (ref["p"] <= 1e-6))[0]
+        got = hs[hs["trait"] == tr]["idx"]
+        flips += int(len(np.setxor1d(exp, got))) + abs(int(nps[tr]) - ref["n_passed"])
     host = k.host_alloc((mh, rw))
     from supergwas_amd.api import lib
     k._check(lib().kg_memcpy(k._h, host.ctypes.data, dev.ptr, mh * rw * 8, 1))
     dev.free()
-    walls, copies, kern = [], [], []
-    for _ in range(3):
-        hits, npass = k.scan(host, prm)
+    k.scan(host, prm)  # warm-up of the chunk buffers
+    passes = max(1, int(round(args.c5_stream_kmers / mh)))
+    t0 = time.perf_counter()
+    copies, kern, nhits, npass = 0.0, 0.0, 0, np.zeros(t, np.int64)
+    for _ in range(passes):
+        hits, np_ = k.scan(host, prm)
         tm = k.timing()
-        walls.append(tm["wall_ms"])
-        copies.append(tm["copy_ms"])
-        kern.append(tm["bulk_ms"] + tm["screen_ms"])
+        copies += tm["copy_ms"]
+        kern += tm["bulk_ms"] + tm["screen_ms"]
+        nhits += len(hits)
+        npass += np_
+    w = time.perf_counter() - t0
     k.host_free(host)
-    w = min(walls) / 1e3
+    total = passes * mh
     out = {"workload": "n=4096, p=64, t=8 (corr 0.2, 100 causal each), 3 int8 limbs",
-           "autotune": {"tiles_per_wave": tune[0], "workgroups_per_cu": tune[1]},
+           "autotune": tune,
            "kernel_only": {"m": mk, "kmers_per_s": mk / kb, "GBps": mk * rw * 8 / kb / 1e9,
-                           "trait_tests_per_s": 8 * mk / kb, "survivors": int(surv)},
-           "streamed": {"m_per_pass": mh, "kmers_per_s": mh / w, "GBps": mh * rw * 8 / w / 1e9,
-                        "copy_ms": min(copies), "kernel_ms": min(kern), "wall_ms": 1e3 * w,
-                        "chunk_MiB": 64, "host_memory": "page-locked (kg_host_alloc)",
-                        "hits": int(len(hits)), "passed_per_trait": [int(x) for x in npass],
-                        "projected_1e9_kmers_s": 1e9 / (mh / w)}}
-    log(f"[bench] c5 kernel {mk / kb:.3e} k-mers/s ({mk * rw * 8 / kb / 1e9:.0f} GB/s); streamed "
-        f"{mh / w:.3e} k-mers/s ({mh * rw * 8 / w / 1e9:.1f} GB/s, copy {min(copies):.0f} ms, "
-        f"kernel {min(kern):.0f} ms, wall {1e3 * w:.0f} ms)")
+                           "trait_tests_per_s": t * mk / kb, "survivors": int(surv)},
+           "error_vs_fp64": {"sample": es, "traits": t, "stat_max_rel_err": err,
+                             "flips": flips, "counts_bit_exact": counts_exact,
+                             "stat_refined_in_fp64": int(refined)},
+           "streamed": {"kmers": total, "passes": passes, "kmers_per_pass": mh,
+                        "kmers_per_s": total / w, "GBps": total * rw * 8 / w / 1e9,
+                        "copy_ms": copies, "kernel_ms": kern, "wall_ms": 1e3 * w,
+                        "chunk_MiB": 64, "host_memory": "page-locked ring of 8e6 rows "
+                        "(kg_host_alloc), scanned by kg_scan once per pass",
+                        "hits": int(nhits), "passed_per_trait": [int(x) for x in npass]}}
+    log(f"[bench] c5 kernel {mk / kb:.3e} k-mers/s ({mk * rw * 8 / kb / 1e9:.0f} GB/s), stat err "
+        f"{err:.2e} (refined {refined}), flips {flips}, counts exact {counts_exact}; streamed "
+        f"{total:.2e} k-mers at {total / w:.3e} k-mers/s ({total * rw * 8 / w / 1e9:.1f} GB/s, "
+        f"copy {copies:.0f} ms, kernel {kern:.0f} ms, wall {1e3 * w:.0f} ms)")
     k.close()
     return out
 
@@ -485,7 +601,7 @@ def leg_pairs(args, synth, ds):
                             initial=0.0)) if same else None
         rp = R * (R - 1) // 2
         out["cpu_baseline"] = {"value": rp / tc, "unit": "pairs/s", "cores": os.cpu_count(),
-                               "kind": "port", "sample": f"all {rp} pairs of the first {R} rows "
+                               "kind": "port", **host_cpu(), "sample": f"all {rp} pairs of the first {R} rows "
                                "(FP64 oracle, OpenMP over i)"}
         out["parity_first_rows"] = {"rows": R, "hits": int(len(sub)), "identical_pairs": same,
                                     "T_max_rel_err_vs_fp64": terr}
@@ -546,7 +662,7 @@ def leg_ingest(args, synth, ds):
            "threads": threads, "pack_s": min(packs), "pack_GBps": ascii_gb / min(packs),
            "process_wall_s": min(walls), "rows_bit_exact": exact,
            "cpu_baseline": {"value": ascii_gb / min(tref), "unit": "GB/s of ASCII",
-                            "cores": threads, "kind": "port",
+                            "cores": threads, "kind": "port", **host_cpu(),
                             "sample": "load_matrix_rows restated (float n x N), same columns",
                             "seconds": min(tref)}}
     log(f"[bench] ingest: pack {min(packs):.3f} s ({ascii_gb / min(packs):.2f} GB/s), process "

@@ -217,13 +217,30 @@ int kg_set_screen(kg_ctx* ctx, int32_t mode);
  * (the reference computes every k-mer). */
 int kg_set_dedup(kg_ctx* ctx, int32_t on);
 
-/* Launch-shape autotuner (no reference counterpart; the reference has one
- * code path): times the int8 bulk scan of `sample` random rows with model 0
- * for each candidate k-mer tiles per wave (k-mers per workgroup) and
- * persistent workgroups per CU, keeps the fastest for later scans on ctx and
- * returns it.  The default shape is timed first and is replaced only by one
- * that is more than 2% faster.  A no-op for KG_PREC_FP32. */
-int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t* grid_mult);
+/* Launch shape of the int8 scan (no reference counterpart; the reference has
+ * one code path).  Every shape gives identical results. */
+typedef struct kg_tune {
+  int32_t tiles_per_wave;      /* full kernel: k-mer tiles of 16 per wave, 4 or 2 (0: the
+                                  model's default, 4 for one trait and at most 6 weight
+                                  tiles, else 2; 2 is available at 3 limbs) */
+  int32_t workgroups_per_cu;   /* full kernel: persistent 8-wave workgroups per CU (1-64) */
+  int32_t screen_wgs_per_cu;   /* pre-filter screen: 1 (one 16-wave workgroup per CU for one
+                                  trait, one 8-wave workgroup otherwise) or 2 (two 8-wave
+                                  workgroups of at most 128 registers per wave) */
+  int32_t kmers_per_workgroup; /* out: k-mers per full-kernel workgroup block */
+  int32_t screen_kmers_per_workgroup; /* out: k-mers per screen workgroup block */
+  int32_t reserved[3];
+} kg_tune;
+
+/* Autotuner: times the int8 bulk scan of `sample` random rows with model 0 for
+ * each candidate (k-mer tiles per wave x persistent workgroups per CU for the
+ * full kernel, then the screen's workgroups per CU), keeps the fastest for
+ * later scans on ctx and returns it in *out.  The default shape is timed first
+ * and is replaced only by one that is more than 2% faster.  A no-op for
+ * KG_PREC_FP32.  kg_set_tune sets a shape explicitly; kg_get_tune reads it. */
+int kg_autotune(kg_ctx* ctx, int64_t sample, kg_tune* out);
+int kg_set_tune(kg_ctx* ctx, const kg_tune* shape);
+int kg_get_tune(const kg_ctx* ctx, kg_tune* out);
 
 /* Device memory on the context's device for callers without their own
  * allocator (tests, benchmarks, ctypes hosts).  kg_memcpy direction:

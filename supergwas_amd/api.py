@@ -60,6 +60,17 @@ SHARDS_PER_LAUNCH = 1
 SHARDS_GRAPH = 2
 
 
+class Tune(ctypes.Structure):
+    """kg_tune: launch shape of the int8 scan."""
+    _fields_ = [("tiles_per_wave", ctypes.c_int32), ("workgroups_per_cu", ctypes.c_int32),
+                ("screen_wgs_per_cu", ctypes.c_int32), ("kmers_per_workgroup", ctypes.c_int32),
+                ("screen_kmers_per_workgroup", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
 class _Timing(ctypes.Structure):
     _fields_ = [("bulk_ms", ctypes.c_double), ("refine_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("candidates", ctypes.c_int64),
@@ -98,7 +109,9 @@ SIGNATURES = {
     "kg_dev_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
     "kg_dev_free": ([_vp, _vp], ctypes.c_int),
     "kg_host_alloc": ([_vp, ctypes.c_size_t, ctypes.POINTER(_vp)], ctypes.c_int),
-    "kg_autotune": ([_vp, _i64, ctypes.POINTER(_i32), ctypes.POINTER(_i32)], ctypes.c_int),
+    "kg_autotune": ([_vp, _i64, ctypes.POINTER(Tune)], ctypes.c_int),
+    "kg_set_tune": ([_vp, ctypes.POINTER(Tune)], ctypes.c_int),
+    "kg_get_tune": ([_vp, ctypes.POINTER(Tune)], ctypes.c_int),
     "kg_host_free": ([_vp, _vp], ctypes.c_int),
     "kg_memcpy": ([_vp, _vp, _vp, ctypes.c_size_t, _i32], ctypes.c_int),
     "kg_synth_bits": ([_vp, _vp, ctypes.c_uint64, _i64, _i64, _i32, _i64, _vp, _vp, _i32, _i32],
@@ -343,11 +356,21 @@ class KgwasHip:
                     screen_ms=t.screen_ms, survivors=t.survivors, stat_refined=t.stat_refined)
 
     def autotune(self, sample=2_000_000):
-        """Pick k-mer tiles per wave and workgroups per CU by timing `sample` random
-        rows with model 0; returns (tiles, grid_mult)."""
-        tl, gm = _i32(), _i32()
-        self._check(lib().kg_autotune(self._h, int(sample), ctypes.byref(tl), ctypes.byref(gm)))
-        return tl.value, gm.value
+        """Pick the launch shape (k-mer tiles per wave, full-kernel and screen workgroups
+        per CU) by timing `sample` random rows with model 0; returns it as a dict."""
+        t = Tune()
+        self._check(lib().kg_autotune(self._h, int(sample), ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tune(self, tiles_per_wave=0, workgroups_per_cu=1, screen_wgs_per_cu=1):
+        """Set the launch shape explicitly (kg_set_tune)."""
+        t = Tune(tiles_per_wave, workgroups_per_cu, screen_wgs_per_cu)
+        self._check(lib().kg_set_tune(self._h, ctypes.byref(t)))
+
+    def get_tune(self):
+        t = Tune()
+        self._check(lib().kg_get_tune(self._h, ctypes.byref(t)))
+        return t.as_dict()
 
     def host_alloc(self, shape, dtype=np.uint64):
         """Page-locked host array (numpy view); kg_scan streams such rows without a

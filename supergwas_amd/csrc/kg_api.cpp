@@ -1912,10 +1912,47 @@ extern "C" int kg_scan_pairs(kg_ctx* ctx, const uint64_t* d_bits, int64_t N, int
   return finish_hits(ctx, hits, out, cap, n_out);
 }
 
-// Launch-shape autotuner: times the bulk scan of `sample` synthetic rows
-// (random bits) with model 0 for each candidate (k-mer tiles per wave x
-// persistent workgroups per CU) and keeps the fastest for this context.
-extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t* grid_mult) {
+static void fill_tune(const kg_ctx* ctx, kg_tune* out) {
+  *out = kg_tune{};
+  out->tiles_per_wave = ctx->tiles;
+  out->workgroups_per_cu = ctx->grid_mult;
+  out->screen_wgs_per_cu = ctx->scr_w8 ? 2 : 1;
+  if (!ctx->models.empty() && ctx->precision == KG_PREC_MIXED) {
+    const ModelDev& M = *ctx->models[0];
+    out->tiles_per_wave = i8_tiles_for(M.NT, ctx->limbs, M.md.t, ctx->tiles);
+    out->kmers_per_workgroup = (int32_t)i8_block_kmers(M.NT, ctx->limbs, M.md.t, ctx->tiles);
+    out->screen_kmers_per_workgroup = (int32_t)screen_block_kmers(M.md.t, ctx->scr_w8);
+  }
+}
+
+extern "C" int kg_get_tune(const kg_ctx* ctx, kg_tune* out) {
+  if (!ctx || !out) return KG_ERR_ARG;
+  fill_tune(ctx, out);
+  return KG_OK;
+}
+
+extern "C" int kg_set_tune(kg_ctx* ctx, const kg_tune* shape) {
+  if (!ctx || !shape) return KG_ERR_ARG;
+  if (shape->tiles_per_wave != 0 && shape->tiles_per_wave != 2 && shape->tiles_per_wave != 4)
+    return fail(ctx, KG_ERR_ARG, "tiles_per_wave must be 0, 2 or 4");
+  if (shape->workgroups_per_cu < 1 || shape->workgroups_per_cu > 64)
+    return fail(ctx, KG_ERR_ARG, "workgroups_per_cu must be 1-64");
+  if (shape->screen_wgs_per_cu < 0 || shape->screen_wgs_per_cu > 2)
+    return fail(ctx, KG_ERR_ARG, "screen_wgs_per_cu must be 0, 1 or 2");
+  // 4 tiles is the default wherever it is available; asking for it elsewhere
+  // keeps the default (2)
+  ctx->tiles = shape->tiles_per_wave == 2 ? 2 : 0;
+  ctx->grid_mult = shape->workgroups_per_cu;
+  ctx->scr_w8 = shape->screen_wgs_per_cu == 2;
+  drop_graph(ctx);
+  return KG_OK;
+}
+
+// Autotuner: times the bulk scan of `sample` synthetic rows (random bits)
+// with model 0 for each candidate (k-mer tiles per wave x persistent
+// workgroups per CU for the full kernel, then the screen's workgroups per CU)
+// and keeps the fastest for this context.
+extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, kg_tune* out) {
   if (!ctx) return KG_ERR_ARG;
   if (ctx->models.empty()) return fail(ctx, KG_ERR_STATE, "autotune before kg_set_model");
   if (sample <= 0) return fail(ctx, KG_ERR_ARG, "autotune sample must be positive");
@@ -1935,13 +1972,35 @@ extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t*
                             rw, pop.as<uint8_t>(), thr.as<uint32_t>(), 1, 1));
     KG_CK(ctx, hipStreamSynchronize(ctx->stream));
     const kg_scan_params prm{0.2, 6.0, 0.0};
+    const int keep_tiles = ctx->tiles, keep_gm = ctx->grid_mult, keep_mode = ctx->screen_mode,
+              keep_skip = ctx->screen_skip;
+    const bool keep_w8 = ctx->scr_w8;
+    auto restore = [&]() {
+      ctx->tiles = keep_tiles;
+      ctx->grid_mult = keep_gm;
+      ctx->scr_w8 = keep_w8;
+      ctx->screen_mode = keep_mode;
+      ctx->screen_skip = keep_skip;
+    };
+    // best of 3 timed scans after one warm-up; `screen` selects the timing read
+    auto time_scan = [&](bool screen, double* best_ms) -> int {
+      *best_ms = 1e300;
+      for (int rep = 0; rep < 4; ++rep) {
+        std::vector<kg_hit> hits;
+        std::vector<int64_t> np(md.t, 0);
+        HShard s{rows.as<uint64_t>(), sample, rw, 0, 0};
+        const int rc = scan_dispatch(ctx, {s}, prm, 0, hits, np.data(), false);
+        if (rc) return rc;
+        const double ms = screen ? ctx->timing.screen_ms : ctx->timing.bulk_ms;
+        if (rep > 0) *best_ms = std::min(*best_ms, ms);
+      }
+      return KG_OK;
+    };
+    // the full kernel's shape, on every row (random rows fail the screen)
     const int R_def = i8_tiles_for(M.NT, ctx->limbs, md.t, 0);
     std::vector<int> Rs{0};
     if (i8_tiles_for(M.NT, ctx->limbs, md.t, 2) != R_def) Rs.push_back(2);
     const int gms[3] = {1, 2, 4};
-    const int keep_tiles = ctx->tiles, keep_gm = ctx->grid_mult;
-    // the full kernel's shape is tuned on every row (random rows fail the screen)
-    const int keep_mode = ctx->screen_mode, keep_skip = ctx->screen_skip;
     ctx->screen_mode = KG_SCREEN_OFF;
     double best = 1e300;
     int best_R = keep_tiles, best_gm = keep_gm;
@@ -1949,20 +2008,11 @@ extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t*
       for (int gm : gms) {
         ctx->tiles = R;
         ctx->grid_mult = gm;
-        double tmin = 1e300;
-        for (int rep = 0; rep < 4; ++rep) {
-          std::vector<kg_hit> hits;
-          std::vector<int64_t> np(md.t, 0);
-          HShard s{rows.as<uint64_t>(), sample, rw, 0, 0};
-          const int rc = scan_dispatch(ctx, {s}, prm, 0, hits, np.data(), false);
-          if (rc) {
-            ctx->tiles = keep_tiles;
-            ctx->grid_mult = keep_gm;
-            ctx->screen_mode = keep_mode;
-            ctx->screen_skip = keep_skip;
-            return rc;
-          }
-          if (rep > 0) tmin = std::min(tmin, ctx->timing.bulk_ms);
+        double tmin = 0.0;
+        const int rc = time_scan(false, &tmin);
+        if (rc) {
+          restore();
+          return rc;
         }
         // a shape replaces the earlier (default first) one only when it is
         // faster by more than the run-to-run spread of these short scans
@@ -1974,12 +2024,31 @@ extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, int32_t* tiles, int32_t*
       }
     ctx->tiles = best_R;
     ctx->grid_mult = best_gm;
+    // the screen's workgroups per CU (where the model has a screen)
+    bool best_w8 = keep_w8;
+    if (M.screen_ok) {
+      ctx->screen_mode = KG_SCREEN_ON;
+      double sbest = 1e300;
+      for (int v = 0; v < 2; ++v) {
+        ctx->scr_w8 = v == 1;
+        double tmin = 0.0;
+        const int rc = time_scan(true, &tmin);
+        if (rc) {
+          restore();
+          return rc;
+        }
+        if (tmin < sbest * 0.98) {
+          sbest = tmin;
+          best_w8 = v == 1;
+        }
+      }
+    }
+    ctx->scr_w8 = best_w8;
     ctx->screen_mode = keep_mode;
     ctx->screen_skip = keep_skip;
     drop_graph(ctx);
   }
-  if (tiles) *tiles = i8_tiles_for(M.NT, ctx->limbs, md.t, ctx->tiles);
-  if (grid_mult) *grid_mult = ctx->grid_mult;
+  if (out) fill_tune(ctx, out);
   return KG_OK;
 }
 

@@ -301,12 +301,51 @@ def test_autotune_keeps_results(built):
     with KgwasHip() as k:
         k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
         h0, n0 = k.scan(bits, prm)
-        tiles, gm = k.autotune(300_000)
+        tune = k.autotune(300_000)
         h1, n1 = k.scan(bits, prm)
-    print(f"[autotune] tiles/wave {tiles}, workgroups/CU {gm}", flush=True)
-    assert tiles in (2, 4) and gm in (1, 2, 4)
+    print(f"[autotune] {tune}", flush=True)
+    assert tune["tiles_per_wave"] in (2, 4) and tune["workgroups_per_cu"] in (1, 2, 4)
+    assert tune["screen_wgs_per_cu"] in (1, 2)
     assert np.array_equal(n0, n1)
     assert np.array_equal(h0["idx"], h1["idx"]) and np.array_equal(h0["T"], h1["T"])
+
+
+@pytest.mark.parametrize("t", [1, 3])
+def test_every_launch_shape_gives_identical_results(built, t):
+    """Every shape the autotuner can pick (tiles per wave x full-kernel workgroups per CU x
+    screen workgroups per CU), screened and unscreened, reports the same hits, values and
+    counts as the default shape."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    m = 20_000
+    ds = synth.make_dataset(seed=80 + t, n_acc=2048, n_missing=40, npop=8, p=32, m=m,
+                            n_causal=20, effect_sd=0.4, traits=t, trait_corr=0.2)
+    bits = ds.fill_bits(0, m)
+    prm = ScanParams(0.2, 3.0, 6.0)
+    out = {}
+    with KgwasHip(screen="on") as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.Y)
+        dev = k.to_device(bits)
+        for tiles in (0, 2):
+            for gm in (1, 2, 4):
+                for sw in (1, 2):
+                    k.set_tune(tiles, gm, sw)
+                    for mode in ("on", "off"):
+                        k.set_screen(mode)
+                        out[(tiles, gm, sw, mode)] = (k.scan_device(dev.ptr, m, bits.shape[1], prm),
+                                                      k.get_tune())
+        dev.free()
+    (h0, n0), _ = out[(0, 1, 1, "on")]
+    assert len(h0) > 0
+    kpw = set()
+    for key, ((h, n), tune) in out.items():
+        kpw.add((tune["kmers_per_workgroup"], tune["screen_kmers_per_workgroup"]))
+        assert np.array_equal(n, n0), key
+        for f in ("idx", "trait", "c", "T", "effect", "pval", "flags"):
+            assert np.array_equal(h[f], h0[f]), (key, f)
+    print(f"[shapes] t={t}: {len(out)} shapes, (k-mers per workgroup, screen) {sorted(kpw)}",
+          flush=True)
+    if t == 1:  # one trait: 4 or 2 tiles per wave, 16- or 8-wave screen workgroups
+        assert len(kpw) >= 2
 
 
 def _shard_setup(k, nsh=5, seed=30):
