@@ -31,6 +31,7 @@ hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t
 int64_t screen_block_kmers(int t, bool w8);
 int screen_rows(int t);
 bool screen_count_row(int t);
+int screen_limbs(int t);
 int screen_max_chunks();
 int screen_wgs_per_cu(int t, int nch, bool dd, bool w8);
 hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
@@ -186,8 +187,9 @@ struct ModelDev {
   int NT = 0, nz = 0, nb = 0;
   DBuf wts, mask, rbar, B64, bpfull, colrec, trc;  // int8 path
   DBuf wf32, uyf;                                                     // FP32 path
-  // pre-filter screen: two int8 limbs of y - ybar per trait (absent when the
-  // model is too wide for the screen's LDS weight table)
+  // pre-filter screen: int8 limbs of y - ybar (three for one trait, two per
+  // trait otherwise; absent when the model is too wide for the screen's LDS
+  // weight table)
   DBuf swts;
   bool screen_ok = false;
   double sw[kMaxTraits] = {}, sh[kMaxTraits] = {};
@@ -590,6 +592,9 @@ static int build_screen(kg_ctx* ctx, ModelDev& M) {
   if (t > kMaxTraits || nch > screen_max_chunks()) return KG_OK;
   const int Q = screen_rows(t), KG = 16 / Q;
   const bool count_row = screen_count_row(t);
+  // one trait: three limbs (rows 0-2 of the 4-row shift group, count in row 3);
+  // more traits: two limbs per trait, rows 2 tr + l, count in row 2 t
+  const int SL = screen_limbs(t);
   std::vector<std::vector<int8_t>> lim(t);
   std::vector<double> yc(n);
   for (int tr = 0; tr < t; ++tr) {
@@ -600,14 +605,16 @@ static int build_screen(kg_ctx* ctx, ModelDev& M) {
       yc[j] = y[j] - ym;
       ymax = std::max(ymax, std::fabs(y[j]));
     }
-    double w2[2], hq;
-    quantize(yc.data(), n, 2, lim[tr], w2, &hq);
+    double wl[3], hq;
+    quantize(yc.data(), n, SL, lim[tr], wl, &hq);
+    // S~ = w_last * (integer limb combination); h = the exact remainder bound
     double h = 0.0;
     for (int j = 0; j < n; ++j) {
-      const double v = w2[1] * (254.0 * lim[tr][2 * j] + lim[tr][2 * j + 1]);
-      h = std::max(h, std::fabs(yc[j] - v));
+      double iv = 0.0;
+      for (int l = 0; l < SL; ++l) iv = 254.0 * iv + lim[tr][(size_t)SL * j + l];
+      h = std::max(h, std::fabs(yc[j] - wl[SL - 1] * iv));
     }
-    M.sw[tr] = w2[1];
+    M.sw[tr] = wl[SL - 1];
     M.sh[tr] = h * (1.0 + 1e-9) + 2.3e-16 * ((double)n + 8.0) * (ymax + std::fabs(ym));
   }
   std::vector<int> samp(md.n_acc, -1);
@@ -619,15 +626,15 @@ static int build_screen(kg_ctx* ctx, ModelDev& M) {
       for (int lane = 0; lane < 64; ++lane) {
         const int g = lane >> 4, row = lane & 15;
         if (row / Q != k % KG) continue;  // rows of another shift group
-        const int qn = row % Q, tr = qn >> 1, l = qn & 1;
-        const bool is_count = count_row && qn == 2 * t;
+        const int qn = row % Q, tr = qn / SL, l = qn % SL;
+        const bool is_count = count_row && qn == SL * t;
         if (tr >= t && !is_count) continue;
         int8_t* dst = &W[(size_t)C * per + ((size_t)k * 64 + lane) * 16];
         for (int e = 0; e < 16; ++e) {
           // operand byte e = 4 i + q: bit 8 q + k of the lane group's dword i
           const int a = 512 * C + 128 * g + 32 * (e >> 2) + 8 * (e & 3) + k;
           if (a < md.n_acc && samp[a] >= 0)
-            dst[e] = is_count ? (int8_t)1 : lim[tr][(size_t)samp[a] * 2 + l];
+            dst[e] = is_count ? (int8_t)1 : lim[tr][(size_t)samp[a] * SL + l];
         }
       }
   KG_CK(ctx, M.swts.ensure(W.size()));
@@ -1049,7 +1056,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   int64_t mtot = 0;
   for (const auto& s : sh) {
     const ModelDev& M = *ctx->models[s.model];
-    if (!M.screen_ok || s.m > (int64_t)UINT32_MAX) screen = false;
+    if (!M.screen_ok || s.m > (int64_t)kRowMask) screen = false;
     nch_max = std::max(nch_max, M.md.nch_i8);
     mtot += s.m;
   }
@@ -1093,7 +1100,11 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     const int64_t sbk = screen_block_kmers(t, sr.w8);
     const int64_t cw = screen_chunk_words();
     int64_t bm_words = 0;
-    for (int s = 0; s < nsh; ++s) bm_words += (sh[s].m + sbk - 1) / sbk * (sbk / 32);
+    // one trait (the 4-row screen): each 64 k-mers' kept words are followed by
+    // their proven words
+    const bool ilv = screen_rows(t) == 4;
+    const int64_t wpb = (sbk / 32) * (ilv ? 2 : 1);  // bitmap words per screen block
+    for (int s = 0; s < nsh; ++s) bm_words += (sh[s].m + sbk - 1) / sbk * wpb;
     KG_CK(ctx, ctx->surv.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(mtot, 1)));
     KG_CK(ctx, ctx->bmap.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(bm_words, 1)));
     if (dd) {
@@ -1120,7 +1131,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       S.mask = M.mask.p;
       S.bmap = bmw + boff;
       S.dmap = dd ? dmw + boff : nullptr;
-      boff += (sh[s].m + sbk - 1) / sbk * (sbk / 32);
+      boff += (sh[s].m + sbk - 1) / sbk * wpb;
       S.list = sv + off;
       S.mult = dd ? mlw + off : nullptr;
       S.nsurv = dc + base_surv + s;
@@ -1137,7 +1148,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       S.nchunks = md.nch_i8;
       S.t = md.t;
       S.n = md.n;
-      S.pad = 0;
+      S.ilv = ilv ? 1 : 0;
       // FP32 screen constants: w and the threshold to nearest (their relative
       // error sits inside the kernel's 1e-5 margin), h rounded up
       for (int tr = 0; tr < kMaxTraits; ++tr) {

@@ -20,6 +20,9 @@ struct Cand {
   int32_t c;       // allele count over the (shard-)masked accessions
   int32_t pad;
 };
+// Survivor lists hold rows below 2^31; a single-trait screen sets the top bit
+// of an entry whose pre-filter pass it proved.
+constexpr uint32_t kRowMask = 0x7FFFFFFFu;
 constexpr uint32_t kPfPass = 1u;  // pre-filter decided pass
 constexpr uint32_t kPfAmb = 2u;   // pre-filter within the error bound of tau
 constexpr uint32_t kSigDef = 4u;  // 1D threshold decided pass
@@ -107,12 +110,13 @@ struct ShardDesc {
 };
 
 // Pre-filter screen of one shard: a cheap pass that contracts each k-mer with
-// y - mean(y) only (two int8 limbs per trait) and keeps every k-mer whose
-// pre-filter is not certainly failed for some trait.
+// y - mean(y) only (SL int8 limbs per trait: three for one trait, two for
+// more) and keeps every k-mer whose pre-filter is not certainly failed for
+// some trait.
 //
 // Weights: [chunk][shift k][lane] 16-byte MFMA A operands.  Lane (g, row),
 // operand byte e = 4 i + q holds accession 512 C + 128 g + 32 i + 8 q + k;
-// row = Q (k % KG) + 2 tr + l for trait tr, limb l, and row Q (k % KG) + 2 t
+// row = Q (k % KG) + SL tr + l for trait tr, limb l, and row Q (k % KG) + SL t
 // is the count row (1 for phenotyped accessions; t <= 7); Q = 4, 8 or 16
 // rows per shift group, KG = 16 / Q; zero in the other rows.
 struct ScreenDesc {
@@ -130,7 +134,9 @@ struct ScreenDesc {
   uint32_t* dmap;
   uint32_t* mult;
   int chunk0, chunk1;            // this shard's compaction chunks [chunk0, chunk1)
-  int nchunks, t, n, pad;
+  int nchunks, t, n;
+  int ilv;                       // one trait: bmap interleaves kept and proven words
+                                 // (scr_word), list entries carry the proof in bit 31
   float wf[kMaxTraits];          // S~ = w_tr (254 A_0 + A_1), A_l = limb sums
   float hf[kMaxTraits];          // |S - S~| <= c h_tr (quantisation + rounding), rounded up
   float thrf[kMaxTraits];        // tau^2 |y - ybar|^2 / n; < 0: trait never passes

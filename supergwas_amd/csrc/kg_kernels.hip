@@ -223,7 +223,7 @@ __device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wa
   for (int r = 0; r < R; ++r) {
     int64_t k = kbase + 16 * r + col;
     if (k > D.m - 1) k = D.m - 1;  // clamp: rows past m are read, never reported
-    const int64_t row = D.rows ? (int64_t)gptr(D.rows)[k] : k;
+    const int64_t row = D.rows ? (int64_t)(gptr(D.rows)[k] & kRowMask) : k;
     rowp[r] = D.bits + row * D.rw + 2 * g;
   }
 }
@@ -511,7 +511,8 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 
     const int64_t kidx = kbase + 16 * rme + col;
     const bool valid = primary && (kidx < D.m);
-    const int64_t krow = (D.rows != nullptr && valid) ? (int64_t)gptr(D.rows)[kidx] : kidx;
+    const int64_t krow =
+        (D.rows != nullptr && valid) ? (int64_t)(gptr(D.rows)[kidx] & kRowMask) : kidx;
     const uint32_t kmult = (D.mult != nullptr && valid) ? gptr(D.mult)[kidx] : 1u;
     // |dz_k| <= c' h_k (c' = min(c, n - c))  =>
     //   |d |z|^2| <= 2c' sum|z_k| h_k + c'^2 sum h_k^2 <= 2c' |z| |h| + c'^2 |h|^2
@@ -609,7 +610,8 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 // ----------------------------------------------------------------------------
 // Pre-filter screen (src/kgwas.cpp:183-218 runs the correlation filter before
 // the projections; this is the same order on the device).  Per k-mer it forms
-// S~ = x'(y - ybar) from two int8 limbs of y - ybar and the allele count c,
+// S~ = x'(y - ybar) from int8 limbs of y - ybar (three for one trait, two per
+// trait otherwise) and the allele count c,
 // and keeps the k-mer unless |cor| < tau is certain for every trait:
 //   (|S~| + c h)^2 < tau^2 |y - ybar|^2 c (n - c) / n  (1 - 1e-5)
 // (evaluated in FP32 with every rounding on the safe side).
@@ -748,7 +750,8 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
   const int g = lane >> 4, col = lane & 15;
   const v4u* const mk_lds = reinterpret_cast<const v4u*>(scr_lds);
   // output rows 4 g .. 4 g + 3 of accumulator a: shift a KG + g / (Q / 4),
-  // trait slots tb and tb + 1 (limbs 0, 1 each); the count row is limb 0 of
+  // trait slots tb and tb + 1 (limbs 0, 1 each; Q = 4: one trait, limbs 0-2 in
+  // rows 0-2 and the count in row 3); the count row is limb 0 of
   // slot t
   const int ksh = g / (Q / 4);
   const int tb = 2 * (g % (Q / 4));
@@ -826,9 +829,15 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
 #pragma unroll
           for (int i = 0; i < 4; ++i) v[i] += sg * (acc[r][a][i] >> kk);
         }
-        part[r][0] = 254 * v[0] + v[1];
-        part[r][1] = 254 * v[2] + v[3];
-        if (CR) cpart[r] = (uint32_t)((tb == D.t) ? v[0] : ((tb + 1 == D.t) ? v[2] : 0));
+        if constexpr (Q == 4) {  // one trait, three limbs: rows 0-2, count in row 3
+          part[r][0] = 254 * v[0] + v[1];
+          part[r][1] = v[2];
+          cpart[r] = (uint32_t)v[3];
+        } else {
+          part[r][0] = 254 * v[0] + v[1];
+          part[r][1] = 254 * v[2] + v[3];
+          if (CR) cpart[r] = (uint32_t)((tb == D.t) ? v[0] : ((tb + 1 == D.t) ? v[2] : 0));
+        }
       }
       const int c = CR ? (int)group_scatter<R>(cpart) : (int)group_scatter<R>(cnt);
       const int64_t kbase = ((w - D.blk0) * W + wave) * (16 * R);
@@ -839,17 +848,40 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
       // positive terms), so the margin 1e-5 keeps "certainly fails" certain.
       const int n = D.n;
       const float cf = (float)c, vv = cf * (float)(n - c);
-      bool keep = false;
+      bool keep = false, sure = false;
 #pragma unroll
       for (int tr = 0; tr < TT; ++tr) {
         if (tr < D.t) {
-          uint32_t x[R];
+          float sA;
+          if constexpr (Q == 4) {
+            // 254 A_0 + A_1 is exact in int32 (|.| <= 254 * 127 n); A_2 is added in
+            // FP32 (relative error below 2e-7, inside the 4e-6 scaling below)
+            uint32_t x01[R], x2[R];
 #pragma unroll
-          for (int r = 0; r < R; ++r)
-            x[r] = (uint32_t)((tb == tr) ? part[r][0] : ((tb + 1 == tr) ? part[r][1] : 0));
-          const int A = (int)group_scatter<R>(x);
-          const float hi = fabsf(D.wf[tr] * (float)A) * (1.0f + 4e-6f) + cf * D.hf[tr];
+            for (int r = 0; r < R; ++r) {
+              x01[r] = (uint32_t)part[r][0];
+              x2[r] = (uint32_t)part[r][1];
+            }
+            const int A01 = (int)group_scatter<R>(x01), A2 = (int)group_scatter<R>(x2);
+            sA = fmaf((float)A01, 254.0f, (float)A2);
+          } else {
+            uint32_t x[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              x[r] = (uint32_t)((tb == tr) ? part[r][0] : ((tb + 1 == tr) ? part[r][1] : 0));
+            sA = (float)(int)group_scatter<R>(x);
+          }
+          const float aS = fabsf(D.wf[tr] * sA);
+          const float hi = aS * (1.0f + 4e-6f) + cf * D.hf[tr];
           if (D.thrf[tr] >= 0.0f && !(hi * hi < D.thrf[tr] * vv * (1.0f - 1e-5f))) keep = true;
+          if constexpr (Q == 4) {
+            // one trait: the pass is proven when (|S~| - c h)^2 clears the
+            // threshold by 2e-5 (every FP32 error here is below 1e-6 relative,
+            // |S~| is scaled down and h was rounded up)
+            const float lo = aS * (1.0f - 4e-6f) - cf * D.hf[tr];
+            sure = D.thrf[tr] >= 0.0f && lo > 0.0f &&
+                   lo * lo >= D.thrf[tr] * vv * (1.0f + 2e-5f);
+          }
         }
       }
       keep = keep && valid && c > 0 && c < n;
@@ -876,10 +908,18 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
       }
       // lane l owns k-mer kbase + l (lanes < 16 R): the ballot is the bitmap
       const unsigned long long bal = __ballot(keep);
-      if (lane == 0) {
-        auto* bm = gptr_w(D.bmap) + (kbase >> 5);
-        bm[0] = (uint32_t)bal;
-        if (R == 4) bm[1] = (uint32_t)(bal >> 32);
+      if constexpr (Q == 4) {
+        // kept and proven words of the wave's 64 k-mers in one store
+        const unsigned long long pb = __ballot(keep && sure);
+        if (lane == 0)
+          gptr_w(reinterpret_cast<v4u*>(D.bmap))[kbase >> 6] =
+              v4u{(uint32_t)bal, (uint32_t)(bal >> 32), (uint32_t)pb, (uint32_t)(pb >> 32)};
+      } else {
+        if (lane == 0) {
+          auto* bm = gptr_w(D.bmap) + (kbase >> 5);
+          bm[0] = (uint32_t)bal;
+          if (R == 4) bm[1] = (uint32_t)(bal >> 32);
+        }
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -899,6 +939,13 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
   }
 }
 
+// Physical index of kept-bitmap word w: one trait interleaves each 64 k-mers'
+// two kept words with their two proven words (kept lo, kept hi, proven lo,
+// proven hi), so the screen writes them in one store.
+__device__ __forceinline__ int64_t scr_word(int64_t w, bool ilv) {
+  return ilv ? 2 * w - (w & 1) : w;
+}
+
 // Compaction of the screen bitmaps into row lists, in three launches over
 // chunks of at most kScrChunkWords bitmap words (a chunk lies in one shard):
 //   scr_count: kept k-mers per chunk;
@@ -912,8 +959,9 @@ __global__ __launch_bounds__(256) void scr_count(const ScrChunk* __restrict__ ch
   __shared__ uint32_t red[4];
   const ScrChunk C = ch[blockIdx.x];
   const uint32_t* bm = ss[C.shard].bmap;
+  const bool ilv = ss[C.shard].ilv != 0;
   uint32_t v = 0;
-  for (int64_t w = C.w0 + threadIdx.x; w < C.w1; w += 256) v += __popc(bm[w]);
+  for (int64_t w = C.w0 + threadIdx.x; w < C.w1; w += 256) v += __popc(bm[scr_word(w, ilv)]);
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
@@ -991,12 +1039,14 @@ __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch
   const int nw = (int)(a1 - a0);
   const uint32_t* __restrict__ bm = D.bmap;
   const bool dd = D.mult != nullptr;  // duplicate-row scan: write run lengths too
-  uint32_t wv[8], dv[8];
+  const bool ilv = D.ilv != 0;        // one trait: proven passes set the entry's top bit
+  uint32_t wv[8], dv[8], pv[8];
   uint32_t v = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int j = 64 * i + lane;
-    wv[i] = (j < nw) ? bm[a0 + j] : 0u;
+    wv[i] = (j < nw) ? bm[scr_word(a0 + j, ilv)] : 0u;
+    pv[i] = (ilv && j < nw) ? bm[scr_word(a0 + j, true) + 2] : 0u;
     dv[i] = 0u;
     if (dd && j < nw) dv[i] = D.dmap[a0 + j];
     v += __popc(wv[i]);
@@ -1024,10 +1074,13 @@ __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch
         dlo = (uint32_t)__builtin_amdgcn_readlane((int)dv[i], 2 * j);
         dhi = (uint32_t)__builtin_amdgcn_readlane((int)dv[i], 2 * j + 1);
       }
+      const uint32_t plo = (uint32_t)__builtin_amdgcn_readlane((int)pv[i], 2 * j);
+      const uint32_t phi = (uint32_t)__builtin_amdgcn_readlane((int)pv[i], 2 * j + 1);
       if (set) {
         const int64_t pos = o + __popcll(bal & below);
         const int64_t wi = a0 + 64 * i + 2 * j + h;
-        list[pos] = (uint32_t)(32 * wi + (lane & 31));
+        const uint32_t proven = (((h ? phi : plo) >> (lane & 31)) & 1u) << 31;
+        list[pos] = (uint32_t)(32 * wi + (lane & 31)) | proven;
         if (dd) {
           // the run of dup bits after this row: to the end of its word, then
           // into the next (a run never crosses a wave's first row, whose dup
@@ -1413,6 +1466,9 @@ int64_t screen_block_kmers(int t, bool w8) {
 int screen_max_chunks() { return kScrMaxChunks; }
 int screen_rows(int t) { return scr_rows(t); }
 bool screen_count_row(int t) { return scr_count_row(t); }
+// Limbs of y - ybar in the screen's weights: three for one trait (the 4-row
+// shift group has the room), two otherwise.
+int screen_limbs(int t) { return scr_rows(t) == 4 && t == 1 ? 3 : 2; }
 
 template <int Q, bool CR, bool DD, bool W8>
 static const void* screen_fn() {
