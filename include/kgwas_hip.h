@@ -229,7 +229,13 @@ typedef struct kg_tune {
                                   workgroups of at most 128 registers per wave) */
   int32_t kmers_per_workgroup; /* out: k-mers per full-kernel workgroup block */
   int32_t screen_kmers_per_workgroup; /* out: k-mers per screen workgroup block */
-  int32_t reserved[3];
+  int32_t stream_weights;      /* 0: single-trait screened scans run their survivors through
+                                  the resident-weight kernel (two int8 limbs, every weight
+                                  chunk in LDS, the screen's proven pre-filter passes) where
+                                  the model's weights fit; 1: always the streaming kernel */
+  int32_t resident_active;     /* out: 1 when screened scans of model 0 use the
+                                  resident-weight kernel */
+  int32_t reserved[1];
 } kg_tune;
 
 /* Autotuner: times the int8 bulk scan of `sample` random rows with model 0 for

@@ -65,7 +65,8 @@ class Tune(ctypes.Structure):
     _fields_ = [("tiles_per_wave", ctypes.c_int32), ("workgroups_per_cu", ctypes.c_int32),
                 ("screen_wgs_per_cu", ctypes.c_int32), ("kmers_per_workgroup", ctypes.c_int32),
                 ("screen_kmers_per_workgroup", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("stream_weights", ctypes.c_int32), ("resident_active", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 1)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
@@ -362,9 +363,12 @@ class KgwasHip:
         self._check(lib().kg_autotune(self._h, int(sample), ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tune(self, tiles_per_wave=0, workgroups_per_cu=1, screen_wgs_per_cu=1):
-        """Set the launch shape explicitly (kg_set_tune)."""
+    def set_tune(self, tiles_per_wave=0, workgroups_per_cu=1, screen_wgs_per_cu=1,
+                 stream_weights=0):
+        """Set the launch shape explicitly (kg_set_tune); stream_weights=1 keeps
+        single-trait screened scans off the resident-weight kernel."""
         t = Tune(tiles_per_wave, workgroups_per_cu, screen_wgs_per_cu)
+        t.stream_weights = int(stream_weights)
         self._check(lib().kg_set_tune(self._h, ctypes.byref(t)))
 
     def get_tune(self):

@@ -43,6 +43,11 @@ hipError_t launch_screen_compact(hipStream_t s, const ScrChunk* ch, int ncc, con
 int64_t i8_block_kmers(int NT, int L, int t, int R_req);
 int i8_tiles_for(int NT, int L, int t, int R_req);
 int i8_column_capacity(int NT, int L);
+hipError_t launch_scan_i8_res(int NT, int nch_max, int grid, hipStream_t s, const ShardDesc* sd,
+                              int nshards, int64_t total_blocks, const ScanOut& o,
+                              const int64_t* total_dev);
+bool i8_res_fits(int NT, int nch);
+int64_t i8_res_block_kmers();
 hipError_t set_kernel_attributes();
 hipError_t launch_scan_f32(int NT, int nch, int grid, hipStream_t s, const uint64_t* bits,
                            int64_t m, int64_t rw, int64_t row0, const float* wts,
@@ -182,10 +187,22 @@ struct StatPatch {
 };
 
 // A null model with its device-side weights and constants.
+// An int8-limb weight set of a model at some limb count: chunk-major MFMA
+// operands, column and trait records, epilogue constants.
+struct I8Set {
+  int NT = 0;
+  DBuf wts, colrec, trc;
+  EpiI8 epi{};
+};
+
 struct ModelDev {
   Model md;
   int NT = 0, nz = 0, nb = 0;
   DBuf wts, mask, rbar, B64, bpfull, colrec, trc;  // int8 path
+  // two-limb set for the resident-weight kernel (single trait, when every
+  // chunk's weights fit its LDS)
+  I8Set res;
+  bool res_ok = false;
   DBuf wf32, uyf;                                                     // FP32 path
   // pre-filter screen: int8 limbs of y - ybar (three for one trait, two per
   // trait otherwise; absent when the model is too wide for the screen's LDS
@@ -230,6 +247,7 @@ struct kg_ctx {
   int tiles = 0;      // int8 scan: k-mer tiles per wave (0 = the kernel's default)
   int grid_mult_f32 = 8;  // FP32 scan: workgroups per CU
   bool scr_w8 = false;    // screen: the 8-wave workgroup shape (kg_autotune may pick it)
+  bool resident = true;   // single-trait screened scans: the resident-weight kernel
   void* scratch = nullptr;  // device address of this GPU's allocation-free scratch array
   // pre-filter screen (int8 path): mode, and in auto mode the unscreened
   // scans left before the screen is tried again
@@ -435,9 +453,15 @@ static void quantize(const double* col, int n, int L, std::vector<int8_t>& limbs
   }
 }
 
-static int build_i8(kg_ctx* ctx, ModelDev& M) {
+constexpr int kNoFit = -100;  // a secondary weight set that does not fit its kernel
+
+// Weight set at L limbs.  primary: the model's main set (also uploads the mask,
+// the FP64 refinement tables and b'; a model too wide for it is an error);
+// otherwise kNoFit when the columns need more tiles than L allows.
+static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_out,
+                        DBuf& wts_b, DBuf& colrec_b, DBuf& trc_b, EpiI8& ep) {
   Model& md = M.md;
-  const int n = md.n, p = md.p, t = md.t, L = ctx->limbs;
+  const int n = md.n, p = md.p, t = md.t;
   // 32385 n < 2^31 keeps every limb sum (|sum| <= 127 n) and the epilogue's
   // int32 arithmetic exact; n is far below this in practice
   if (n > 60000)
@@ -450,15 +474,19 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
   const int nt_max = (L == 3) ? 16 : (L == 2 ? 6 : 3);
   int NT = 1;
   while (NT <= nt_max && i8_column_capacity(NT, L) < nb) ++NT;
-  if (NT > nt_max)
+  if (NT > nt_max) {
+    if (!primary) return kNoFit;
     return fail(ctx, KG_ERR_ARG,
                 "p + t exceeds the int8 kernel's weight columns for this limb count (at most " +
                     std::to_string(i8_column_capacity(nt_max, L)) + " structure + trait columns)");
+  }
   const int NU = 16 * NT;
   const int ncol = i8_column_capacity(NT, L);
-  M.NT = NT;
-  M.nz = nz;
-  M.nb = nb;
+  NT_out = NT;
+  if (primary) {
+    M.nz = nz;
+    M.nb = nb;
+  }
   // base column b -> source column, limb l -> MFMA output row u
   auto src_col = [&](int b) -> const double* {
     return (b < t) ? &md.rc[(size_t)b * n] : &md.Uc[(size_t)zc[b - t] * n];
@@ -548,17 +576,19 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
     if (e != hipSuccess) return e;
     return hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice);
   };
-  KG_CK(ctx, up(M.wts, W.data(), W.size()));
-  KG_CK(ctx, up(M.mask, md.maskw.data(), md.maskw.size() * 8));
-  KG_CK(ctx, up(M.rbar, md.rbar.data(), t * 8));
-  KG_CK(ctx, up(M.B64, B.data(), B.size() * 8));
-  // b' has t * p entries: none when U has no columns (one zero keeps the upload valid)
-  std::vector<double> bpf(md.bprime);
-  if (bpf.empty()) bpf.push_back(0.0);
-  KG_CK(ctx, up(M.bpfull, bpf.data(), bpf.size() * 8));
-  KG_CK(ctx, up(M.colrec, rec.data(), rec.size() * 8));
-  KG_CK(ctx, up(M.trc, trc.data(), trc.size() * 8));
-  EpiI8& ep = M.epi;
+  KG_CK(ctx, up(wts_b, W.data(), W.size()));
+  if (primary) {
+    KG_CK(ctx, up(M.mask, md.maskw.data(), md.maskw.size() * 8));
+    KG_CK(ctx, up(M.rbar, md.rbar.data(), t * 8));
+    KG_CK(ctx, up(M.B64, B.data(), B.size() * 8));
+    // b' has t * p entries: none when U has no columns (one zero keeps the upload valid)
+    std::vector<double> bpf(md.bprime);
+    if (bpf.empty()) bpf.push_back(0.0);
+    KG_CK(ctx, up(M.bpfull, bpf.data(), bpf.size() * 8));
+  }
+  KG_CK(ctx, up(colrec_b, rec.data(), rec.size() * 8));
+  KG_CK(ctx, up(trc_b, trc.data(), trc.size() * 8));
+  ep = EpiI8{};
   ep.n = n;
   ep.nz = nz;
   ep.t = t;
@@ -570,13 +600,26 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
   ep.dmax_drop = (md.rotated && md.drop_d) ? md.dmax : 0.0;
   ep.h_d = ep.has_d ? colh[t] : 0.0;  // structure column 0 is d when it is kept
   ep.inv_n = 1.0 / (double)n;
-  ep.colrec = M.colrec.as<double>();
-  ep.trc = M.trc.as<double>();
+  ep.colrec = colrec_b.as<double>();
+  ep.trc = trc_b.as<double>();
   ep.sum_h2 = sum_h2 * (1.0 + 1e-12);
   for (int tr = 0; tr < kMaxTraits; ++tr) ep.sum_hbp[tr] = sum_hbp[tr] * (1.0 + 1e-12);
   ep.sum_E2 = sum_E2;
   ep.E_d = ep.has_d ? colE[t] : 0.0;
   ep.stat_est = 0;
+  return KG_OK;
+}
+
+static int build_i8(kg_ctx* ctx, ModelDev& M) {
+  int rc = build_i8_set(ctx, M, ctx->limbs, true, M.NT, M.wts, M.colrec, M.trc, M.epi);
+  if (rc != KG_OK) return rc;
+  // the resident-weight kernel's set: two limbs, one trait, every chunk in LDS
+  M.res_ok = false;
+  if (M.md.t == 1) {
+    rc = build_i8_set(ctx, M, 2, false, M.res.NT, M.res.wts, M.res.colrec, M.res.trc, M.res.epi);
+    if (rc == KG_OK) M.res_ok = i8_res_fits(M.res.NT, M.md.nch_i8);
+    else if (rc != kNoFit) return rc;
+  }
   return KG_OK;
 }
 
@@ -936,6 +979,10 @@ struct ScreenRun {
   int nch_max = 0;
   bool dd = false;              // with the duplicate-row check
   bool w8 = false;              // the screen's 8-wave shape
+  bool res = false;             // survivors through the resident-weight kernel
+  int res_nt = 0;               // its weight tiles
+  int64_t bk = 0;               // k-mers per block of the full kernel over the survivors
+  int64_t sblocks_full = 0;     // its block count if every k-mer survived (grid bound)
 };
 
 // Device part of a batched int8 scan: zero counters (unless the table upload
@@ -953,10 +1000,13 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
                              sr.dd, sr.w8));
     KG_CK(ctx, hipEventRecord(ctx->evs, ctx->stream));
     KG_CK(ctx, launch_screen_compact(ctx->stream, sr.d_ch, sr.ncc, sr.d_ss, sr.ccount, sr.P,
-                                     sr.d_sd2, nsh, i8_block_kmers(NT, L, t, ctx->tiles),
-                                     sr.d_total));
-    KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream, sr.d_sd2, nsh,
-                              total_blocks, o, sr.d_total));
+                                     sr.d_sd2, nsh, sr.bk, sr.d_total));
+    if (sr.res)
+      KG_CK(ctx, launch_scan_i8_res(sr.res_nt, sr.nch_max, ctx->cus, ctx->stream, sr.d_sd2, nsh,
+                                    sr.sblocks_full, o, sr.d_total));
+    else
+      KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream, sr.d_sd2, nsh,
+                                total_blocks, o, sr.d_total));
   } else if (!per_launch) {
     KG_CK(ctx, launch_scan_i8(NT, L, t, ctx->tiles, grid, ctx->stream, d_sd, nsh, total_blocks,
                               o, nullptr));
@@ -1095,7 +1145,22 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   std::vector<ScrChunk> chunks;
   ScreenRun sr;
   const bool dd = screen && ctx->dedup != 0;
+  // single trait: the survivors go through the resident-weight kernel when
+  // every model's two-limb weights fit its LDS (the screen then proves the
+  // pre-filter passes, which two limbs could not decide as tightly)
+  bool use_res = screen && t == 1 && ctx->resident;
+  int res_nt = 0;
+  for (const auto& s : sh) {
+    if (!use_res) break;
+    const ModelDev& M = *ctx->models[s.model];
+    if (!M.res_ok || (res_nt != 0 && M.res.NT != res_nt)) use_res = false;
+    res_nt = M.res.NT;
+  }
+  if (use_res && !i8_res_fits(res_nt, nch_max)) use_res = false;
   if (screen) {
+    sr.res = use_res;
+    sr.res_nt = res_nt;
+    sr.bk = use_res ? i8_res_block_kmers() : bk;
     sr.w8 = ctx->scr_w8;
     const int64_t sbk = screen_block_kmers(t, sr.w8);
     const int64_t cw = screen_chunk_words();
@@ -1163,6 +1228,19 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       sd2[s].mult = S.mult;
       sd2[s].m = 0;     // written on the device by the fix-up
       sd2[s].blk0 = 0;
+      if (use_res) {  // the two-limb set, with this scan's thresholds
+        const EpiI8& e0 = sd[s].ep;
+        EpiI8 e = M.res.epi;
+        e.tau = e0.tau;
+        e.tau_trivial = e0.tau_trivial;
+        e.all_sig = e0.all_sig;
+        e.pairs = e0.pairs;
+        e.stat_est = e0.stat_est;
+        e.r2star = e0.r2star;
+        sd2[s].wts = M.res.wts.p;
+        sd2[s].ep = e;
+      }
+      sr.sblocks_full += (sh[s].m + sr.bk - 1) / sr.bk;
       off += sh[s].m;
     }
     sr.on = true;
@@ -1933,7 +2011,9 @@ static void fill_tune(const kg_ctx* ctx, kg_tune* out) {
     out->tiles_per_wave = i8_tiles_for(M.NT, ctx->limbs, M.md.t, ctx->tiles);
     out->kmers_per_workgroup = (int32_t)i8_block_kmers(M.NT, ctx->limbs, M.md.t, ctx->tiles);
     out->screen_kmers_per_workgroup = (int32_t)screen_block_kmers(M.md.t, ctx->scr_w8);
+    out->resident_active = (ctx->resident && M.res_ok && M.screen_ok) ? 1 : 0;
   }
+  out->stream_weights = ctx->resident ? 0 : 1;
 }
 
 extern "C" int kg_get_tune(const kg_ctx* ctx, kg_tune* out) {
@@ -1950,11 +2030,14 @@ extern "C" int kg_set_tune(kg_ctx* ctx, const kg_tune* shape) {
     return fail(ctx, KG_ERR_ARG, "workgroups_per_cu must be 1-64");
   if (shape->screen_wgs_per_cu < 0 || shape->screen_wgs_per_cu > 2)
     return fail(ctx, KG_ERR_ARG, "screen_wgs_per_cu must be 0, 1 or 2");
+  if (shape->stream_weights < 0 || shape->stream_weights > 1)
+    return fail(ctx, KG_ERR_ARG, "stream_weights must be 0 or 1");
   // 4 tiles is the default wherever it is available; asking for it elsewhere
   // keeps the default (2)
   ctx->tiles = shape->tiles_per_wave == 2 ? 2 : 0;
   ctx->grid_mult = shape->workgroups_per_cu;
   ctx->scr_w8 = shape->screen_wgs_per_cu == 2;
+  ctx->resident = shape->stream_weights == 0;
   drop_graph(ctx);
   return KG_OK;
 }

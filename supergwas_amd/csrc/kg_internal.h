@@ -23,6 +23,9 @@ struct Cand {
 // Survivor lists hold rows below 2^31; a single-trait screen sets the top bit
 // of an entry whose pre-filter pass it proved.
 constexpr uint32_t kRowMask = 0x7FFFFFFFu;
+// Dynamic LDS of the resident-weight int8 kernel (its static LDS, the column
+// and trait records, stays below 8 KiB of the CU's 160 KiB).
+constexpr size_t kResLdsBytes = 152 * 1024;
 constexpr uint32_t kPfPass = 1u;  // pre-filter decided pass
 constexpr uint32_t kPfAmb = 2u;   // pre-filter within the error bound of tau
 constexpr uint32_t kSigDef = 4u;  // 1D threshold decided pass

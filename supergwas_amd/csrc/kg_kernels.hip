@@ -29,6 +29,7 @@ namespace kg {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
 
 // Global-address-space view of a pointer read from a descriptor.  Loads
 // through a generic pointer become flat loads, which also count against the
@@ -95,10 +96,12 @@ __device__ __forceinline__ void append_cand(bool want, const Cand& rec, const Sc
 // *pf_def is set when the pre-filter is decided "pass" (counted on device).
 // Both tests compare squares (no division or square root); the relative
 // margins 3e-12 and 1e-9 dwarf the rounding of the squared forms.
+// pf_sure: the screen proved this k-mer's pre-filter pass (single-trait
+// screened scans), so S and its bound are not consulted.
 __device__ __forceinline__ uint32_t classify_i8(int c, double xxM, double bxx, double S,
                                                 double bS, double xyM, double bxy,
                                                 const EpiI8& ep, double ycn, double yy,
-                                                bool* pf_def) {
+                                                bool pf_sure, bool* pf_def) {
   *pf_def = false;
   const int n = ep.n;
   if (c <= 0) return 0;                         // x = 0 (src/kgwas_2d.cpp:364-374)
@@ -107,7 +110,7 @@ __device__ __forceinline__ uint32_t classify_i8(int c, double xxM, double bxx, d
     if (!(ycn >= 1e-12)) return 0;              // |y - mean y| < eps
   }
   uint32_t f;
-  if (ep.tau_trivial) {
+  if (ep.tau_trivial || pf_sure) {
     f = kPfPass;
   } else {
     // |cor| >= tau  <=>  S^2 >= tau^2 (c (n - c) / n) |y - ybar|^2
@@ -215,17 +218,41 @@ __device__ __forceinline__ void dma_mask(const void* mask, int C, int g, v4i* ds
                                    (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
 }
 
+// Rows of block w for this lane, in two steps so that the survivor-list loads
+// are in flight long before the addresses are needed: block_idx issues them
+// (unconditionally: without a list it reads the accession mask, a valid word
+// it then ignores), rows_at turns them into row pointers at first use.
 template <int R>
-__device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wave, int g, int col,
-                                           const uint64_t* (&rowp)[R]) {
+__device__ __forceinline__ void block_idx(const ShardDesc& D, int64_t w, int wave, int col,
+                                          int64_t (&k)[R], uint32_t (&ridx)[R]) {
   const int64_t kbase = ((w - D.blk0) * kI8Waves + wave) * (16 * R);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    int64_t k = kbase + 16 * r + col;
-    if (k > D.m - 1) k = D.m - 1;  // clamp: rows past m are read, never reported
-    const int64_t row = D.rows ? (int64_t)(gptr(D.rows)[k] & kRowMask) : k;
+    k[r] = kbase + 16 * r + col;
+    if (k[r] > D.m - 1) k[r] = D.m - 1;  // clamp: rows past m are read, never reported
+    const uint32_t* src = D.rows ? D.rows + k[r] : reinterpret_cast<const uint32_t*>(D.mask);
+    ridx[r] = *gptr(src);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void rows_at(const ShardDesc& D, const int64_t (&k)[R],
+                                        const uint32_t (&ridx)[R], int g,
+                                        const uint64_t* (&rowp)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t row = D.rows ? (int64_t)(ridx[r] & kRowMask) : k[r];
     rowp[r] = D.bits + row * D.rw + 2 * g;
   }
+}
+
+template <int R>
+__device__ __forceinline__ void block_rows(const ShardDesc& D, int64_t w, int wave, int g, int col,
+                                           const uint64_t* (&rowp)[R]) {
+  int64_t k[R];
+  uint32_t ridx[R];
+  block_idx<R>(D, w, wave, col, k, ridx);
+  rows_at<R>(D, k, ridx, g, rowp);
 }
 
 // Lane-swap sums.  swap16(x, y): rows of 16 lanes pair up (0,1), (2,3); the
@@ -280,7 +307,15 @@ __host__ __device__ constexpr int i8_tiles(int NT, int TT) { return (TT == 1 && 
 // R = k-mer tiles of 16 per wave.
 // total_dev, when given, holds the block count (written on the device by the
 // screen's fixup; total_blocks is then only an upper bound).
-template <int NT, int L, int TT, int R>
+//
+// RES (resident weights): every chunk of the model's weights sits in dynamic
+// LDS for as long as the workgroup scans that model (staged when the block's
+// model changes), each lane loads its next chunk of bits straight into
+// registers, and the chunk loop has no barrier at all, so the two waves of a
+// SIMD never wait for each other.  Used for single-trait screened scans at two
+// limbs when nchunks x NT x 8 KiB fits in LDS; the survivor list's top bit
+// then carries the screen's proof of the pre-filter pass (pf_sure).
+template <int NT, int L, int TT, int R, bool RES>
 __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
                                                         int nshards, int64_t total_blocks,
                                                         ScanOut out,
@@ -291,16 +326,18 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
   constexpr int J = (4 * NT) / L;                    // base columns per lane group
   constexpr int JSP = (TT + 4) / 4;                  // slots that may hold r_c or d columns
   static_assert(4 * J <= kMaxCols, "column records exceed the LDS table");
+  static_assert(!RES || TT == 1, "resident weights: single-trait epilogue");
   // Separate LDS objects, so that reads of one buffer do not wait for the
   // direct-to-LDS loads still filling another.
-  __shared__ v4i buf0[FRAG2];
-  __shared__ v4i buf1[FRAG2];
-  __shared__ v4i bitsbuf[kI8Waves * (R + 1) * 64];   // [wave][r | mask][lane] next chunk
+  __shared__ v4i buf0[RES ? 1 : FRAG2];
+  __shared__ v4i buf1[RES ? 1 : FRAG2];
+  __shared__ v4i bitsbuf[RES ? 1 : kI8Waves * (R + 1) * 64];  // [wave][r | mask][lane]
   __shared__ double colrec_lds[4 * J * kColRec];
   __shared__ double trc_lds[kMaxTraits * kTrcRec];
+  extern __shared__ v4i wres[];  // RES: [chunk][step][tile][lane] of the staged model
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, col = lane & 15;
-  v4i* const mybits = bitsbuf + wave * (R + 1) * 64;
+  v4i* const mybits = bitsbuf + (RES ? 0 : wave * (R + 1) * 64);
   // this wave's decided pre-filter passes of the current shard (flushed when
   // the shard changes, so no workgroup barrier closes a block)
   uint32_t wpass[TT];
@@ -314,10 +351,18 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
   while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
   const uint64_t* rowp[R];  // this lane's rows of the current block
   block_rows<R>(sd[s], w, wave, g, col, rowp);
-  dma_bits<R>(rowp, 0, mybits, lane);
-  dma_mask(sd[s].mask, 0, g, mybits + R * 64);
-  dma_stage<NT>(reinterpret_cast<const v4i*>(sd[s].wts), buf0, wave, lane);
+  v4u rcur[R], mcur;        // RES: this lane's bits and mask slice of the next chunk
+  if constexpr (RES) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) rcur[r] = *gptr(reinterpret_cast<const v4u*>(rowp[r]));
+    mcur = gptr(reinterpret_cast<const v4u*>(sd[s].mask))[g];
+  } else {
+    dma_bits<R>(rowp, 0, mybits, lane);
+    dma_mask(sd[s].mask, 0, g, mybits + R * 64);
+    dma_stage<NT>(reinterpret_cast<const v4i*>(sd[s].wts), buf0, wave, lane);
+  }
   const double* staged_cr = nullptr;  // model whose epilogue constants are in LDS
+  const void* staged_w = nullptr;     // RES: model whose weights are in LDS
 
   while (w < total_blocks) {
     const ShardDesc& D = sd[s];
@@ -330,8 +375,11 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     const int64_t wn = w + gridDim.x;
     int sn = s;
     while (sn + 1 < nshards && sd[sn + 1].blk0 <= wn) ++sn;
+    // (row pointers of the next block: nrowp, formed where first used)
+    int64_t nk[R];
+    uint32_t nridx[R];
+    if (wn < total_blocks) block_idx<R>(sd[sn], wn, wave, col, nk, nridx);
     const uint64_t* nrowp[R];
-    if (wn < total_blocks) block_rows<R>(sd[sn], wn, wave, g, col, nrowp);
 
     v4i acc[R][NT];
 #pragma unroll
@@ -342,6 +390,80 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 #pragma unroll
     for (int r = 0; r < R; ++r) cnt[r] = 0;
 
+    if constexpr (RES) {
+      if (D.wts != staged_w) {  // workgroup-uniform: every wave is on block w
+        // every wave is past the previous model's weights and constants
+        __syncthreads();
+        const v4i* __restrict__ gw = reinterpret_cast<const v4i*>(D.wts);
+        const int nv = nchunks * 8 * NT * 64;
+        for (int i = threadIdx.x; i < nv; i += kI8Threads) wres[i] = gw[i];
+        const double* __restrict__ gcr = ep.colrec;
+        for (int i = threadIdx.x; i < ep.ncol * kColRec; i += kI8Threads) colrec_lds[i] = gcr[i];
+        for (int i = threadIdx.x; i < ep.t * kTrcRec; i += kI8Threads) trc_lds[i] = ep.trc[i];
+        __syncthreads();
+        staged_w = D.wts;
+        staged_cr = ep.colrec;
+      }
+      for (int C = 0; C < nchunks; ++C) {
+        uint32_t wd[R][4];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          wd[r][0] = rcur[r].x & mcur.x;
+          wd[r][1] = rcur[r].y & mcur.y;
+          wd[r][2] = rcur[r].z & mcur.z;
+          wd[r][3] = rcur[r].w & mcur.w;
+          cnt[r] += __popc(wd[r][0]) + __popc(wd[r][1]) + __popc(wd[r][2]) + __popc(wd[r][3]);
+        }
+        // the next chunk loads under this one's MFMAs (the last chunk reloads
+        // itself: unconditional loads into the same registers keep every wait
+        // where the data is used), issued before the MFMAs
+        {
+          const int Cn = C + 1 < nchunks ? C + 1 : C;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            rcur[r] = *gptr(reinterpret_cast<const v4u*>(rowp[r] + 8 * Cn));
+          mcur = gptr(reinterpret_cast<const v4u*>(D.mask))[4 * Cn + g];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const v4i* const wc = wres + (size_t)C * (8 * NT * 64);
+        // the same software pipeline as the streaming form: fragment q + PD is
+        // read while q's MFMAs run, step jj + 1's operands are unpacked beside
+        // step jj's MFMAs
+        constexpr int NQ = 8 * NT;
+        constexpr int PD = 4;
+        v4i bf[R], bn[R], af[PD + 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) bf[r] = unpack_word(wd[r][0], 0);
+#pragma unroll
+        for (int i = 0; i < PD; ++i) af[i] = wc[i * 64 + lane];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int jj = q / NT, t = q % NT;
+          if (jj + 1 < 8) {
+#pragma unroll
+            for (int r = t; r < R; r += NT) bn[r] = unpack_word(wd[r][(jj + 1) >> 1], 4 * ((jj + 1) & 1));
+          }
+          if (q + PD < NQ) af[(q + PD) % (PD + 1)] = wc[(q + PD) * 64 + lane];
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[q % (PD + 1)], bf[r], acc[r][t],
+                                                              0, 0, 0);
+          if (t == NT - 1 && jj + 1 < 8) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) bf[r] = bn[r];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      // the next block's first chunk, under this block's epilogue
+      if (wn < total_blocks) {
+        rows_at<R>(sd[sn], nk, nridx, g, nrowp);
+#pragma unroll
+        for (int r = 0; r < R; ++r) rcur[r] = *gptr(reinterpret_cast<const v4u*>(nrowp[r]));
+        mcur = gptr(reinterpret_cast<const v4u*>(sd[sn].mask))[g];
+      }
+    }
+    if constexpr (!RES) {
     __builtin_amdgcn_s_waitcnt(0);  // stage 0 weights and chunk 0 bits have landed
     __syncthreads();
     uint32_t wd[R][4];  // the current chunk's 128 bits per tile (masked)
@@ -374,6 +496,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
             dma_bits<R>(rowp, C + 1, mybits, lane);
             dma_mask(D.mask, C + 1, g, mybits + R * 64);
           } else if (wn < total_blocks) {  // the next block's first chunk
+            rows_at<R>(sd[sn], nk, nridx, g, nrowp);
             dma_bits<R>(nrowp, 0, mybits, lane);
             dma_mask(sd[sn].mask, 0, g, mybits + R * 64);
           }
@@ -423,12 +546,14 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         }
       }
     }
+    }  // !RES
 
     // this model's epilogue constants into LDS (when the model changes; the
     // wait also drains the next block's loads, so the epilogue then reads only
     // LDS); workgroup-uniform, and every wave finished the previous block's
-    // epilogue before this block's opening barrier
-    const bool restage = ep.colrec != staged_cr;
+    // epilogue before this block's opening barrier (RES: staged with the
+    // weights, behind barriers of their own)
+    const bool restage = !RES && ep.colrec != staged_cr;
     if (restage) {
       const double* __restrict__ gcr = ep.colrec;
       for (int i = threadIdx.x; i < ep.ncol * kColRec; i += kI8Threads) colrec_lds[i] = gcr[i];
@@ -441,8 +566,10 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     // read buf0, released by the barrier that closed it); with an even count
     // it was loaded at the start of the last stage, and the first bits chunk
     // at the start of the last chunk
-    if (wn < total_blocks && ((nchunks * SPC) & 1))
-      dma_stage<NT>(reinterpret_cast<const v4i*>(sd[sn].wts), buf0, wave, lane);
+    if constexpr (!RES) {
+      if (wn < total_blocks && ((nchunks * SPC) & 1))
+        dma_stage<NT>(reinterpret_cast<const v4i*>(sd[sn].wts), buf0, wave, lane);
+    }
     if (restage) lds_barrier();  // column records staged by every wave
 
     // ---- epilogue, from registers ----
@@ -511,8 +638,10 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 
     const int64_t kidx = kbase + 16 * rme + col;
     const bool valid = primary && (kidx < D.m);
-    const int64_t krow =
-        (D.rows != nullptr && valid) ? (int64_t)(gptr(D.rows)[kidx] & kRowMask) : kidx;
+    // survivor lists: the row, and (top bit) the screen's proof of the pass
+    const uint32_t lrow = (D.rows != nullptr && valid) ? gptr(D.rows)[kidx] : 0u;
+    const int64_t krow = (D.rows != nullptr && valid) ? (int64_t)(lrow & kRowMask) : kidx;
+    const bool pf_sure = RES && (lrow & ~kRowMask) != 0u;
     const uint32_t kmult = (D.mult != nullptr && valid) ? gptr(D.mult)[kidx] : 1u;
     // |dz_k| <= c' h_k (c' = min(c, n - c))  =>
     //   |d |z|^2| <= 2c' sum|z_k| h_k + c'^2 sum h_k^2 <= 2c' |z| |h| + c'^2 |h|^2
@@ -562,7 +691,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         const double bxy = es + 1e-12 * (fabs(sv) + cd * fabs(rbar) + 1.0);
         bool pf_def = false;
         uint32_t fl = 0;
-        if (valid) fl = classify_i8(c, xxM, bxx, S, bS, xyM, bxy, ep, tc[1], tc[2], &pf_def);
+        if (valid) fl = classify_i8(c, xxM, bxx, S, bS, xyM, bxy, ep, tc[1], tc[2], pf_sure, &pf_def);
         if (out.st_c != nullptr && valid) {
           out.st_xy[kidx * t + tr] = xyM;
           out.st_S[kidx * t + tr] = S;
@@ -600,10 +729,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     }
     w = wn;
     s = sn;
-    if (w < total_blocks) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) rowp[r] = nrowp[r];
-    }
+    if (w < total_blocks) rows_at<R>(sd[s], nk, nridx, g, rowp);
   }
 }
 
@@ -650,8 +776,6 @@ __host__ __device__ constexpr int scr_group_chunks(int R) { return 8 / R; }
 __host__ __device__ constexpr size_t scr_lds_bytes(int nch) {
   return (size_t)nch * 64 + (size_t)nch * 8 * 64 * 16;
 }
-
-typedef unsigned v4u __attribute__((ext_vector_type(4)));
 
 // Load group (block w of shard D, chunks C0 .. C0 + DC - 1) of this lane's R
 // rows.  Chunks past the model's last are loaded again as the last chunk
@@ -1402,9 +1526,45 @@ __global__ void synth_bits(uint64_t* __restrict__ out, uint64_t seed, int64_t km
 template <int NT, int L, int TT, int R>
 static hipError_t launch_i8_tr(dim3 grid, hipStream_t s, const ShardDesc* sd, int nshards,
                                int64_t total_blocks, const ScanOut& o, const int64_t* total_dev) {
-  auto kfn = scan_i8<NT, L, TT, R>;
+  auto kfn = scan_i8<NT, L, TT, R, false>;
   hipLaunchKernelGGL(kfn, grid, dim3(kI8Threads), 0, s, sd, nshards, total_blocks, o, total_dev);
   return hipGetLastError();
+}
+
+// Resident-weight instantiations: two limbs, one trait, 4 tiles per wave,
+// NT = 1..kResMaxTiles; the dynamic LDS holds nchunks x NT x 8 KiB.
+constexpr int kResMaxTiles = 4;
+template <int NT>
+static hipError_t launch_res_nt(dim3 grid, size_t shm, hipStream_t s, const ShardDesc* sd,
+                                int nshards, int64_t total_blocks, const ScanOut& o,
+                                const int64_t* total_dev) {
+  auto kfn = scan_i8<NT, 2, 1, 4, true>;
+  hipLaunchKernelGGL(kfn, grid, dim3(kI8Threads), shm, s, sd, nshards, total_blocks, o, total_dev);
+  return hipGetLastError();
+}
+
+size_t i8_res_lds_bytes(int NT, int nch) { return (size_t)nch * NT * 8 * 64 * 16; }
+
+bool i8_res_fits(int NT, int nch) {
+  return NT >= 1 && NT <= kResMaxTiles && i8_res_lds_bytes(NT, nch) <= kResLdsBytes;
+}
+
+// k-mers per workgroup block of the resident kernel
+int64_t i8_res_block_kmers() { return (int64_t)kI8Waves * 16 * 4; }
+
+hipError_t launch_scan_i8_res(int NT, int nch_max, int grid, hipStream_t s, const ShardDesc* sd,
+                              int nshards, int64_t total_blocks, const ScanOut& o,
+                              const int64_t* total_dev) {
+  if (!i8_res_fits(NT, nch_max)) return hipErrorInvalidValue;
+  dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
+  const size_t shm = i8_res_lds_bytes(NT, nch_max);
+  switch (NT) {
+    case 1: return launch_res_nt<1>(g, shm, s, sd, nshards, total_blocks, o, total_dev);
+    case 2: return launch_res_nt<2>(g, shm, s, sd, nshards, total_blocks, o, total_dev);
+    case 3: return launch_res_nt<3>(g, shm, s, sd, nshards, total_blocks, o, total_dev);
+    case 4: return launch_res_nt<4>(g, shm, s, sd, nshards, total_blocks, o, total_dev);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // Tiles per wave actually used for (NT, t) and a requested count (0 = default).
@@ -1608,6 +1768,13 @@ static hipError_t set_attr_nt() {
 
 hipError_t set_kernel_attributes() {
   hipError_t e = hipSuccess;
+  const void* res_fns[] = {(const void*)scan_i8<1, 2, 1, 4, true>,
+                           (const void*)scan_i8<2, 2, 1, 4, true>,
+                           (const void*)scan_i8<3, 2, 1, 4, true>,
+                           (const void*)scan_i8<4, 2, 1, 4, true>};
+  for (const void* f : res_fns)
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLdsBytes);
   const int scr = (int)scr_lds_bytes(kScrMaxChunks);
   for (int t = 1; t <= kMaxTraits; ++t)
     for (int v = 0; v < 4; ++v)

@@ -313,8 +313,8 @@ def test_autotune_keeps_results(built):
 @pytest.mark.parametrize("t", [1, 3])
 def test_every_launch_shape_gives_identical_results(built, t):
     """Every shape the autotuner can pick (tiles per wave x full-kernel workgroups per CU x
-    screen workgroups per CU), screened and unscreened, reports the same hits, values and
-    counts as the default shape."""
+    screen workgroups per CU), screened and unscreened, with the resident-weight kernel and
+    with streamed weights, reports the same hits, values and counts as the default shape."""
     from supergwas_amd.api import KgwasHip, ScanParams
     m = 20_000
     ds = synth.make_dataset(seed=80 + t, n_acc=2048, n_missing=40, npop=8, p=32, m=m,
@@ -328,14 +328,18 @@ def test_every_launch_shape_gives_identical_results(built, t):
         for tiles in (0, 2):
             for gm in (1, 2, 4):
                 for sw in (1, 2):
-                    k.set_tune(tiles, gm, sw)
-                    for mode in ("on", "off"):
-                        k.set_screen(mode)
-                        out[(tiles, gm, sw, mode)] = (k.scan_device(dev.ptr, m, bits.shape[1], prm),
-                                                      k.get_tune())
+                    for stw in (0, 1):
+                        k.set_tune(tiles, gm, sw, stw)
+                        for mode in ("on", "off"):
+                            k.set_screen(mode)
+                            out[(tiles, gm, sw, stw, mode)] = (
+                                k.scan_device(dev.ptr, m, bits.shape[1], prm), k.get_tune())
         dev.free()
-    (h0, n0), _ = out[(0, 1, 1, "on")]
+    (h0, n0), tune0 = out[(0, 1, 1, 0, "on")]
     assert len(h0) > 0
+    # one trait at n=2048, p=32: the two-limb weights (128 KiB) sit in LDS
+    assert tune0["resident_active"] == (1 if t == 1 else 0)
+    assert out[(0, 1, 1, 1, "on")][1]["resident_active"] == 0
     kpw = set()
     for key, ((h, n), tune) in out.items():
         kpw.add((tune["kmers_per_workgroup"], tune["screen_kmers_per_workgroup"]))
