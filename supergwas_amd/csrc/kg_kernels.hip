@@ -1000,11 +1000,12 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
           if (D.thrf[tr] >= 0.0f && !(hi * hi < D.thrf[tr] * vv * (1.0f - 1e-5f))) keep = true;
           if constexpr (Q == 4) {
             // one trait: the pass is proven when (|S~| - c h)^2 clears the
-            // threshold by 2e-5 (every FP32 error here is below 1e-6 relative,
-            // |S~| is scaled down and h was rounded up)
+            // threshold by 1e-5 (|S~| is scaled down by 4e-6 and h was rounded
+            // up, so lo <= |S| before its last rounding; the roundings of lo^2
+            // and of the threshold product total below 5e-7 relative)
             const float lo = aS * (1.0f - 4e-6f) - cf * D.hf[tr];
             sure = D.thrf[tr] >= 0.0f && lo > 0.0f &&
-                   lo * lo >= D.thrf[tr] * vv * (1.0f + 2e-5f);
+                   lo * lo >= D.thrf[tr] * vv * (1.0f + 1e-5f);
           }
         }
       }
