@@ -31,7 +31,6 @@ hipError_t launch_scan_i8(int NT, int L, int t, int R_req, int grid, hipStream_t
 int64_t screen_block_kmers(int t, bool w8);
 int screen_rows(int t);
 bool screen_count_row(int t);
-int screen_limbs(int t);
 int screen_max_chunks();
 int screen_wgs_per_cu(int t, int nch, bool dd, bool w8);
 hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
@@ -628,6 +627,33 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
 // every phenotyped accession, in the screen kernel's operand layout
 // (ScreenDesc, kg_internal.h).  h bounds |(y_j - exact mean) - v_j|: the
 // measured limb residual plus the FP64 rounding of the mean and of y - ybar.
+// FP6 e2m3 code of the digit d / 8 (|d| <= 15): sign bit 5, then |d| itself
+// (0-7: exponent 0, mantissa |d|; 8-15: exponent 1, mantissa |d| - 8).
+static inline uint32_t f6_code(int d) { return (d < 0 ? 32u : 0u) | (uint32_t)std::abs(d); }
+
+// Digits of y - ybar for the one-trait screen: v = s4 sum_l d_l 30^(4 - l),
+// |d_l| <= 15, s0 = max|v| / 15, s_{l+1} = s_l / 30; returns s4 and writes the
+// digits [sample][5] (the residual after digit l is at most s_l / 2 = 15 s_{l+1}).
+static double quantize_f6(const double* col, int n, std::vector<int8_t>& dig) {
+  double scale = 0.0;
+  for (int j = 0; j < n; ++j) scale = std::max(scale, std::fabs(col[j]));
+  dig.assign((size_t)n * 5, 0);
+  if (scale == 0.0) return 0.0;
+  double sl[5];
+  sl[0] = scale / 15.0;
+  for (int l = 1; l < 5; ++l) sl[l] = sl[l - 1] / 30.0;
+  for (int j = 0; j < n; ++j) {
+    double res = col[j];
+    for (int l = 0; l < 5; ++l) {
+      double d = std::nearbyint(res / sl[l]);
+      d = std::min(15.0, std::max(-15.0, d));
+      dig[(size_t)j * 5 + l] = (int8_t)d;
+      res -= d * sl[l];
+    }
+  }
+  return sl[4];
+}
+
 static int build_screen(kg_ctx* ctx, ModelDev& M) {
   const Model& md = M.md;
   const int n = md.n, t = md.t, nch = md.nch_i8;
@@ -635,9 +661,7 @@ static int build_screen(kg_ctx* ctx, ModelDev& M) {
   if (t > kMaxTraits || nch > screen_max_chunks()) return KG_OK;
   const int Q = screen_rows(t), KG = 16 / Q;
   const bool count_row = screen_count_row(t);
-  // one trait: three limbs (rows 0-2 of the 4-row shift group, count in row 3);
-  // more traits: two limbs per trait, rows 2 tr + l, count in row 2 t
-  const int SL = screen_limbs(t);
+  const bool f6 = Q == 4;  // one trait: FP6 digits against FP4 bits
   std::vector<std::vector<int8_t>> lim(t);
   std::vector<double> yc(n);
   for (int tr = 0; tr < t; ++tr) {
@@ -648,38 +672,72 @@ static int build_screen(kg_ctx* ctx, ModelDev& M) {
       yc[j] = y[j] - ym;
       ymax = std::max(ymax, std::fabs(y[j]));
     }
-    double wl[3], hq;
-    quantize(yc.data(), n, SL, lim[tr], wl, &hq);
-    // S~ = w_last * (integer limb combination); h = the exact remainder bound
+    // S~ = w * (integer digit combination over the set bits); h bounds each
+    // sample's remainder (the FP64 rounding of it allowed for)
+    const int SL = f6 ? 5 : 2;
+    const double base = f6 ? 30.0 : 254.0;
+    double w;
+    if (f6) {
+      w = quantize_f6(yc.data(), n, lim[tr]);
+    } else {
+      double wl[2], hq;
+      quantize(yc.data(), n, 2, lim[tr], wl, &hq);
+      w = wl[1];
+    }
     double h = 0.0;
     for (int j = 0; j < n; ++j) {
       double iv = 0.0;
-      for (int l = 0; l < SL; ++l) iv = 254.0 * iv + lim[tr][(size_t)SL * j + l];
-      h = std::max(h, std::fabs(yc[j] - wl[SL - 1] * iv));
+      for (int l = 0; l < SL; ++l) iv = base * iv + lim[tr][(size_t)SL * j + l];
+      h = std::max(h, std::fabs(yc[j] - w * iv));
     }
-    M.sw[tr] = wl[SL - 1];
+    M.sw[tr] = w;
     M.sh[tr] = h * (1.0 + 1e-9) + 2.3e-16 * ((double)n + 8.0) * (ymax + std::fabs(ym));
   }
   std::vector<int> samp(md.n_acc, -1);
   for (int j = 0; j < n; ++j) samp[md.idxs[j]] = j;
-  const size_t per = (size_t)8 * 64 * 16;  // bytes per 512-sample chunk
+  const size_t per = (size_t)8 * 64 * 16;  // bytes per 512-sample chunk (both forms)
   std::vector<int8_t> W((size_t)nch * per, 0);
-  for (int C = 0; C < nch; ++C)
-    for (int k = 0; k < 8; ++k)
-      for (int lane = 0; lane < 64; ++lane) {
-        const int g = lane >> 4, row = lane & 15;
-        if (row / Q != k % KG) continue;  // rows of another shift group
-        const int qn = row % Q, tr = qn / SL, l = qn % SL;
-        const bool is_count = count_row && qn == SL * t;
-        if (tr >= t && !is_count) continue;
-        int8_t* dst = &W[(size_t)C * per + ((size_t)k * 64 + lane) * 16];
-        for (int e = 0; e < 16; ++e) {
-          // operand byte e = 4 i + q: bit 8 q + k of the lane group's dword i
-          const int a = 512 * C + 128 * g + 32 * (e >> 2) + 8 * (e & 3) + k;
-          if (a < md.n_acc && samp[a] >= 0)
-            dst[e] = is_count ? (int8_t)1 : lim[tr][(size_t)samp[a] * SL + l];
+  if (f6) {
+    // [chunk][shift k 0..3][lane][32 bytes]: lane (g, row) holds row `row` of
+    // K-block g; shift k uses rows 8 (k % 2) + {0..4: digits, 5: count}; element
+    // j = 8 d + m (6 bits at bit 6 j) is accession 512 C + 128 g + 32 d + 4 m + k
+    for (int C = 0; C < nch; ++C)
+      for (int k = 0; k < 4; ++k)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int g = lane >> 4, row = lane & 15;
+          if (row / 8 != k % 2) continue;
+          const int qn = row % 8;
+          if (qn > 5) continue;
+          uint32_t bits[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          for (int j = 0; j < 32; ++j) {
+            const int a = 512 * C + 128 * g + 32 * (j >> 3) + 4 * (j & 7) + k;
+            if (a >= md.n_acc || samp[a] < 0) continue;
+            const int d = qn == 5 ? 1 : lim[0][(size_t)samp[a] * 5 + qn];
+            const uint64_t code = f6_code(d);
+            const int b0 = 6 * j;
+            bits[b0 >> 5] |= (uint32_t)(code << (b0 & 31));
+            if ((b0 & 31) > 26) bits[(b0 >> 5) + 1] |= (uint32_t)(code >> (32 - (b0 & 31)));
+          }
+          memcpy(&W[(size_t)C * per + (((size_t)k * 64 + lane) * 32)], bits, 32);
         }
-      }
+  } else {
+    for (int C = 0; C < nch; ++C)
+      for (int k = 0; k < 8; ++k)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int g = lane >> 4, row = lane & 15;
+          if (row / Q != k % KG) continue;  // rows of another shift group
+          const int qn = row % Q, tr = qn >> 1, l = qn & 1;
+          const bool is_count = count_row && qn == 2 * t;
+          if (tr >= t && !is_count) continue;
+          int8_t* dst = &W[(size_t)C * per + ((size_t)k * 64 + lane) * 16];
+          for (int e = 0; e < 16; ++e) {
+            // operand byte e = 4 i + q: bit 8 q + k of the lane group's dword i
+            const int a = 512 * C + 128 * g + 32 * (e >> 2) + 8 * (e & 3) + k;
+            if (a < md.n_acc && samp[a] >= 0)
+              dst[e] = is_count ? (int8_t)1 : lim[tr][(size_t)samp[a] * 2 + l];
+          }
+        }
+  }
   KG_CK(ctx, M.swts.ensure(W.size()));
   KG_CK(ctx, hipMemcpy(M.swts.p, W.data(), W.size(), hipMemcpyHostToDevice));
   M.screen_ok = true;

@@ -21,6 +21,7 @@
 
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "kg_internal.h"
 #include "synth_core.h"
@@ -853,6 +854,62 @@ __device__ __forceinline__ void scr_group(const v4u (&q)[DC][R], int C0, int nch
   }
 }
 
+// One-trait screen on FP4 bit operands (v_mfma_scale_f32_16x16x128_f8f6f4,
+// A = FP6 e2m3 weight digits, B = FP4 e2m1 bits).  Shift k (0..3) of a
+// 512-sample chunk keeps bit k of every nibble of the lane's four dwords with
+// one AND (k = 3 is moved to bit 2 first: a set sign bit would read as -0), so
+// each MFMA covers 128 samples: element j = 8 d + m of lane group g is sample
+// 128 g + 32 d + 4 m + k, with the value 0.5, 1, 2, 2 for k = 0..3.  The
+// weights put shift k in rows 8 (k % 2) .. + 7 of accumulator k / 2; every
+// product and sum is exact in FP32 (digits d / 8 with |d| <= 15).
+template <int R, int DC, bool DD>
+__device__ __forceinline__ void scr_group_f6(const v4u (&q)[DC][R], int C0, int nch,
+                                             const v4i* wl, const v4u* mk_lds, int g, int lane,
+                                             v4f (&acc)[R][2], uint32_t (&ne)[R]) {
+  typedef int v8i __attribute__((ext_vector_type(8)));
+#pragma unroll
+  for (int u = 0; u < DC; ++u) {
+    if (C0 + u < nch) {
+      const int C = C0 + u;
+      if (DD) {
+        const v4u mk = mk_lds[4 * C + g];
+        const bool first_col = (lane & 15) == 0;
+        v4u prev = v4u{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const v4u rr = v4u{ror16(q[u][r].x), ror16(q[u][r].y), ror16(q[u][r].z),
+                             ror16(q[u][r].w)};
+          const v4u pv = (r > 0 && first_col) ? prev : rr;
+          ne[r] |= ((q[u][r].x ^ pv.x) & mk.x) | ((q[u][r].y ^ pv.y) & mk.y) |
+                   ((q[u][r].z ^ pv.z) & mk.z) | ((q[u][r].w ^ pv.w) & mk.w);
+          prev = rr;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const v4i a0 = wl[((C * 4 + k) * 64 + lane) * 2];
+        const v4i a1 = wl[((C * 4 + k) * 64 + lane) * 2 + 1];
+        const v8i a = v8i{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, 0, 0};
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          v8i b;
+          if (k < 3) {
+            const uint32_t M = 0x11111111u << k;
+            b = v8i{(int)(q[u][r].x & M), (int)(q[u][r].y & M), (int)(q[u][r].z & M),
+                    (int)(q[u][r].w & M), 0, 0, 0, 0};
+          } else {
+            b = v8i{(int)((q[u][r].x >> 1) & 0x44444444u), (int)((q[u][r].y >> 1) & 0x44444444u),
+                    (int)((q[u][r].z >> 1) & 0x44444444u), (int)((q[u][r].w >> 1) & 0x44444444u),
+                    0, 0, 0, 0};
+          }
+          acc[r][k >> 1] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+              a, b, acc[r][k >> 1], 2, 4, 0, 127, 0, 127);
+        }
+      }
+    }
+  }
+}
+
 // Software pipeline over load groups (block, first chunk): the next group's
 // rows (the same block's next chunks, or the next block's first) are loaded
 // into a second register set while the current group computes, then copied
@@ -874,9 +931,8 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
   const int g = lane >> 4, col = lane & 15;
   const v4u* const mk_lds = reinterpret_cast<const v4u*>(scr_lds);
   // output rows 4 g .. 4 g + 3 of accumulator a: shift a KG + g / (Q / 4),
-  // trait slots tb and tb + 1 (limbs 0, 1 each; Q = 4: one trait, limbs 0-2 in
-  // rows 0-2 and the count in row 3); the count row is limb 0 of
-  // slot t
+  // trait slots tb and tb + 1 (limbs 0, 1 each); the count row is limb 0 of
+  // slot t.  (Q = 4, one trait: FP6 digits instead, scr_group_f6.)
   const int ksh = g / (Q / 4);
   const int tb = 2 * (g % (Q / 4));
 
@@ -889,14 +945,17 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
   int C0 = 0;
   const void* staged = nullptr;
   int nch_staged = 0;
-  v4i acc[R][NA];
+  // Q = 4 (one trait): FP6 digits x FP4 bits, FP32 accumulators (exact)
+  constexpr bool F6 = Q == 4;
+  using AccT = std::conditional_t<F6, v4f, v4i>;
+  AccT acc[R][NA];
   uint32_t cnt[R], ne[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     cnt[r] = 0;
     ne[r] = 0;
 #pragma unroll
-    for (int a = 0; a < NA; ++a) acc[r][a] = v4i{0, 0, 0, 0};
+    for (int a = 0; a < NA; ++a) acc[r][a] = AccT{0, 0, 0, 0};
   }
   v4u cq[DC][R], nq[DC][R];
   // first group through the prefetch set as well, so that the current set is
@@ -935,31 +994,43 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
       }
     }
     const v4i* const wl = scr_lds + 4 * nch_staged;
-    scr_group<KG, NA, R, DC, CR, DD>(cq, C0, nch, wl, mk_lds, g, lane, acc, cnt, ne);
+    if constexpr (F6)
+      scr_group_f6<R, DC, DD>(cq, C0, nch, wl, mk_lds, g, lane, acc, ne);
+    else
+      scr_group<KG, NA, R, DC, CR, DD>(cq, C0, nch, wl, mk_lds, g, lane, acc, cnt, ne);
 
     if (last_group) {
       // exact row totals over the shifts (every row of accumulator a is a
       // multiple of 2^k, k = a KG + ksh), then per-lane partials
       // 254 A_0 + A_1 of trait slots tb, tb + 1 and of the count row
-      int part[R][2];
+      int part[R][2], p4[R];
       uint32_t cpart[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         int v[4] = {0, 0, 0, 0};
+        if constexpr (F6) {
+          // digit sums: accumulator 0 holds shift 0 (rows 0-7, value 0.5) or 1
+          // (rows 8-15, value 1), accumulator 1 shift 2 or 3 (value 2); a digit
+          // d weighs d / 8.  Lane groups 0, 2: digits 0-3; 1, 3: digit 4, count
+          const float m0 = g < 2 ? 16.0f : 8.0f;
 #pragma unroll
-        for (int a = 0; a < NA; ++a) {
-          const int kk = a * KG + ksh;
-          const int sg = (kk == 7) ? -1 : 1;
-#pragma unroll
-          for (int i = 0; i < 4; ++i) v[i] += sg * (acc[r][a][i] >> kk);
-        }
-        if constexpr (Q == 4) {  // one trait, three limbs: rows 0-2, count in row 3
-          part[r][0] = 254 * v[0] + v[1];
-          part[r][1] = v[2];
-          cpart[r] = (uint32_t)v[3];
+          for (int i = 0; i < 4; ++i) v[i] = (int)fmaf(acc[r][0][i], m0, acc[r][1][i] * 4.0f);
+          const bool lo = (g & 1) == 0;
+          part[r][0] = lo ? 30 * v[0] + v[1] : 0;
+          part[r][1] = lo ? 30 * v[2] + v[3] : 0;
+          p4[r] = lo ? 0 : v[0];
+          cpart[r] = lo ? 0u : (uint32_t)v[1];
         } else {
+#pragma unroll
+          for (int a = 0; a < NA; ++a) {
+            const int kk = a * KG + ksh;
+            const int sg = (kk == 7) ? -1 : 1;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] += sg * (acc[r][a][i] >> kk);
+          }
           part[r][0] = 254 * v[0] + v[1];
           part[r][1] = 254 * v[2] + v[3];
+          p4[r] = 0;
           if (CR) cpart[r] = (uint32_t)((tb == D.t) ? v[0] : ((tb + 1 == D.t) ? v[2] : 0));
         }
       }
@@ -977,17 +1048,19 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
       for (int tr = 0; tr < TT; ++tr) {
         if (tr < D.t) {
           float sA;
-          if constexpr (Q == 4) {
-            // 254 A_0 + A_1 is exact in int32 (|.| <= 254 * 127 n); A_2 is added in
-            // FP32 (relative error below 2e-7, inside the 4e-6 scaling below)
-            uint32_t x01[R], x2[R];
+          if constexpr (F6) {
+            // D_01 = 30 D_0 + D_1 and D_23 = 30 D_2 + D_3 are exact in int32
+            // (|.| <= 465 n); the digit combination in int64, so S~ rounds once
+            uint32_t x01[R], x23[R], x4[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
               x01[r] = (uint32_t)part[r][0];
-              x2[r] = (uint32_t)part[r][1];
+              x23[r] = (uint32_t)part[r][1];
+              x4[r] = (uint32_t)p4[r];
             }
-            const int A01 = (int)group_scatter<R>(x01), A2 = (int)group_scatter<R>(x2);
-            sA = fmaf((float)A01, 254.0f, (float)A2);
+            const int D01 = (int)group_scatter<R>(x01), D23 = (int)group_scatter<R>(x23);
+            const int D4 = (int)group_scatter<R>(x4);
+            sA = (float)((int64_t)D01 * 27000 + (int64_t)D23 * 30 + D4);
           } else {
             uint32_t x[R];
 #pragma unroll
@@ -1050,7 +1123,7 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
       for (int r = 0; r < R; ++r) {
         cnt[r] = 0;
 #pragma unroll
-        for (int a = 0; a < NA; ++a) acc[r][a] = v4i{0, 0, 0, 0};
+        for (int a = 0; a < NA; ++a) acc[r][a] = AccT{0, 0, 0, 0};
       }
     }
     w = w2;
@@ -1627,9 +1700,6 @@ int64_t screen_block_kmers(int t, bool w8) {
 int screen_max_chunks() { return kScrMaxChunks; }
 int screen_rows(int t) { return scr_rows(t); }
 bool screen_count_row(int t) { return scr_count_row(t); }
-// Limbs of y - ybar in the screen's weights: three for one trait (the 4-row
-// shift group has the room), two otherwise.
-int screen_limbs(int t) { return scr_rows(t) == 4 && t == 1 ? 3 : 2; }
 
 template <int Q, bool CR, bool DD, bool W8>
 static const void* screen_fn() {
