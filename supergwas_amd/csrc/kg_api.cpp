@@ -1164,7 +1164,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   int64_t mtot = 0;
   for (const auto& s : sh) {
     const ModelDev& M = *ctx->models[s.model];
-    if (!M.screen_ok || s.m > (int64_t)kRowMask) screen = false;
+    if (!M.screen_ok || s.m > (int64_t)kRowMask || s.rw > (int64_t)INT32_MAX) screen = false;
     nch_max = std::max(nch_max, M.md.nch_i8);
     mtot += s.m;
   }

@@ -784,13 +784,17 @@ __host__ __device__ constexpr size_t scr_lds_bytes(int nch) {
 template <int W, int R, int DC>
 __device__ __forceinline__ void scr_load(const ScreenDesc& D, int64_t w, int C0, int wave, int g,
                                          int col, v4u (&q)[DC][R]) {
-  const int64_t kbase = ((w - D.blk0) * W + wave) * (16 * R);
+  // screened shards hold fewer than 2^31 rows: 32-bit row indices, and one
+  // 32 x 32 -> 64-bit multiply-add per row address
+  const uint32_t kbase = (uint32_t)(((w - D.blk0) * W + wave) * (16 * R));
+  const uint32_t klast = (uint32_t)(D.m - 1);
   const int nch = D.nchunks;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    int64_t k = kbase + 16 * r + col;
-    if (k > D.m - 1) k = D.m - 1;  // clamp: rows past m are read, never kept
-    const auto* rp = gptr(reinterpret_cast<const v4u*>(D.bits + k * D.rw)) + g;
+    uint32_t k = kbase + 16 * r + col;
+    if (k > klast) k = klast;  // clamp: rows past m are read, never kept
+    const auto* rp =
+        gptr(reinterpret_cast<const v4u*>(D.bits + (uint64_t)k * (uint32_t)D.rw)) + g;
 #pragma unroll
     for (int u = 0; u < DC; ++u) q[u][r] = rp[4 * min(C0 + u, nch - 1)];
   }
