@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 METRIC = "k-mer tests/sec @n=2048,p=32 mixed vs FP32 GPU vs CPU; stat rel err vs FP64"
 HBM_PEAK_GBS = 8000.0
 I8_PEAK_TOPS = 5000.0
+FP6_PEAK_TOPS = 10100.0
 
 
 def log(*a):
@@ -123,7 +124,7 @@ def main():
     # ---- mixed precision: warmup + timed steps ----
     for _ in range(args.warmup):
         mixed.scan_device(dev.ptr, args.m, rw, prm)
-    bulk_ms, ref_ms, cands, scr_ms, surv, tot_ms = [], [], [], [], [], []
+    bulk_ms, ref_ms, cands, scr_ms, surv, tot_ms, rchk = [], [], [], [], [], [], []
     t2 = time.perf_counter()
     for _ in range(args.steps):
         hits, npass = mixed.scan_device(dev.ptr, args.m, rw, prm)
@@ -134,34 +135,43 @@ def main():
         scr_ms.append(tm["screen_ms"])
         surv.append(tm["survivors"])
         tot_ms.append(tm["total_ms"])
+        rchk.append(tm["rechecked"])
     el = time.perf_counter() - t2
     ms_step = 1e3 * el / args.steps
     value = args.m * args.steps / el
     log(f"[bench] mixed: {ms_step:.2f} ms/step, {value:.3e} k-mers/s, bulk {np.mean(bulk_ms):.3f} ms,"
         f" refine {np.mean(ref_ms):.3f} ms, candidates {int(np.mean(cands))}, hits {len(hits)},"
         f" passed {int(npass[0])}, screen {np.mean(scr_ms):.3f} ms, survivors {int(np.mean(surv))},"
-        f" device total {np.mean(tot_ms):.3f} ms")
+        f" rechecked {int(np.mean(rchk))}, device total {np.mean(tot_ms):.3f} ms")
 
     if args.profile_only:
         return
 
-    # ---- the same scan with the survivors through the streaming kernel (3 int8
-    # limbs, weights staged chunk by chunk) instead of the resident-weight one ----
-    mixed.set_tune(tune["tiles_per_wave"] if tune else 0, tune["workgroups_per_cu"] if tune else 1,
-                   tune["screen_wgs_per_cu"] if tune else 1, stream_weights=1)
-    mixed.scan_device(dev.ptr, args.m, rw, prm)
-    t2b = time.perf_counter()
-    b2, c2 = [], []
-    for _ in range(max(args.steps // 2, 1)):
-        h2, np2 = mixed.scan_device(dev.ptr, args.m, rw, prm)
-        b2.append(mixed.timing()["bulk_ms"])
-        c2.append(mixed.timing()["candidates"])
-    value2 = args.m * max(args.steps // 2, 1) / (time.perf_counter() - t2b)
-    same2 = bool(np.array_equal(h2["idx"], hits["idx"]) and np.array_equal(np2, npass))
-    mixed.set_tune(tune["tiles_per_wave"] if tune else 0, tune["workgroups_per_cu"] if tune else 1,
-                   tune["screen_wgs_per_cu"] if tune else 1, stream_weights=0)
-    log(f"[bench] mixed, streamed weights: {value2:.3e} k-mers/s, bulk {np.mean(b2):.3f} ms, "
-        f"candidates {int(np.mean(c2))}, same hits and counts: {same2}")
+    # ---- the same scan with the survivors' kernel in its other forms: the
+    # resident kernel on two int8 limbs or three FP6 digits, and the streaming
+    # kernel (3 int8 limbs, weights staged chunk by chunk) ----
+    cur = mixed.get_tune()
+    shape = (tune["tiles_per_wave"] if tune else 0, tune["workgroups_per_cu"] if tune else 1,
+             tune["screen_wgs_per_cu"] if tune else 1)
+    variants = {}
+    for vname, kw in (("resident_int8_2limbs", dict(resident_form=1)),
+                      ("resident_fp6_3digits", dict(resident_form=3)),
+                      ("streamed_weights_3limbs", dict(stream_weights=1))):
+        mixed.set_tune(*shape, **kw)
+        mixed.scan_device(dev.ptr, args.m, rw, prm)
+        t2b = time.perf_counter()
+        b2, c2 = [], []
+        for _ in range(max(args.steps // 2, 1)):
+            h2, np2 = mixed.scan_device(dev.ptr, args.m, rw, prm)
+            b2.append(mixed.timing()["bulk_ms"])
+            c2.append(mixed.timing()["candidates"])
+        v2 = args.m * max(args.steps // 2, 1) / (time.perf_counter() - t2b)
+        same2 = bool(np.array_equal(h2["idx"], hits["idx"]) and np.array_equal(np2, npass))
+        variants[vname] = {"value": v2, "unit": "k-mer tests/s", "bulk_kernel_ms": float(np.mean(b2)),
+                           "candidates_per_step": int(np.mean(c2)), "same_hits_and_counts": same2}
+        log(f"[bench] mixed, {vname}: {v2:.3e} k-mers/s, bulk {np.mean(b2):.3f} ms, "
+            f"candidates {int(np.mean(c2))}, same hits and counts: {same2}")
+    mixed.set_tune(*shape, stream_weights=cur["stream_weights"], resident_form=cur["resident_form"])
 
     # ---- plain FP32 kernel ----
     fp32 = KgwasHip(precision="fp32")
@@ -239,9 +249,18 @@ def main():
     achieved = kern[dom]["achieved_GBs"]
     traffic = (traffic_per[dom] * kern[dom]["kmers"]) if dom in traffic_per else None
     bulk_s = np.mean(bulk_ms) / 1e3
-    # weight rows the survivors' kernel executes: 2 limbs with resident weights
-    limbs_exec = 2 if (tune or mixed.get_tune()).get("resident_active") else args.limbs
+    # weight rows the survivors' kernel executes: 2 or 3 digits / limbs per
+    # column with resident weights (FP6 x FP4 or int8 MFMA), else args.limbs
+    tnow = mixed.get_tune()
+    form = tnow["resident_form"] if tnow["resident_active"] else 0
+    limbs_exec = {0: args.limbs, 1: 2, 2: 2, 3: 3}[form]
     ncol_exec = 16 * int(np.ceil(limbs_exec * p / 16.0))
+    surv_ops = {0: f"{args.limbs} int8 limbs (streamed weights)",
+                1: "2 int8 limbs with weights resident in LDS",
+                2: "2 FP6 digits x FP4 bits with weights resident in LDS (3 int8 limbs for the "
+                   "k-mers its bounds leave open)",
+                3: "3 FP6 digits x FP4 bits with weights resident in LDS"}[form]
+    surv_peak = FP6_PEAK_TOPS if form in (2, 3) else I8_PEAK_TOPS
     line = {
         "metric": METRIC,
         "value": value,
@@ -253,19 +272,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8 (survivors), fp6 x fp4 with exact fp32 sums (screen)",
+        "dtype": ("fp6 x fp4 with exact fp32 sums (screen and survivors)" if form in (2, 3)
+                  else "int8 (survivors), fp6 x fp4 with exact fp32 sums (screen)"),
         "data": "synthetic (seeded; bits generated on device)",
         "config": {"workload": f"mixed: n=2048, p=32, m={args.m:.0e} k-mers, 1 trait; screen "
-                               f"5 FP6 digits x FP4 bits, survivors 2 int8 limbs with weights "
-                               f"resident in LDS, statistic {args.limbs} int8 limbs, FP64 "
-                               f"refinement", "n": n, "p": p, "m": args.m,
+                               f"5 FP6 digits x FP4 bits, survivors {surv_ops}, statistic "
+                               f"{args.limbs} int8 limbs, FP64 refinement",
+                   "n": n, "p": p, "m": args.m,
                    "corr_threshold": 0.2, "log10_thresh": 6.0},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "kernels": kern,
         "mfma": {"scan_i8_executed_TOPs": 2.0 * n_acc * ncol_exec * n_surv / bulk_s / 1e12
                  if bulk_s > 0 else 0.0,
-                 "peak_int8_TOPs": I8_PEAK_TOPS},
+                 "operands": "fp6 x fp4" if form in (2, 3) else "int8",
+                 "peak_TOPs": surv_peak},
         "screen_kernel_ms": float(np.mean(scr_ms)),
         "survivors_per_step": int(n_surv),
         "bulk_kernel_ms": float(np.mean(bulk_ms)),
@@ -274,10 +295,8 @@ def main():
         "fp32_gpu": {"value": fp32_value, "unit": "k-mer tests/s", "m": mf,
                      "bulk_kernel_ms": float(np.mean(f_bulk)),
                      "stat_max_rel_err_vs_fp64": float(err_f.max()), "flips": flips_f},
-        "streamed_weights": {"value": value2, "unit": "k-mer tests/s",
-                             "bulk_kernel_ms": float(np.mean(b2)),
-                             "candidates_per_step": int(np.mean(c2)),
-                             "same_hits_and_counts": same2},
+        "rechecked_per_step": int(np.mean(rchk)),
+        "survivor_kernel_forms": variants,
         "speedup_vs_fp32_gpu": value / fp32_value,
         "error_vs_fp64": {"sample": es, "stat_max_rel_err": float(err_m.max()),
                           "reported_stat_max_rel_err": rep_err,

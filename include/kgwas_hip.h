@@ -97,6 +97,9 @@ typedef struct kg_timing {
   int64_t survivors;    /* k-mers the screen passed to the full kernel (= kmers if unscreened) */
   int64_t stat_refined; /* kg_stats_device (KG_PREC_MIXED): k-mer/trait values recomputed in
                            FP64 because their bulk error estimate exceeded the budget */
+  int64_t rechecked;    /* screened scans with two FP6 digits (kg_tune.resident_form 2):
+                           survivors whose decision the digit bounds left open, scanned again
+                           by the three-limb kernel */
 } kg_timing;
 
 /* Device selection, stream creation.  Replaces the OpenBLAS/OpenMP thread
@@ -235,7 +238,11 @@ typedef struct kg_tune {
                                   the model's weights fit; 1: always the streaming kernel */
   int32_t resident_active;     /* out: 1 when screened scans of model 0 use the
                                   resident-weight kernel */
-  int32_t reserved[1];
+  int32_t resident_form;       /* operands of the resident-weight kernel: 1 = two int8 limbs
+                                  (shift-and-mask bit unpack), 2 = two FP6 digits per column
+                                  against FP4 bit operands (one AND per operand dword), with a
+                                  three-limb recheck pass of the k-mers its bounds leave open,
+                                  3 = three FP6 digits; 0 in kg_set_tune: the default (2) */
 } kg_tune;
 
 /* Autotuner: times the int8 bulk scan of `sample` random rows with model 0 for

@@ -66,10 +66,10 @@ class Tune(ctypes.Structure):
                 ("screen_wgs_per_cu", ctypes.c_int32), ("kmers_per_workgroup", ctypes.c_int32),
                 ("screen_kmers_per_workgroup", ctypes.c_int32),
                 ("stream_weights", ctypes.c_int32), ("resident_active", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 1)]
+                ("resident_form", ctypes.c_int32)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class _Timing(ctypes.Structure):
@@ -77,7 +77,8 @@ class _Timing(ctypes.Structure):
                 ("total_ms", ctypes.c_double), ("candidates", ctypes.c_int64),
                 ("kmers", ctypes.c_int64), ("copy_ms", ctypes.c_double),
                 ("wall_ms", ctypes.c_double), ("screen_ms", ctypes.c_double),
-                ("survivors", ctypes.c_int64), ("stat_refined", ctypes.c_int64)]
+                ("survivors", ctypes.c_int64), ("stat_refined", ctypes.c_int64),
+                ("rechecked", ctypes.c_int64)]
 
 
 _lib = None
@@ -354,7 +355,8 @@ class KgwasHip:
         self._check(lib().kg_last_timing(self._h, ctypes.byref(t)))
         return dict(bulk_ms=t.bulk_ms, refine_ms=t.refine_ms, total_ms=t.total_ms,
                     candidates=t.candidates, kmers=t.kmers, copy_ms=t.copy_ms, wall_ms=t.wall_ms,
-                    screen_ms=t.screen_ms, survivors=t.survivors, stat_refined=t.stat_refined)
+                    screen_ms=t.screen_ms, survivors=t.survivors, stat_refined=t.stat_refined,
+                    rechecked=t.rechecked)
 
     def autotune(self, sample=2_000_000):
         """Pick the launch shape (k-mer tiles per wave, full-kernel and screen workgroups
@@ -364,11 +366,13 @@ class KgwasHip:
         return t.as_dict()
 
     def set_tune(self, tiles_per_wave=0, workgroups_per_cu=1, screen_wgs_per_cu=1,
-                 stream_weights=0):
+                 stream_weights=0, resident_form=0):
         """Set the launch shape explicitly (kg_set_tune); stream_weights=1 keeps
-        single-trait screened scans off the resident-weight kernel."""
+        single-trait screened scans off the resident-weight kernel; resident_form
+        picks its operands (1 int8 limbs, 2 or 3 FP6 digits, 0 the default)."""
         t = Tune(tiles_per_wave, workgroups_per_cu, screen_wgs_per_cu)
         t.stream_weights = int(stream_weights)
+        t.resident_form = int(resident_form)
         self._check(lib().kg_set_tune(self._h, ctypes.byref(t)))
 
     def get_tune(self):

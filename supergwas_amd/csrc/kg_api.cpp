@@ -47,6 +47,11 @@ hipError_t launch_scan_i8_res(int NT, int nch_max, int grid, hipStream_t s, cons
                               const int64_t* total_dev);
 bool i8_res_fits(int NT, int nch);
 int64_t i8_res_block_kmers();
+hipError_t launch_scan_f6_res(int NT, int D, int nch_max, int grid, hipStream_t s,
+                              const ShardDesc* sd, int nshards, int64_t total_blocks,
+                              const ScanOut& o, const int64_t* total_dev);
+bool f6_res_fits(int NT, int nch);
+hipError_t launch_rchk_tables(hipStream_t s, ShardDesc* sd3, int nsh, int64_t bk, int64_t* total);
 hipError_t set_kernel_attributes();
 hipError_t launch_scan_f32(int NT, int nch, int grid, hipStream_t s, const uint64_t* bits,
                            int64_t m, int64_t rw, int64_t row0, const float* wts,
@@ -202,6 +207,11 @@ struct ModelDev {
   // chunk's weights fit its LDS)
   I8Set res;
   bool res_ok = false;
+  // FP6-digit set for the resident kernel (f6_digits digits per column; 0:
+  // not built)
+  I8Set f6;
+  bool f6_ok = false;
+  int f6_digits = 0;
   DBuf wf32, uyf;                                                     // FP32 path
   // pre-filter screen: int8 limbs of y - ybar (three for one trait, two per
   // trait otherwise; absent when the model is too wide for the screen's LDS
@@ -247,6 +257,9 @@ struct kg_ctx {
   int grid_mult_f32 = 8;  // FP32 scan: workgroups per CU
   bool scr_w8 = false;    // screen: the 8-wave workgroup shape (kg_autotune may pick it)
   bool resident = true;   // single-trait screened scans: the resident-weight kernel
+  // its operands: 0 = two int8 limbs; 2 or 3 = FP6 digits per column against
+  // FP4 bits (2: with the three-limb recheck pass of the k-mers it leaves open)
+  int res_digits = 2;
   void* scratch = nullptr;  // device address of this GPU's allocation-free scratch array
   // pre-filter screen (int8 path): mode, and in auto mode the unscreened
   // scans left before the screen is tried again
@@ -262,6 +275,7 @@ struct kg_ctx {
   int dedup = 0;  // screened scans check each row against the previous one
   std::vector<StatPatch> stat_patch;  // FP64 values of the last full-statistic scan
   DBuf surv, ssdesc, sdesc2, chdesc;  // survivor rows; graph-path screen tables
+  DBuf rchk, sdesc3;                  // recheck rows of the two-digit kernel; its graph-path table
   DBuf bmap, scr_aux;                 // screen bitmaps; chunk counts and prefixes
   DBuf pmask;                         // all-pairs scan: mask AND row i, per shard
   DBuf dmap, smult;                   // duplicate-row scans: dup bitmaps, survivor run lengths
@@ -452,13 +466,63 @@ static void quantize(const double* col, int n, int L, std::vector<int8_t>& limbs
   }
 }
 
+// FP6 e2m3 code of the digit d / 8 (|d| <= 15): sign bit 5, then |d| itself
+// (0-7: exponent 0, mantissa |d|; 8-15: exponent 1, mantissa |d| - 8).
+static inline uint32_t f6_code(int d) { return (d < 0 ? 32u : 0u) | (uint32_t)std::abs(d); }
+
+// Columns as D base-30 digits for the FP6 resident kernel: v = s * sum_l
+// d_l 30^(D-1-l) with |d_l| <= 15, s_0 = max|v| / 15, s_{l+1} = s_l / 30 and
+// s = s_{D-1} (the remainder after digit l is at most s_l / 2 = 15 s_{l+1}, so
+// no later digit clamps).  w[0] = 8 s, since the kernel's digit sums count
+// d / 8; *h, *E and *hc as for quantize().
+static void quantize_digits(const double* col, int n, int D, std::vector<int8_t>& dig, double* w,
+                            double* h, double* E, double* hc) {
+  double scale = 0.0;
+  for (int j = 0; j < n; ++j) scale = std::max(scale, std::fabs(col[j]));
+  dig.assign((size_t)n * D, 0);
+  for (int l = 0; l < D; ++l) w[l] = 0.0;
+  *E = 0.0;
+  *hc = 0.0;
+  if (scale == 0.0) {
+    *h = 0.0;
+    return;
+  }
+  double sl[3];
+  sl[0] = scale / 15.0;
+  for (int l = 1; l < D; ++l) sl[l] = sl[l - 1] / 30.0;
+  const double s = sl[D - 1];
+  std::vector<double> e(n);
+  double sum = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double res = col[j];
+    double iv = 0.0;
+    for (int l = 0; l < D; ++l) {
+      double d = std::nearbyint(res / sl[l]);
+      d = std::min(15.0, std::max(-15.0, d));
+      dig[(size_t)j * D + l] = (int8_t)d;
+      res -= d * sl[l];
+      iv = 30.0 * iv + d;
+    }
+    e[j] = col[j] - s * iv;
+    sum += e[j];
+  }
+  w[0] = 8.0 * s;
+  *h = 0.5 * s + 1e-15 * scale;
+  const double mean = sum / n;
+  double mx = 0.0;
+  for (int j = 0; j < n; ++j) mx = std::max(mx, std::fabs(e[j] - mean));
+  *E = sum;
+  *hc = mx + 2e-15 * scale + 4e-16 * (double)n * (*h);
+}
+
 constexpr int kNoFit = -100;  // a secondary weight set that does not fit its kernel
 
 // Weight set at L limbs.  primary: the model's main set (also uploads the mask,
 // the FP64 refinement tables and b'; a model too wide for it is an error);
-// otherwise kNoFit when the columns need more tiles than L allows.
+// otherwise kNoFit when the columns need more tiles than L allows.  f6: L FP6
+// digits per column in the resident kernel's operand layout instead.
 static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_out,
-                        DBuf& wts_b, DBuf& colrec_b, DBuf& trc_b, EpiI8& ep) {
+                        DBuf& wts_b, DBuf& colrec_b, DBuf& trc_b, EpiI8& ep, bool f6 = false) {
   Model& md = M.md;
   const int n = md.n, p = md.p, t = md.t;
   // 32385 n < 2^31 keeps every limb sum (|sum| <= 127 n) and the epilogue's
@@ -470,7 +534,7 @@ static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_o
     if (!(k == 0 && md.rotated && md.drop_d)) zc.push_back(k);
   const int nz = (int)zc.size();
   const int nb = nz + t;  // base columns: t residuals, then nz structure columns
-  const int nt_max = (L == 3) ? 16 : (L == 2 ? 6 : 3);
+  const int nt_max = f6 ? 6 : ((L == 3) ? 16 : (L == 2 ? 6 : 3));
   int NT = 1;
   while (NT <= nt_max && i8_column_capacity(NT, L) < nb) ++NT;
   if (NT > nt_max) {
@@ -499,7 +563,10 @@ static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_o
   std::vector<int8_t> ucol((size_t)n * NU, 0);  // [sample][u-row]
   for (int b = 0; b < nb; ++b) {
     double h0 = 0.0;
-    quantize(src_col(b), n, L, lim, &colw[(size_t)b * L], &h0, &colE[b], &colh[b]);
+    if (f6)
+      quantize_digits(src_col(b), n, L, lim, &colw[(size_t)b * L], &h0, &colE[b], &colh[b]);
+    else
+      quantize(src_col(b), n, L, lim, &colw[(size_t)b * L], &h0, &colE[b], &colh[b]);
     for (int l = 0; l < L; ++l) {
       const int u = urow(b, l);
       for (int j = 0; j < n; ++j) ucol[(size_t)j * NU + u] = lim[(size_t)j * L + l];
@@ -510,8 +577,37 @@ static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_o
   for (int j = 0; j < n; ++j) samp[md.idxs[j]] = j;
   const int nch = md.nch_i8;
   const size_t frag = (size_t)8 * NT * 64 * 16;  // bytes per chunk
-  std::vector<int8_t> W((size_t)nch * frag, 0);
-  for (int C = 0; C < nch; ++C)
+  std::vector<int8_t> W(f6 ? 0 : (size_t)nch * frag, 0);
+  if (f6) {
+    // [chunk][shift k][tile][lane] operands of 32 six-bit digit codes: the
+    // first 16 bytes of every operand, then the last 8 (kF6OperandBytes);
+    // element j of lane group g is accession 512 C + 128 g + 32 (j / 8) +
+    // 4 (j % 8) + k, at bit 6 j
+    const size_t nop = (size_t)nch * 4 * NT * 64;
+    W.assign(nop * kF6OperandBytes, 0);
+    uint32_t* X = reinterpret_cast<uint32_t*>(W.data());
+    uint32_t* Y = X + nop * 4;
+    for (int C = 0; C < nch; ++C)
+      for (int k = 0; k < 4; ++k)
+        for (int tt = 0; tt < NT; ++tt)
+          for (int lane = 0; lane < 64; ++lane) {
+            const int g = lane >> 4, uc = 16 * tt + (lane & 15);
+            uint32_t bits[6] = {0, 0, 0, 0, 0, 0};
+            for (int j = 0; j < 32; ++j) {
+              const int a = 512 * C + 128 * g + 32 * (j >> 3) + 4 * (j & 7) + k;
+              if (a >= md.n_acc || samp[a] < 0) continue;
+              const uint64_t code = f6_code(ucol[(size_t)samp[a] * NU + uc]);
+              const int b0 = 6 * j;
+              bits[b0 >> 5] |= (uint32_t)(code << (b0 & 31));
+              if ((b0 & 31) > 26) bits[(b0 >> 5) + 1] |= (uint32_t)(code >> (32 - (b0 & 31)));
+            }
+            const size_t q = (((size_t)C * 4 + k) * NT + tt) * 64 + lane;
+            for (int i = 0; i < 4; ++i) X[q * 4 + i] = bits[i];
+            Y[q * 2] = bits[4];
+            Y[q * 2 + 1] = bits[5];
+          }
+  }
+  for (int C = 0; C < (f6 ? 0 : nch); ++C)
     for (int jj = 0; jj < 8; ++jj)
       for (int tt = 0; tt < NT; ++tt)
         for (int lane = 0; lane < 64; ++lane) {
@@ -609,6 +705,22 @@ static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_o
   return KG_OK;
 }
 
+// The FP6-digit resident set at ctx->res_digits digits (one trait; none when
+// the context uses int8 limbs or the model does not fit the kernel).
+static int build_f6(kg_ctx* ctx, ModelDev& M) {
+  M.f6_ok = false;
+  M.f6_digits = 0;
+  const int D = ctx->res_digits;
+  if (M.md.t != 1 || D == 0) return KG_OK;
+  const int rc = build_i8_set(ctx, M, D, false, M.f6.NT, M.f6.wts, M.f6.colrec, M.f6.trc,
+                              M.f6.epi, true);
+  if (rc == kNoFit) return KG_OK;
+  if (rc != KG_OK) return rc;
+  M.f6_ok = f6_res_fits(M.f6.NT, M.md.nch_i8);
+  M.f6_digits = D;
+  return KG_OK;
+}
+
 static int build_i8(kg_ctx* ctx, ModelDev& M) {
   int rc = build_i8_set(ctx, M, ctx->limbs, true, M.NT, M.wts, M.colrec, M.trc, M.epi);
   if (rc != KG_OK) return rc;
@@ -619,7 +731,7 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
     if (rc == KG_OK) M.res_ok = i8_res_fits(M.res.NT, M.md.nch_i8);
     else if (rc != kNoFit) return rc;
   }
-  return KG_OK;
+  return build_f6(ctx, M);
 }
 
 // Pre-filter screen weights: y - ybar of every trait as two balanced int8
@@ -627,10 +739,6 @@ static int build_i8(kg_ctx* ctx, ModelDev& M) {
 // every phenotyped accession, in the screen kernel's operand layout
 // (ScreenDesc, kg_internal.h).  h bounds |(y_j - exact mean) - v_j|: the
 // measured limb residual plus the FP64 rounding of the mean and of y - ybar.
-// FP6 e2m3 code of the digit d / 8 (|d| <= 15): sign bit 5, then |d| itself
-// (0-7: exponent 0, mantissa |d|; 8-15: exponent 1, mantissa |d| - 8).
-static inline uint32_t f6_code(int d) { return (d < 0 ? 32u : 0u) | (uint32_t)std::abs(d); }
-
 // Digits of y - ybar for the one-trait screen: v = s4 sum_l d_l 30^(4 - l),
 // |d_l| <= 15, s0 = max|v| / 15, s_{l+1} = s_l / 30; returns s4 and writes the
 // digits [sample][5] (the residual after digit l is at most s_l / 2 = 15 s_{l+1}).
@@ -1039,6 +1147,15 @@ struct ScreenRun {
   bool w8 = false;              // the screen's 8-wave shape
   bool res = false;             // survivors through the resident-weight kernel
   int res_nt = 0;               // its weight tiles
+  int res_d = 0;                // its FP6 digits per column (0: two int8 limbs)
+  // recheck pass of the k-mers the two-digit kernel leaves open: the streaming
+  // kernel (primary weights, `rchk_L` limbs, two tiles per wave) over the
+  // device-built table d_sd3 (d_total3 blocks of bk3 k-mers)
+  bool rchk = false;
+  ShardDesc* d_sd3 = nullptr;
+  int64_t* d_total3 = nullptr;
+  int64_t bk3 = 0;
+  int64_t rblocks_full = 0;     // its block count if every k-mer went to it (grid bound)
   int64_t bk = 0;               // k-mers per block of the full kernel over the survivors
   int64_t sblocks_full = 0;     // its block count if every k-mer survived (grid bound)
 };
@@ -1059,7 +1176,15 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
     KG_CK(ctx, hipEventRecord(ctx->evs, ctx->stream));
     KG_CK(ctx, launch_screen_compact(ctx->stream, sr.d_ch, sr.ncc, sr.d_ss, sr.ccount, sr.P,
                                      sr.d_sd2, nsh, sr.bk, sr.d_total));
-    if (sr.res)
+    if (sr.res && sr.res_d > 0) {
+      KG_CK(ctx, launch_scan_f6_res(sr.res_nt, sr.res_d, sr.nch_max, ctx->cus, ctx->stream,
+                                    sr.d_sd2, nsh, sr.sblocks_full, o, sr.d_total));
+      if (sr.rchk) {
+        KG_CK(ctx, launch_rchk_tables(ctx->stream, sr.d_sd3, nsh, sr.bk3, sr.d_total3));
+        KG_CK(ctx, launch_scan_i8(NT, L, t, 2, grid, ctx->stream, sr.d_sd3, nsh,
+                                  sr.rblocks_full, o, sr.d_total3));
+      }
+    } else if (sr.res)
       KG_CK(ctx, launch_scan_i8_res(sr.res_nt, sr.nch_max, ctx->cus, ctx->stream, sr.d_sd2, nsh,
                                     sr.sblocks_full, o, sr.d_total));
     else
@@ -1107,6 +1232,8 @@ static void fill_descs(const kg_ctx* ctx, const HShard& sh, int sid, const kg_sc
   D.pad2 = 0;
   D.rows = nullptr;
   D.mult = nullptr;
+  D.rchk = nullptr;
+  D.rchk_n = nullptr;
   D.ep = M.epi;
   D.ep.tau = prm.corr_threshold;
   D.ep.tau_trivial = prm.corr_threshold <= 0.0 ? 1 : 0;
@@ -1183,8 +1310,40 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       screen = false;
     }
   }
-  const size_t base_surv = 1 + (size_t)nsh * t;  // counters: [cand][passes][survivors][total]
-  ctx->counters_len = base_surv + (screen ? (size_t)nsh + 1 : 0);
+  // counters: [cand][passes][survivors][total][recheck counts][recheck total]
+  const size_t base_surv = 1 + (size_t)nsh * t;
+  const size_t base_rchk = base_surv + (screen ? (size_t)nsh + 1 : 0);
+  const bool dd = screen && ctx->dedup != 0;
+  // single trait: the survivors go through the resident-weight kernel (the
+  // screen then proves the pre-filter passes): FP6 digits when the context
+  // asks for them and every model has that set, else two int8 limbs where every
+  // model's weights fit its LDS; two digits come with the recheck pass (not in
+  // duplicate-row scans, whose run lengths it does not carry)
+  bool use_res = screen && t == 1 && ctx->resident;
+  int res_nt = 0, res_d = 0;
+  if (use_res && ctx->res_digits > 0 && !(ctx->res_digits == 2 && dd)) {
+    res_d = ctx->res_digits;
+    for (const auto& s : sh) {
+      const ModelDev& M = *ctx->models[s.model];
+      if (!M.f6_ok || M.f6_digits != res_d || (res_nt != 0 && M.f6.NT != res_nt)) {
+        res_d = 0;
+        break;
+      }
+      res_nt = M.f6.NT;
+    }
+    if (res_d > 0 && !f6_res_fits(res_nt, nch_max)) res_d = 0;
+    if (res_d == 0) res_nt = 0;
+  }
+  if (use_res && res_d == 0) {
+    for (const auto& s : sh) {
+      const ModelDev& M = *ctx->models[s.model];
+      if (!M.res_ok || (res_nt != 0 && M.res.NT != res_nt)) use_res = false;
+      res_nt = M.res.NT;
+    }
+    if (use_res && !i8_res_fits(res_nt, nch_max)) use_res = false;
+  }
+  const bool rchk = use_res && res_d == 2;
+  ctx->counters_len = base_rchk + (rchk ? (size_t)nsh + 1 : 0);
   KG_CK(ctx, ctx->counters.ensure(sizeof(unsigned long long) * ctx->counters_len));
   unsigned long long* dc = ctx->counters.as<unsigned long long>();
   for (int s = 0; s < nsh; ++s) {
@@ -1199,26 +1358,16 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   // bitmap covers its screen blocks; compaction chunks of at most
   // screen_chunk_words() bitmap words never straddle shards
   std::vector<ScreenDesc> ss;
-  std::vector<ShardDesc> sd2;
+  std::vector<ShardDesc> sd2, sd3;
   std::vector<ScrChunk> chunks;
   ScreenRun sr;
-  const bool dd = screen && ctx->dedup != 0;
-  // single trait: the survivors go through the resident-weight kernel when
-  // every model's two-limb weights fit its LDS (the screen then proves the
-  // pre-filter passes, which two limbs could not decide as tightly)
-  bool use_res = screen && t == 1 && ctx->resident;
-  int res_nt = 0;
-  for (const auto& s : sh) {
-    if (!use_res) break;
-    const ModelDev& M = *ctx->models[s.model];
-    if (!M.res_ok || (res_nt != 0 && M.res.NT != res_nt)) use_res = false;
-    res_nt = M.res.NT;
-  }
-  if (use_res && !i8_res_fits(res_nt, nch_max)) use_res = false;
   if (screen) {
     sr.res = use_res;
     sr.res_nt = res_nt;
+    sr.res_d = res_d;
+    sr.rchk = rchk;
     sr.bk = use_res ? i8_res_block_kmers() : bk;
+    sr.bk3 = i8_block_kmers(NT, ctx->limbs, t, 2);
     sr.w8 = ctx->scr_w8;
     const int64_t sbk = screen_block_kmers(t, sr.w8);
     const int64_t cw = screen_chunk_words();
@@ -1234,12 +1383,14 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       KG_CK(ctx, ctx->dmap.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(bm_words, 1)));
       KG_CK(ctx, ctx->smult.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(mtot, 1)));
     }
+    if (rchk) KG_CK(ctx, ctx->rchk.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(mtot, 1)));
     uint32_t* sv = ctx->surv.as<uint32_t>();
     uint32_t* bmw = ctx->bmap.as<uint32_t>();
     uint32_t* dmw = dd ? ctx->dmap.as<uint32_t>() : nullptr;
     uint32_t* mlw = dd ? ctx->smult.as<uint32_t>() : nullptr;
     ss.resize(nsh);
     sd2 = sd;
+    if (rchk) sd3 = sd;
     int64_t off = 0, boff = 0;
     for (int s = 0; s < nsh; ++s) {
       const ModelDev& M = *ctx->models[sh[s].model];
@@ -1286,17 +1437,31 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       sd2[s].mult = S.mult;
       sd2[s].m = 0;     // written on the device by the fix-up
       sd2[s].blk0 = 0;
-      if (use_res) {  // the two-limb set, with this scan's thresholds
+      if (use_res) {  // the resident set, with this scan's thresholds
         const EpiI8& e0 = sd[s].ep;
-        EpiI8 e = M.res.epi;
+        const I8Set& rs = res_d > 0 ? M.f6 : M.res;
+        EpiI8 e = rs.epi;
         e.tau = e0.tau;
         e.tau_trivial = e0.tau_trivial;
         e.all_sig = e0.all_sig;
         e.pairs = e0.pairs;
         e.stat_est = e0.stat_est;
         e.r2star = e0.r2star;
-        sd2[s].wts = M.res.wts.p;
+        sd2[s].wts = rs.wts.p;
         sd2[s].ep = e;
+      }
+      if (rchk) {
+        // shard s's open k-mers go to rchk[off_s, ...) (at most its m_s
+        // survivors) and are scanned again by the streaming kernel
+        uint32_t* rl = ctx->rchk.as<uint32_t>() + off;
+        unsigned long long* rn = dc + base_rchk + s;
+        sd2[s].rchk = rl;
+        sd2[s].rchk_n = rn;
+        sd3[s].rows = rl;
+        sd3[s].rchk_n = rn;  // read by rchk_tables
+        sd3[s].m = 0;        // written on the device
+        sd3[s].blk0 = 0;
+        sr.rblocks_full += (sh[s].m + sr.bk3 - 1) / sr.bk3;
       }
       sr.sblocks_full += (sh[s].m + sr.bk - 1) / sr.bk;
       off += sh[s].m;
@@ -1305,6 +1470,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     sr.dd = dd;
     sr.nch_max = nch_max;
     sr.d_total = reinterpret_cast<int64_t*>(dc + base_surv + nsh);
+    if (rchk) sr.d_total3 = reinterpret_cast<int64_t*>(dc + base_rchk + nsh);
     sr.ncc = (int)chunks.size();
     KG_CK(ctx, ctx->scr_aux.ensure(sizeof(uint32_t) * (chunks.size() + 2) +
                                    sizeof(int64_t) * (chunks.size() + 2)));
@@ -1366,7 +1532,8 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       const size_t kb = sizeof(ShardDesc) * nsh * 2 + sizeof(RefDesc) * nsh + sizeof(o) + 16 +
                         (screen ? (sizeof(ScreenDesc) + sizeof(ShardDesc)) * nsh + 32 +
                                       sizeof(ScrChunk) * chunks.size()
-                                : 0);
+                                : 0) +
+                        sizeof(ShardDesc) * sd3.size();
       key.resize(kb);
       char* kp = key.data();
       if (screen) {
@@ -1381,6 +1548,8 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
         kp += 32;
         if (!chunks.empty()) memcpy(kp, chunks.data(), sizeof(ScrChunk) * chunks.size());
         kp += sizeof(ScrChunk) * chunks.size();
+        if (!sd3.empty()) memcpy(kp, sd3.data(), sizeof(ShardDesc) * sd3.size());
+        kp += sizeof(ShardDesc) * sd3.size();
       }
       memcpy(kp, sd.data(), sizeof(ShardDesc) * nsh);
       kp += sizeof(ShardDesc) * nsh;
@@ -1417,12 +1586,18 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
           if (!chunks.empty())
             KG_CK(ctx, hipMemcpy(ctx->chdesc.p, chunks.data(), sizeof(ScrChunk) * chunks.size(),
                                  hipMemcpyHostToDevice));
+          if (!sd3.empty()) {
+            KG_CK(ctx, ctx->sdesc3.ensure(sizeof(ShardDesc) * nsh));
+            KG_CK(ctx, hipMemcpy(ctx->sdesc3.p, sd3.data(), sizeof(ShardDesc) * nsh,
+                                 hipMemcpyHostToDevice));
+          }
         }
       }
       if (screen) {
         sr.d_ss = ctx->ssdesc.as<ScreenDesc>();
         sr.d_sd2 = ctx->sdesc2.as<ShardDesc>();
         sr.d_ch = ctx->chdesc.as<ScrChunk>();
+        if (!sd3.empty()) sr.d_sd3 = ctx->sdesc3.as<ShardDesc>();
       }
     } else {
       // descriptor tables: packed into one page-locked staging buffer and
@@ -1434,8 +1609,9 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       const size_t b0 = sizeof(ShardDesc) * nsh, b2 = sizeof(RefDesc) * nsh;
       const size_t b3 = screen ? sizeof(ScreenDesc) * nsh : 0, b4 = screen ? b0 : 0;
       const size_t b5 = screen ? sizeof(ScrChunk) * chunks.size() : 0;
+      const size_t b6 = sizeof(ShardDesc) * sd3.size();
       const size_t o1 = al(b0), o2 = o1 + al(b0), o3 = o2 + al(b2), o4 = o3 + al(b3),
-                   o5 = o4 + al(b4), tot = o5 + al(std::max<size_t>(b5, 1));
+                   o5 = o4 + al(b4), o6 = o5 + al(std::max<size_t>(b5, 1)), tot = o6 + al(b6);
       KG_CK(ctx, ctx->hdesc.ensure(tot));
       KG_CK(ctx, ctx->ddesc.ensure(tot));
       char* hp = static_cast<char*>(ctx->hdesc.p);
@@ -1446,6 +1622,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
         memcpy(hp + o3, ss.data(), b3);
         memcpy(hp + o4, sd2.data(), b4);
         if (b5) memcpy(hp + o5, chunks.data(), b5);
+        if (b6) memcpy(hp + o6, sd3.data(), b6);
       }
       if (tot <= setup_max_bytes() && ctx->counters_len <= (size_t)INT32_MAX) {
         KG_CK(ctx, launch_setup(ctx->stream, hp, tot, ctx->ddesc.p, dc, (int)ctx->counters_len));
@@ -1461,6 +1638,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
         sr.d_ss = reinterpret_cast<const ScreenDesc*>(dp + o3);
         sr.d_sd2 = reinterpret_cast<ShardDesc*>(dp + o4);
         sr.d_ch = reinterpret_cast<const ScrChunk*>(dp + o5);
+        if (b6) sr.d_sd3 = reinterpret_cast<ShardDesc*>(dp + o6);
       }
     }
     KG_CK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
@@ -1530,6 +1708,9 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
   ctx->timing.kmers = mtot;
   ctx->timing.survivors = mtot;
   ctx->timing.stat_refined = 0;
+  ctx->timing.rechecked = 0;
+  if (rchk)
+    for (int s = 0; s < nsh; ++s) ctx->timing.rechecked += (int64_t)cnt[base_rchk + s];
   if (screen) {
     int64_t ns = 0;
     for (int s = 0; s < nsh; ++s) ns += (int64_t)cnt[base_surv + s];
@@ -1817,6 +1998,7 @@ extern "C" int kg_scan(kg_ctx* ctx, const uint64_t* bits, int64_t m, int64_t row
     acc.candidates += ctx->timing.candidates;
     acc.screen_ms += ctx->timing.screen_ms;
     acc.survivors += ctx->timing.survivors;
+    acc.rechecked += ctx->timing.rechecked;
   }
   KG_CK(ctx, hipStreamSynchronize(ctx->cstream));
   acc.kmers = m;
@@ -2069,9 +2251,12 @@ static void fill_tune(const kg_ctx* ctx, kg_tune* out) {
     out->tiles_per_wave = i8_tiles_for(M.NT, ctx->limbs, M.md.t, ctx->tiles);
     out->kmers_per_workgroup = (int32_t)i8_block_kmers(M.NT, ctx->limbs, M.md.t, ctx->tiles);
     out->screen_kmers_per_workgroup = (int32_t)screen_block_kmers(M.md.t, ctx->scr_w8);
-    out->resident_active = (ctx->resident && M.res_ok && M.screen_ok) ? 1 : 0;
+    out->resident_active =
+        (ctx->resident && M.screen_ok && (M.res_ok || (M.f6_ok && M.f6_digits == ctx->res_digits)))
+            ? 1 : 0;
   }
   out->stream_weights = ctx->resident ? 0 : 1;
+  out->resident_form = ctx->res_digits == 0 ? 1 : ctx->res_digits;
 }
 
 extern "C" int kg_get_tune(const kg_ctx* ctx, kg_tune* out) {
@@ -2090,6 +2275,18 @@ extern "C" int kg_set_tune(kg_ctx* ctx, const kg_tune* shape) {
     return fail(ctx, KG_ERR_ARG, "screen_wgs_per_cu must be 0, 1 or 2");
   if (shape->stream_weights < 0 || shape->stream_weights > 1)
     return fail(ctx, KG_ERR_ARG, "stream_weights must be 0 or 1");
+  if (shape->resident_form < 0 || shape->resident_form > 3)
+    return fail(ctx, KG_ERR_ARG, "resident_form must be 0-3");
+  const int digits = shape->resident_form == 1 ? 0 : (shape->resident_form == 3 ? 3 : 2);
+  if (digits != ctx->res_digits) {
+    KG_CK(ctx, hipSetDevice(ctx->device));
+    ctx->res_digits = digits;
+    if (ctx->precision == KG_PREC_MIXED)
+      for (auto& M : ctx->models) {
+        const int rc = build_f6(ctx, *M);
+        if (rc != KG_OK) return rc;
+      }
+  }
   // 4 tiles is the default wherever it is available; asking for it elsewhere
   // keeps the default (2)
   ctx->tiles = shape->tiles_per_wave == 2 ? 2 : 0;

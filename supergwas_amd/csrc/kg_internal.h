@@ -26,6 +26,10 @@ constexpr uint32_t kRowMask = 0x7FFFFFFFu;
 // Dynamic LDS of the resident-weight int8 kernel (its static LDS, the column
 // and trait records, stays below 8 KiB of the CU's 160 KiB).
 constexpr size_t kResLdsBytes = 152 * 1024;
+// FP6-digit resident weights: per 512-sample chunk, 4 shifts x NT tiles x 64
+// lanes of 24-byte A operands (32 six-bit digits), stored as a 16-byte part
+// [chunk][shift][tile][lane] followed by an 8-byte part in the same order.
+constexpr size_t kF6OperandBytes = 24;
 constexpr uint32_t kPfPass = 1u;  // pre-filter decided pass
 constexpr uint32_t kPfAmb = 2u;   // pre-filter within the error bound of tau
 constexpr uint32_t kSigDef = 4u;  // 1D threshold decided pass
@@ -109,6 +113,12 @@ struct ShardDesc {
   int pad2;
   const uint32_t* rows;          // k-mer j of the scan is row rows[j] (null: row j)
   const uint32_t* mult;          // k-mer j stands for mult[j] equal rows (null: 1 each)
+  // FP6-digit resident kernel with a recheck pass (null otherwise): survivor
+  // entries it cannot decide (pre-filter or threshold inside its bounds) are
+  // appended here, proof bit kept, and neither counted nor reported; the
+  // streaming three-limb kernel then scans this list
+  uint32_t* rchk;
+  unsigned long long* rchk_n;    // entries appended to rchk
   EpiI8 ep;
 };
 

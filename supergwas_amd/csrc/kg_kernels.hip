@@ -316,7 +316,17 @@ __host__ __device__ constexpr int i8_tiles(int NT, int TT) { return (TT == 1 && 
 // SIMD never wait for each other.  Used for single-trait screened scans at two
 // limbs when nchunks x NT x 8 KiB fits in LDS; the survivor list's top bit
 // then carries the screen's proof of the pre-filter pass (pf_sure).
-template <int NT, int L, int TT, int R, bool RES>
+//
+// F6 (resident only): the weights are L FP6 e2m3 digits per column (base 30,
+// digit d stored as d / 8, |d| <= 15) against FP4 bit operands on
+// v_mfma_scale_f32_16x16x128_f8f6f4, as in the screen: shift k (0..3) keeps
+// bit k of every nibble of the lane's four dwords, so one MFMA covers 128
+// samples; the B scale (2, 1, 1/2, 1/2 for k = 0..3) makes every bit weigh 1,
+// so all four shifts accumulate into the same rows, and every sum is exact in
+// FP32.  Element j = 8 d + m of lane group g, shift k, is accession
+// 512 C + 128 g + 32 d + 4 m + k.  With D.rchk set, k-mers whose decision the
+// digit bounds leave open go to the recheck list instead of the candidates.
+template <int NT, int L, int TT, int R, bool RES, bool F6 = false>
 __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
                                                         int nshards, int64_t total_blocks,
                                                         ScanOut out,
@@ -328,6 +338,10 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
   constexpr int JSP = (TT + 4) / 4;                  // slots that may hold r_c or d columns
   static_assert(4 * J <= kMaxCols, "column records exceed the LDS table");
   static_assert(!RES || TT == 1, "resident weights: single-trait epilogue");
+  static_assert(!F6 || (RES && (L == 2 || L == 3)), "FP6 digits: resident form, 2 or 3 digits");
+  typedef int v8i __attribute__((ext_vector_type(8)));
+  typedef int v2i __attribute__((ext_vector_type(2)));
+  using AccV = std::conditional_t<F6, v4f, v4i>;
   // Separate LDS objects, so that reads of one buffer do not wait for the
   // direct-to-LDS loads still filling another.
   __shared__ v4i buf0[RES ? 1 : FRAG2];
@@ -382,11 +396,11 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     if (wn < total_blocks) block_idx<R>(sd[sn], wn, wave, col, nk, nridx);
     const uint64_t* nrowp[R];
 
-    v4i acc[R][NT];
+    AccV acc[R][NT];
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int t = 0; t < NT; ++t) acc[r][t] = v4i{0, 0, 0, 0};
+      for (int t = 0; t < NT; ++t) acc[r][t] = AccV{0, 0, 0, 0};
     uint32_t cnt[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) cnt[r] = 0;
@@ -396,7 +410,9 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         // every wave is past the previous model's weights and constants
         __syncthreads();
         const v4i* __restrict__ gw = reinterpret_cast<const v4i*>(D.wts);
-        const int nv = nchunks * 8 * NT * 64;
+        // int8: 8 steps x NT tiles of 16-byte operands per chunk; FP6: 4 shifts
+        // x NT tiles of 24-byte operands (16-byte part, then 8-byte part)
+        const int nv = F6 ? nchunks * 4 * NT * 64 * 3 / 2 : nchunks * 8 * NT * 64;
         for (int i = threadIdx.x; i < nv; i += kI8Threads) wres[i] = gw[i];
         const double* __restrict__ gcr = ep.colrec;
         for (int i = threadIdx.x; i < ep.ncol * kColRec; i += kI8Threads) colrec_lds[i] = gcr[i];
@@ -426,6 +442,57 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
           mcur = gptr(reinterpret_cast<const v4u*>(D.mask))[4 * Cn + g];
         }
         __builtin_amdgcn_sched_barrier(0);
+        if constexpr (F6) {
+          // fragments q = NT k + t (shift k, tile t); fragment q + PD is read
+          // while q's MFMAs run, shift k + 1's operands are formed beside
+          // shift k's MFMAs
+          const v4i* const xc = wres + (size_t)C * (4 * NT * 64);
+          const v2i* const yc = reinterpret_cast<const v2i*>(wres + (size_t)nchunks * (4 * NT * 64)) +
+                                (size_t)C * (4 * NT * 64);
+          constexpr int NQ = 4 * NT;
+          constexpr int PD = 4;
+          v8i af[PD + 1];
+          v4u bf[R], bn[R];
+          auto opnd = [&](int r, int k) -> v4u {
+            if (k < 3) {
+              const uint32_t M = 0x11111111u << k;
+              return v4u{wd[r][0] & M, wd[r][1] & M, wd[r][2] & M, wd[r][3] & M};
+            }
+            return v4u{(wd[r][0] >> 1) & 0x44444444u, (wd[r][1] >> 1) & 0x44444444u,
+                       (wd[r][2] >> 1) & 0x44444444u, (wd[r][3] >> 1) & 0x44444444u};
+          };
+          auto afrag = [&](int q) -> v8i {
+            const v4i a0 = xc[q * 64 + lane];
+            const v2i a1 = yc[q * 64 + lane];
+            return v8i{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, 0, 0};
+          };
+#pragma unroll
+          for (int r = 0; r < R; ++r) bf[r] = opnd(r, 0);
+#pragma unroll
+          for (int i = 0; i < PD; ++i) af[i] = afrag(i);
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const int k = q / NT, t = q % NT;
+            if (k + 1 < 4) {
+#pragma unroll
+              for (int r = t; r < R; r += NT) bn[r] = opnd(r, k + 1);
+            }
+            if (q + PD < NQ) af[(q + PD) % (PD + 1)] = afrag(q + PD);
+            // B scale (E8M0): bit values 0.5, 1, 2, 2 -> 1
+            constexpr int kSb[4] = {128, 127, 126, 126};
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const v8i b = v8i{(int)bf[r].x, (int)bf[r].y, (int)bf[r].z, (int)bf[r].w, 0, 0, 0, 0};
+              acc[r][t] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                  af[q % (PD + 1)], b, acc[r][t], 2, 4, 0, 127, 0, kSb[k]);
+            }
+            if (t == NT - 1 && k + 1 < 4) {
+#pragma unroll
+              for (int r = 0; r < R; ++r) bf[r] = bn[r];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        } else {
         const v4i* const wc = wres + (size_t)C * (8 * NT * 64);
         // the same software pipeline as the streaming form: fragment q + PD is
         // read while q's MFMAs run, step jj + 1's operands are unpacked beside
@@ -455,6 +522,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
           }
           __builtin_amdgcn_sched_barrier(0);
         }
+        }  // int8 limbs
       }
       // the next block's first chunk, under this block's epilogue
       if (wn < total_blocks) {
@@ -595,15 +663,27 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
       const int sl = L * jc;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
+        if constexpr (F6) {
+          // digit sums A_l (exact FP32 multiples of 1/8): z = w0 sum_l 30^(L-1-l) A_l,
+          // exact in FP32 for two digits (|30 A_0 + A_1| < 2^21) and in FP64 for three
+          const float d0 = acc[r][sl >> 2][sl & 3], d1 = acc[r][(sl + 1) >> 2][(sl + 1) & 3];
+          if (L == 2) {
+            zc[r][jc] = w0 * (double)fmaf(30.0f, d0, d1);
+          } else {
+            const float d2 = acc[r][(sl + 2) >> 2][(sl + 2) & 3];
+            zc[r][jc] = w0 * fma(900.0, (double)d0, fma(30.0, (double)d1, (double)d2));
+          }
+          continue;
+        }
         const int a0 = acc[r][sl >> 2][sl & 3];
         if (L == 3) {
           // limbs 1 and 2 merged exactly: |a| <= 127 n < 2^23 for n <= 60000, so
           // the 24-bit multiply-add is exact
-          const int a12 = __mul24(acc[r][(sl + 1) >> 2][(sl + 1) & 3], 254) +
-                          acc[r][(sl + 2) >> 2][(sl + 2) & 3];
+          const int a12 = __mul24((int)acc[r][(sl + 1) >> 2][(sl + 1) & 3], 254) +
+                          (int)acc[r][(sl + 2) >> 2][(sl + 2) & 3];
           zc[r][jc] = fma(w2, (double)a12, w0 * (double)a0);
         } else if (L == 2) {
-          zc[r][jc] = fma(w1, (double)acc[r][(sl + 1) >> 2][(sl + 1) & 3], w0 * (double)a0);
+          zc[r][jc] = fma(w1, (double)(int)acc[r][(sl + 1) >> 2][(sl + 1) & 3], w0 * (double)a0);
         } else {
           zc[r][jc] = w0 * (double)a0;
         }
@@ -642,7 +722,8 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     // survivor lists: the row, and (top bit) the screen's proof of the pass
     const uint32_t lrow = (D.rows != nullptr && valid) ? gptr(D.rows)[kidx] : 0u;
     const int64_t krow = (D.rows != nullptr && valid) ? (int64_t)(lrow & kRowMask) : kidx;
-    const bool pf_sure = RES && (lrow & ~kRowMask) != 0u;
+    // (single-trait lists only: a multi-trait screen proves nothing)
+    const bool pf_sure = (RES || TT == 1) && (lrow & ~kRowMask) != 0u;
     const uint32_t kmult = (D.mult != nullptr && valid) ? gptr(D.mult)[kidx] : 1u;
     // |dz_k| <= c' h_k (c' = min(c, n - c))  =>
     //   |d |z|^2| <= 2c' sum|z_k| h_k + c'^2 sum h_k^2 <= 2c' |z| |h| + c'^2 |h|^2
@@ -693,6 +774,25 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         bool pf_def = false;
         uint32_t fl = 0;
         if (valid) fl = classify_i8(c, xxM, bxx, S, bS, xyM, bxy, ep, tc[1], tc[2], pf_sure, &pf_def);
+        if constexpr (F6) {
+          if (D.rchk != nullptr) {
+            // an open pre-filter or threshold goes to the recheck pass, which
+            // counts and reports the k-mer itself
+            const bool rc = (fl & (kPfAmb | kSigAmb)) != 0u;
+            if (rc) {
+              fl = 0;
+              pf_def = false;
+            }
+            const unsigned long long bal = __ballot(rc);
+            if (bal != 0ull) {
+              const int leader = __ffsll((long long)bal) - 1;
+              unsigned long long base = 0;
+              if (lane == leader) base = atomicAdd(D.rchk_n, (unsigned long long)__popcll(bal));
+              base = __shfl(base, leader);
+              if (rc) gptr_w(D.rchk)[base + __popcll(bal & ((1ull << lane) - 1ull))] = lrow;
+            }
+          }
+        }
         if (out.st_c != nullptr && valid) {
           out.st_xy[kidx * t + tr] = xyM;
           out.st_S[kidx * t + tr] = S;
@@ -1630,6 +1730,62 @@ bool i8_res_fits(int NT, int nch) {
 // k-mers per workgroup block of the resident kernel
 int64_t i8_res_block_kmers() { return (int64_t)kI8Waves * 16 * 4; }
 
+// FP6-digit resident instantiations: D = 2 or 3 digits per column, NT = 1..6.
+constexpr int kF6MaxTiles = 6;
+template <int NT, int D>
+static hipError_t launch_f6_nt(dim3 grid, size_t shm, hipStream_t s, const ShardDesc* sd,
+                               int nshards, int64_t total_blocks, const ScanOut& o,
+                               const int64_t* total_dev) {
+  auto kfn = scan_i8<NT, D, 1, 4, true, true>;
+  hipLaunchKernelGGL(kfn, grid, dim3(kI8Threads), shm, s, sd, nshards, total_blocks, o, total_dev);
+  return hipGetLastError();
+}
+
+size_t f6_res_lds_bytes(int NT, int nch) { return (size_t)nch * 4 * NT * 64 * kF6OperandBytes; }
+
+bool f6_res_fits(int NT, int nch) {
+  return NT >= 1 && NT <= kF6MaxTiles && f6_res_lds_bytes(NT, nch) <= kResLdsBytes;
+}
+
+hipError_t launch_scan_f6_res(int NT, int D, int nch_max, int grid, hipStream_t s,
+                              const ShardDesc* sd, int nshards, int64_t total_blocks,
+                              const ScanOut& o, const int64_t* total_dev) {
+  if (!f6_res_fits(NT, nch_max) || (D != 2 && D != 3)) return hipErrorInvalidValue;
+  dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
+  const size_t shm = f6_res_lds_bytes(NT, nch_max);
+  switch (NT * 4 + D) {
+#define KG_CASE(X, Y) \
+  case X * 4 + Y:     \
+    return launch_f6_nt<X, Y>(g, shm, s, sd, nshards, total_blocks, o, total_dev);
+    KG_CASE(1, 2) KG_CASE(2, 2) KG_CASE(3, 2) KG_CASE(4, 2) KG_CASE(5, 2) KG_CASE(6, 2)
+    KG_CASE(1, 3) KG_CASE(2, 3) KG_CASE(3, 3) KG_CASE(4, 3) KG_CASE(5, 3) KG_CASE(6, 3)
+#undef KG_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+// Block table of the recheck pass: shard s scans its rchk_n[s] listed rows in
+// blocks of bk k-mers; the total goes to *total (one thread: shard tables are
+// small).
+__global__ void rchk_tables(ShardDesc* __restrict__ sd3, int nsh, int64_t bk,
+                            int64_t* __restrict__ total) {
+  if (threadIdx.x != 0) return;
+  int64_t b = 0;
+  for (int s = 0; s < nsh; ++s) {
+    const int64_t m = (int64_t)*sd3[s].rchk_n;
+    sd3[s].m = m;
+    sd3[s].blk0 = b;
+    b += (m + bk - 1) / bk;
+  }
+  *total = b;
+}
+
+hipError_t launch_rchk_tables(hipStream_t s, ShardDesc* sd3, int nsh, int64_t bk, int64_t* total) {
+  hipLaunchKernelGGL(rchk_tables, dim3(1), dim3(64), 0, s, sd3, nsh, bk, total);
+  return hipGetLastError();
+}
+
 hipError_t launch_scan_i8_res(int NT, int nch_max, int grid, hipStream_t s, const ShardDesc* sd,
                               int nshards, int64_t total_blocks, const ScanOut& o,
                               const int64_t* total_dev) {
@@ -1848,6 +2004,16 @@ hipError_t set_kernel_attributes() {
                            (const void*)scan_i8<3, 2, 1, 4, true>,
                            (const void*)scan_i8<4, 2, 1, 4, true>};
   for (const void* f : res_fns)
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLdsBytes);
+  const void* f6_fns[] = {
+      (const void*)scan_i8<1, 2, 1, 4, true, true>, (const void*)scan_i8<2, 2, 1, 4, true, true>,
+      (const void*)scan_i8<3, 2, 1, 4, true, true>, (const void*)scan_i8<4, 2, 1, 4, true, true>,
+      (const void*)scan_i8<5, 2, 1, 4, true, true>, (const void*)scan_i8<6, 2, 1, 4, true, true>,
+      (const void*)scan_i8<1, 3, 1, 4, true, true>, (const void*)scan_i8<2, 3, 1, 4, true, true>,
+      (const void*)scan_i8<3, 3, 1, 4, true, true>, (const void*)scan_i8<4, 3, 1, 4, true, true>,
+      (const void*)scan_i8<5, 3, 1, 4, true, true>, (const void*)scan_i8<6, 3, 1, 4, true, true>};
+  for (const void* f : f6_fns)
     if (e == hipSuccess)
       e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLdsBytes);
   const int scr = (int)scr_lds_bytes(kScrMaxChunks);

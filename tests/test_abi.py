@@ -62,12 +62,13 @@ def test_bad_options_rejected():
 
 
 def test_option_and_timing_layouts():
-    # kg_opts: 8 int32; kg_timing: 3 doubles, 2 int64, 3 doubles, 2 int64
+    # kg_opts: 8 int32; kg_timing: 3 doubles, 2 int64, 3 doubles, 3 int64; kg_tune: 8 int32
     assert ctypes.sizeof(api._Opts) == 32
-    assert ctypes.sizeof(api._Timing) == 80
+    assert ctypes.sizeof(api._Timing) == 88
     assert api._Opts.screen.offset == 12 and api._Timing.survivors.offset == 64
-    assert api._Timing.stat_refined.offset == 72
+    assert api._Timing.stat_refined.offset == 72 and api._Timing.rechecked.offset == 80
     assert api._Opts.dedup.offset == 16
+    assert ctypes.sizeof(api.Tune) == 32 and api.Tune.resident_form.offset == 28
 
 
 def test_null_context_calls():

@@ -352,6 +352,59 @@ def test_every_launch_shape_gives_identical_results(built, t):
         assert len(kpw) >= 2
 
 
+@pytest.mark.parametrize("name", ["n2048_p32", "n1024_p20", "n333_nointercept"])
+def test_resident_forms_identical(built, name):
+    """The resident kernel's operand forms -- two int8 limbs, two FP6 digits against FP4 bits
+    (with the three-limb recheck pass of the k-mers its bounds leave open), three FP6 digits --
+    and the streaming kernel report the same hits, values and counts, equal to the FP64
+    oracle outside the +-1e-6 band.  The p threshold sits on a passing k-mer's own p-value near
+    T = 2, in the bulk of the statistic's distribution, so that many k-mers fall inside the
+    two-digit bounds and the recheck pass has work."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    kw, m, corr, _, _ = CASES[name]
+    ds = dataset(name, **kw)
+    bits = ds.fill_bits(0, m)
+    ref = oracle.scan_f64(bits, ds.idxs, ds.y, ds.U, corr)
+    pas = np.nonzero(ref["passed"])[0]
+    j = pas[np.argmin(np.abs(ref["T"][pas] - 2.0))]
+    l10 = float(-np.log10(ref["p"][j]))
+    l2d = l10 + 1.0
+    prm = ScanParams(corr, l10, l2d)
+    out = {}
+    with KgwasHip(screen="on") as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        dev = k.to_device(bits)
+        for form in (2, 1, 3):
+            for stw in (0, 1):
+                k.set_tune(0, 1, 1, stw, form)
+                out[(form, stw)] = (k.scan_device(dev.ptr, m, bits.shape[1], prm), k.timing(),
+                                    k.get_tune())
+        dev.free()
+    (h0, n0), t0, tune0 = out[(2, 0)]
+    print(f"[resident forms] {name} n={ds.n} p={ds.p} m={m} log10={l10:.6f} hits={len(h0)} "
+          f"passed={int(n0[0])} survivors={t0['survivors']} rechecked={t0['rechecked']} "
+          f"candidates={t0['candidates']}", flush=True)
+    assert tune0["resident_form"] == 2 and tune0["resident_active"] == 1
+    assert out[(1, 0)][2]["resident_form"] == 1 and out[(3, 0)][2]["resident_form"] == 3
+    assert t0["rechecked"] > 0
+    for (form, stw), ((h, n), tm, _) in out.items():
+        if form != 2 or stw:
+            assert tm["rechecked"] == 0, (form, stw)
+        assert np.array_equal(n, n0), (form, stw)
+        for f in ("idx", "trait", "c", "T", "effect", "pval", "flags"):
+            assert np.array_equal(h[f], h0[f]), (form, stw, f)
+    cor = np.abs(pearson(bits, ds.idxs, ds.y))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        near_c = np.abs(cor / corr - 1.0) <= BAND
+        near_p = np.abs(ref["p"] / 10.0 ** (-l10) - 1.0) <= BAND
+    assert abs(int(n0[0]) - ref["n_passed"]) <= int(near_c.sum())
+    exp_idx, _ = expected_hits(ref, l10, l2d)
+    sym = np.setxor1d(exp_idx, h0["idx"])
+    assert np.all((near_c | near_p)[sym]), sym[:10]
+    assert np.array_equal(h0["c"], ref["c"][h0["idx"]])
+    assert np.allclose(h0["T"], ref["T"][h0["idx"]], rtol=REPORT_TOL, atol=0)
+
+
 def _shard_setup(k, nsh=5, seed=30):
     """Shards of uneven size, each with its own phenotyped-sample mask (own model)."""
     sizes = [1, 300, 0, 5003, 2500][:nsh]
