@@ -227,9 +227,15 @@ typedef struct kg_tune {
                                   model's default, 4 for one trait and at most 6 weight
                                   tiles, else 2; 2 is available at 3 limbs) */
   int32_t workgroups_per_cu;   /* full kernel: persistent 8-wave workgroups per CU (1-64) */
-  int32_t screen_wgs_per_cu;   /* pre-filter screen: 1 (one 16-wave workgroup per CU for one
-                                  trait, one 8-wave workgroup otherwise) or 2 (two 8-wave
-                                  workgroups of at most 128 registers per wave) */
+  int32_t screen_wgs_per_cu;   /* pre-filter screen shape: 1 (one 16-wave workgroup per CU for
+                                  one trait, one 8-wave workgroup otherwise, rows loaded into
+                                  registers), 2 (two 8-wave workgroups of at most 128 registers
+                                  per wave), 3 (one trait, models of 4 or 8 512-sample chunks:
+                                  one 8-wave workgroup per CU streaming its rows through an LDS
+                                  ring with contiguous nontemporal loads), 4 (the same with 12
+                                  waves, 4-chunk models); 0 in kg_set_tune: the default (3
+                                  where it applies, else 1).  kg_get_tune reports the shape
+                                  model 0's scans use */
   int32_t kmers_per_workgroup; /* out: k-mers per full-kernel workgroup block */
   int32_t screen_kmers_per_workgroup; /* out: k-mers per screen workgroup block */
   int32_t stream_weights;      /* 0: single-trait screened scans run their survivors through

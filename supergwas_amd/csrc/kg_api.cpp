@@ -33,6 +33,9 @@ int screen_rows(int t);
 bool screen_count_row(int t);
 int screen_max_chunks();
 int screen_wgs_per_cu(int t, int nch, bool dd, bool w8);
+bool screen_ring_fits(int nch, int W);
+hipError_t launch_screen_ring(int W, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
+                              int nshards, int64_t total_blocks, bool dd);
 hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
                          int nshards, int64_t total_blocks, bool dd, bool w8);
 int screen_chunk_words();
@@ -256,6 +259,9 @@ struct kg_ctx {
   int tiles = 0;      // int8 scan: k-mer tiles per wave (0 = the kernel's default)
   int grid_mult_f32 = 8;  // FP32 scan: workgroups per CU
   bool scr_w8 = false;    // screen: the 8-wave workgroup shape (kg_autotune may pick it)
+  // one-trait screen of models of 4 or 8 chunks: rows streamed through an LDS
+  // ring by workgroups of this many waves (8 or 12; 0: the register-loaded screen)
+  int scr_ring = 8;
   bool resident = true;   // single-trait screened scans: the resident-weight kernel
   // its operands: 0 = two int8 limbs; 2 or 3 = FP6 digits per column against
   // FP4 bits (2: with the three-limb recheck pass of the k-mers it leaves open)
@@ -1145,6 +1151,7 @@ struct ScreenRun {
   int nch_max = 0;
   bool dd = false;              // with the duplicate-row check
   bool w8 = false;              // the screen's 8-wave shape
+  int ring = 0;                 // the LDS-ring screen's waves per workgroup (0: not used)
   bool res = false;             // survivors through the resident-weight kernel
   int res_nt = 0;               // its weight tiles
   int res_d = 0;                // its FP6 digits per column (0: two int8 limbs)
@@ -1171,8 +1178,12 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
   if (!zeroed) KG_CK(ctx, launch_zero(ctx->stream, o.counters, (int)ctx->counters_len));
   KG_CK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (sr.on) {
-    KG_CK(ctx, launch_screen(t, sr.nch_max, sr.sgrid, ctx->stream, sr.d_ss, nsh, sr.sblocks,
-                             sr.dd, sr.w8));
+    if (sr.ring)
+      KG_CK(ctx, launch_screen_ring(sr.ring, sr.nch_max, sr.sgrid, ctx->stream, sr.d_ss, nsh,
+                                    sr.sblocks, sr.dd));
+    else
+      KG_CK(ctx, launch_screen(t, sr.nch_max, sr.sgrid, ctx->stream, sr.d_ss, nsh, sr.sblocks,
+                               sr.dd, sr.w8));
     KG_CK(ctx, hipEventRecord(ctx->evs, ctx->stream));
     KG_CK(ctx, launch_screen_compact(ctx->stream, sr.d_ch, sr.ncc, sr.d_ss, sr.ccount, sr.P,
                                      sr.d_sd2, nsh, sr.bk, sr.d_total));
@@ -1369,7 +1380,12 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     sr.bk = use_res ? i8_res_block_kmers() : bk;
     sr.bk3 = i8_block_kmers(NT, ctx->limbs, t, 2);
     sr.w8 = ctx->scr_w8;
-    const int64_t sbk = screen_block_kmers(t, sr.w8);
+    // the LDS-ring screen where every model fits it (one trait, 4 or 8 chunks)
+    sr.ring = (t == 1) ? ctx->scr_ring : 0;
+    for (const auto& s : sh)
+      if (sr.ring && !screen_ring_fits(ctx->models[s.model]->md.nch_i8, sr.ring)) sr.ring = 0;
+    if (sr.ring && !screen_ring_fits(nch_max, sr.ring)) sr.ring = 0;
+    const int64_t sbk = sr.ring ? 64 * (int64_t)sr.ring : screen_block_kmers(t, sr.w8);
     const int64_t cw = screen_chunk_words();
     int64_t bm_words = 0;
     // one trait (the 4-row screen): each 64 k-mers' kept words are followed by
@@ -1481,7 +1497,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       ctx->scr_occ = screen_wgs_per_cu(t, nch_max, dd, sr.w8);
       ctx->scr_occ_key = okey;
     }
-    sr.sgrid = ctx->cus * ctx->scr_occ;
+    sr.sgrid = sr.ring ? ctx->cus : ctx->cus * ctx->scr_occ;  // the ring: one workgroup per CU
   }
   if (ctx->cand_cap == 0) {
     int64_t mt = 0;
@@ -1559,7 +1575,9 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
       kp += sizeof(RefDesc) * nsh;
       memcpy(kp, &o, sizeof(o));
       kp += sizeof(o);
-      const int64_t extra[2] = {(int64_t)flags, ((int64_t)ctx->grid_mult * 64 + ctx->tiles) * 2 + (ctx->scr_w8 ? 1 : 0)};
+      const int64_t extra[2] = {(int64_t)flags,
+                                (((int64_t)ctx->grid_mult * 64 + ctx->tiles) * 2 + (ctx->scr_w8 ? 1 : 0)) * 16 +
+                                    sr.ring};
       memcpy(kp, extra, 16);
       key.insert(key.end(), reinterpret_cast<const char*>(&hb),
                  reinterpret_cast<const char*>(&hb) + sizeof(hb));
@@ -2246,16 +2264,22 @@ static void fill_tune(const kg_ctx* ctx, kg_tune* out) {
   out->tiles_per_wave = ctx->tiles;
   out->workgroups_per_cu = ctx->grid_mult;
   out->screen_wgs_per_cu = ctx->scr_w8 ? 2 : 1;
+  int ring = 0;
   if (!ctx->models.empty() && ctx->precision == KG_PREC_MIXED) {
     const ModelDev& M = *ctx->models[0];
     out->tiles_per_wave = i8_tiles_for(M.NT, ctx->limbs, M.md.t, ctx->tiles);
     out->kmers_per_workgroup = (int32_t)i8_block_kmers(M.NT, ctx->limbs, M.md.t, ctx->tiles);
-    out->screen_kmers_per_workgroup = (int32_t)screen_block_kmers(M.md.t, ctx->scr_w8);
+    // the shape model 0's screened scans use: the ring where it fits
+    ring = (M.md.t == 1 && ctx->scr_ring && screen_ring_fits(M.md.nch_i8, ctx->scr_ring))
+               ? ctx->scr_ring : 0;
+    out->screen_kmers_per_workgroup =
+        (int32_t)(ring ? 64 * ring : screen_block_kmers(M.md.t, ctx->scr_w8));
     out->resident_active =
         (ctx->resident && M.screen_ok && (M.res_ok || (M.f6_ok && M.f6_digits == ctx->res_digits)))
             ? 1 : 0;
   }
   out->stream_weights = ctx->resident ? 0 : 1;
+  if (ring) out->screen_wgs_per_cu = ring == 8 ? 3 : 4;
   out->resident_form = ctx->res_digits == 0 ? 1 : ctx->res_digits;
 }
 
@@ -2271,8 +2295,8 @@ extern "C" int kg_set_tune(kg_ctx* ctx, const kg_tune* shape) {
     return fail(ctx, KG_ERR_ARG, "tiles_per_wave must be 0, 2 or 4");
   if (shape->workgroups_per_cu < 1 || shape->workgroups_per_cu > 64)
     return fail(ctx, KG_ERR_ARG, "workgroups_per_cu must be 1-64");
-  if (shape->screen_wgs_per_cu < 0 || shape->screen_wgs_per_cu > 2)
-    return fail(ctx, KG_ERR_ARG, "screen_wgs_per_cu must be 0, 1 or 2");
+  if (shape->screen_wgs_per_cu < 0 || shape->screen_wgs_per_cu > 4)
+    return fail(ctx, KG_ERR_ARG, "screen_wgs_per_cu must be 0-4");
   if (shape->stream_weights < 0 || shape->stream_weights > 1)
     return fail(ctx, KG_ERR_ARG, "stream_weights must be 0 or 1");
   if (shape->resident_form < 0 || shape->resident_form > 3)
@@ -2292,6 +2316,9 @@ extern "C" int kg_set_tune(kg_ctx* ctx, const kg_tune* shape) {
   ctx->tiles = shape->tiles_per_wave == 2 ? 2 : 0;
   ctx->grid_mult = shape->workgroups_per_cu;
   ctx->scr_w8 = shape->screen_wgs_per_cu == 2;
+  // 0: the default (the 8-wave ring where it fits, else one 16-wave workgroup)
+  ctx->scr_ring = shape->screen_wgs_per_cu == 4 ? 12
+                  : (shape->screen_wgs_per_cu == 0 || shape->screen_wgs_per_cu == 3) ? 8 : 0;
   ctx->resident = shape->stream_weights == 0;
   drop_graph(ctx);
   return KG_OK;
@@ -2324,10 +2351,12 @@ extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, kg_tune* out) {
     const int keep_tiles = ctx->tiles, keep_gm = ctx->grid_mult, keep_mode = ctx->screen_mode,
               keep_skip = ctx->screen_skip;
     const bool keep_w8 = ctx->scr_w8;
+    const int keep_ring = ctx->scr_ring;
     auto restore = [&]() {
       ctx->tiles = keep_tiles;
       ctx->grid_mult = keep_gm;
       ctx->scr_w8 = keep_w8;
+      ctx->scr_ring = keep_ring;
       ctx->screen_mode = keep_mode;
       ctx->screen_skip = keep_skip;
     };
@@ -2373,13 +2402,23 @@ extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, kg_tune* out) {
       }
     ctx->tiles = best_R;
     ctx->grid_mult = best_gm;
-    // the screen's workgroups per CU (where the model has a screen)
+    // the screen's shape (where the model has a screen): the default first
+    // (the 8-wave LDS ring where it fits), then the register-loaded shapes
+    // (one 16-wave or two 8-wave workgroups per CU) and the 12-wave ring
     bool best_w8 = keep_w8;
+    int best_ring = keep_ring;
     if (M.screen_ok) {
       ctx->screen_mode = KG_SCREEN_ON;
       double sbest = 1e300;
-      for (int v = 0; v < 2; ++v) {
-        ctx->scr_w8 = v == 1;
+      std::vector<std::pair<bool, int>> shapes;  // (w8, ring waves)
+      const bool ring_ok = md.t == 1 && screen_ring_fits(md.nch_i8, 8);
+      if (ring_ok) shapes.push_back({false, 8});
+      shapes.push_back({false, 0});
+      shapes.push_back({true, 0});
+      if (md.t == 1 && screen_ring_fits(md.nch_i8, 12)) shapes.push_back({false, 12});
+      for (const auto& sh : shapes) {
+        ctx->scr_w8 = sh.first;
+        ctx->scr_ring = sh.second;
         double tmin = 0.0;
         const int rc = time_scan(true, &tmin);
         if (rc) {
@@ -2388,11 +2427,15 @@ extern "C" int kg_autotune(kg_ctx* ctx, int64_t sample, kg_tune* out) {
         }
         if (tmin < sbest * 0.98) {
           sbest = tmin;
-          best_w8 = v == 1;
+          best_w8 = sh.first;
+          best_ring = sh.second;
         }
       }
+      // a register-loaded shape keeps the ring setting off only when it won
+      if (best_ring == 0 && !ring_ok) best_ring = keep_ring;
     }
     ctx->scr_w8 = best_w8;
+    ctx->scr_ring = best_ring;
     ctx->screen_mode = keep_mode;
     ctx->screen_skip = keep_skip;
     drop_graph(ctx);

@@ -1014,6 +1014,49 @@ __device__ __forceinline__ void scr_group_f6(const v4u (&q)[DC][R], int C0, int 
   }
 }
 
+// Decision of the one-trait FP6 screen for the k-mer lane (g, col) owns after
+// the reduce-scatter of R tiles: c from the count row, S~ = w (27000 D_01 +
+// 30 D_23 + D_4) from the digit sums; keep unless |cor| < tau is certain, sure
+// when |cor| >= tau is proven.  acc[r][0] holds shift 0 (rows 0-7, bit value
+// 0.5) or 1 (rows 8-15, value 1), acc[r][1] shift 2 or 3 (value 2); a digit d
+// weighs d / 8; lane groups 0, 2 hold digits 0-3, groups 1, 3 digit 4 and the
+// count row.  The test runs in FP32: every FP32 value is within 2e-6 relative
+// of its exact counterpart (a handful of correctly rounded operations on
+// positive terms), so the margin 1e-5 keeps "certainly fails" certain; for the
+// proof |S~| is scaled down by 4e-6 and h was rounded up, so lo <= |S| before
+// its last rounding, and the roundings of lo^2 and of the threshold product
+// total below 5e-7 relative.
+template <int R>
+__device__ __forceinline__ void f6_decide(const v4f (&acc)[R][2], const ScreenDesc& D, int g,
+                                          int* c_out, bool* keep, bool* sure) {
+  uint32_t x01[R], x23[R], x4[R], cpart[R];
+  const float m0 = g < 2 ? 16.0f : 8.0f;
+  const bool lo_g = (g & 1) == 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = (int)fmaf(acc[r][0][i], m0, acc[r][1][i] * 4.0f);
+    // D_01 = 30 D_0 + D_1 and D_23 = 30 D_2 + D_3 are exact in int32 (|.| <= 465 n)
+    x01[r] = lo_g ? (uint32_t)(30 * v[0] + v[1]) : 0u;
+    x23[r] = lo_g ? (uint32_t)(30 * v[2] + v[3]) : 0u;
+    x4[r] = lo_g ? 0u : (uint32_t)v[0];
+    cpart[r] = lo_g ? 0u : (uint32_t)v[1];
+  }
+  const int c = (int)group_scatter<R>(cpart);
+  const int D01 = (int)group_scatter<R>(x01), D23 = (int)group_scatter<R>(x23);
+  const int D4 = (int)group_scatter<R>(x4);
+  // the digit combination in int64, so S~ rounds once
+  const float sA = (float)((int64_t)D01 * 27000 + (int64_t)D23 * 30 + D4);
+  const float cf = (float)c, vv = cf * (float)(D.n - c);
+  const float aS = fabsf(D.wf[0] * sA);
+  const float hi = aS * (1.0f + 4e-6f) + cf * D.hf[0];
+  const float lo = aS * (1.0f - 4e-6f) - cf * D.hf[0];
+  *keep = D.thrf[0] >= 0.0f && !(hi * hi < D.thrf[0] * vv * (1.0f - 1e-5f));
+  *sure = D.thrf[0] >= 0.0f && lo > 0.0f && lo * lo >= D.thrf[0] * vv * (1.0f + 1e-5f);
+  *c_out = c;
+}
+
 // Software pipeline over load groups (block, first chunk): the next group's
 // rows (the same block's next chunks, or the next block's first) are loaded
 // into a second register set while the current group computes, then copied
@@ -1104,88 +1147,53 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
       scr_group<KG, NA, R, DC, CR, DD>(cq, C0, nch, wl, mk_lds, g, lane, acc, cnt, ne);
 
     if (last_group) {
+      const int64_t kbase = ((w - D.blk0) * W + wave) * (16 * R);
+      const int64_t kidx = kbase + 16 * (g % R) + col;
+      const bool valid = (g < R) && kidx < D.m;
+      const int n = D.n;
+      int c = 0;
+      bool keep = false, sure = false;
+      if constexpr (F6) {
+        f6_decide<R>(acc, D, g, &c, &keep, &sure);
+      } else {
       // exact row totals over the shifts (every row of accumulator a is a
       // multiple of 2^k, k = a KG + ksh), then per-lane partials
       // 254 A_0 + A_1 of trait slots tb, tb + 1 and of the count row
-      int part[R][2], p4[R];
+      int part[R][2];
       uint32_t cpart[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         int v[4] = {0, 0, 0, 0};
-        if constexpr (F6) {
-          // digit sums: accumulator 0 holds shift 0 (rows 0-7, value 0.5) or 1
-          // (rows 8-15, value 1), accumulator 1 shift 2 or 3 (value 2); a digit
-          // d weighs d / 8.  Lane groups 0, 2: digits 0-3; 1, 3: digit 4, count
-          const float m0 = g < 2 ? 16.0f : 8.0f;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) v[i] = (int)fmaf(acc[r][0][i], m0, acc[r][1][i] * 4.0f);
-          const bool lo = (g & 1) == 0;
-          part[r][0] = lo ? 30 * v[0] + v[1] : 0;
-          part[r][1] = lo ? 30 * v[2] + v[3] : 0;
-          p4[r] = lo ? 0 : v[0];
-          cpart[r] = lo ? 0u : (uint32_t)v[1];
-        } else {
+        for (int a = 0; a < NA; ++a) {
+          const int kk = a * KG + ksh;
+          const int sg = (kk == 7) ? -1 : 1;
 #pragma unroll
-          for (int a = 0; a < NA; ++a) {
-            const int kk = a * KG + ksh;
-            const int sg = (kk == 7) ? -1 : 1;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] += sg * (acc[r][a][i] >> kk);
-          }
-          part[r][0] = 254 * v[0] + v[1];
-          part[r][1] = 254 * v[2] + v[3];
-          p4[r] = 0;
-          if (CR) cpart[r] = (uint32_t)((tb == D.t) ? v[0] : ((tb + 1 == D.t) ? v[2] : 0));
+          for (int i = 0; i < 4; ++i) v[i] += sg * (acc[r][a][i] >> kk);
         }
+        part[r][0] = 254 * v[0] + v[1];
+        part[r][1] = 254 * v[2] + v[3];
+        if (CR) cpart[r] = (uint32_t)((tb == D.t) ? v[0] : ((tb + 1 == D.t) ? v[2] : 0));
       }
-      const int c = CR ? (int)group_scatter<R>(cpart) : (int)group_scatter<R>(cnt);
-      const int64_t kbase = ((w - D.blk0) * W + wave) * (16 * R);
-      const int64_t kidx = kbase + 16 * (g % R) + col;
-      const bool valid = (g < R) && kidx < D.m;
+      c = CR ? (int)group_scatter<R>(cpart) : (int)group_scatter<R>(cnt);
       // Decision in FP32: every FP32 value below is within 2e-6 relative of
       // its exact counterpart (a handful of correctly rounded operations on
       // positive terms), so the margin 1e-5 keeps "certainly fails" certain.
-      const int n = D.n;
       const float cf = (float)c, vv = cf * (float)(n - c);
-      bool keep = false, sure = false;
 #pragma unroll
       for (int tr = 0; tr < TT; ++tr) {
         if (tr < D.t) {
-          float sA;
-          if constexpr (F6) {
-            // D_01 = 30 D_0 + D_1 and D_23 = 30 D_2 + D_3 are exact in int32
-            // (|.| <= 465 n); the digit combination in int64, so S~ rounds once
-            uint32_t x01[R], x23[R], x4[R];
+          uint32_t x[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-              x01[r] = (uint32_t)part[r][0];
-              x23[r] = (uint32_t)part[r][1];
-              x4[r] = (uint32_t)p4[r];
-            }
-            const int D01 = (int)group_scatter<R>(x01), D23 = (int)group_scatter<R>(x23);
-            const int D4 = (int)group_scatter<R>(x4);
-            sA = (float)((int64_t)D01 * 27000 + (int64_t)D23 * 30 + D4);
-          } else {
-            uint32_t x[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-              x[r] = (uint32_t)((tb == tr) ? part[r][0] : ((tb + 1 == tr) ? part[r][1] : 0));
-            sA = (float)(int)group_scatter<R>(x);
-          }
+          for (int r = 0; r < R; ++r)
+            x[r] = (uint32_t)((tb == tr) ? part[r][0] : ((tb + 1 == tr) ? part[r][1] : 0));
+          const float sA = (float)(int)group_scatter<R>(x);
           const float aS = fabsf(D.wf[tr] * sA);
           const float hi = aS * (1.0f + 4e-6f) + cf * D.hf[tr];
           if (D.thrf[tr] >= 0.0f && !(hi * hi < D.thrf[tr] * vv * (1.0f - 1e-5f))) keep = true;
-          if constexpr (Q == 4) {
-            // one trait: the pass is proven when (|S~| - c h)^2 clears the
-            // threshold by 1e-5 (|S~| is scaled down by 4e-6 and h was rounded
-            // up, so lo <= |S| before its last rounding; the roundings of lo^2
-            // and of the threshold product total below 5e-7 relative)
-            const float lo = aS * (1.0f - 4e-6f) - cf * D.hf[tr];
-            sure = D.thrf[tr] >= 0.0f && lo > 0.0f &&
-                   lo * lo >= D.thrf[tr] * vv * (1.0f + 1e-5f);
-          }
         }
       }
+      }  // int8 limbs
       keep = keep && valid && c > 0 && c < n;
       if (DD) {
         // dup bit 16 r + col: every lane group's slice of row kbase + 16 r + col
@@ -1239,6 +1247,297 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
 #pragma unroll
       for (int r = 0; r < R; ++r) cq[u][r] = nq[u][r];
   }
+}
+
+// ----------------------------------------------------------------------------
+// One-trait screen with its rows streamed through an LDS ring (the same test
+// as screen_i8<4, ...>: five FP6 digits of y - ybar and a count row against
+// FP4 bits).  Every load instruction of a wave reads a contiguous KiB -- four
+// whole 256-byte row segments -- with the nontemporal hint, straight into LDS
+// (global_load_lds: nothing held in registers while in flight).  A linear
+// nontemporal stream reads at about 7.0 TB/s on this part, against about
+// 6.3 TB/s for the 64-byte row slices the register-loaded screen reads (with
+// the hint those slices drop to 5.3 TB/s).
+//
+// Unit = 16 rows x 4 chunks (4 KiB, one ring slot); a wave has NBUF units in
+// flight.  Lane L of load instruction u (rows 4u .. 4u + 3 of the unit)
+// fetches the 16 bytes of row 4u + rho, chunk C, part g that belong in slot
+// L = 16 C + (4 rho + g + u) % 16, so the operand reads (lane (g, col) takes
+// row col, chunk C, part g) hit 16 distinct bank groups in every 16
+// consecutive lanes.  A wave block is 4 tiles of 16 k-mers (64 k-mers), a
+// workgroup block W of those; units run tile by tile, segment (4 chunks) by
+// segment.  Models of 4 or 8 chunks (kRingMaxSeg segments).
+//
+// WREG (4-chunk models): the model's 16 weight fragments (4 chunks x 4 shifts)
+// stay in registers, loaded from global memory by each wave when its model
+// changes, so the LDS serves only the rows (and has room for a fourth ring
+// slot); the LDS weight reads would otherwise be 2 KiB per k-mer, since a ring
+// slot holds one tile and every fragment would be read once per tile.
+//
+// The wave's bitmap words collect in an LDS stash and go out in batches: a
+// global store between the ring's loads would leave the vmcnt wait that
+// tracks them unreliable (stores and loads may retire out of order with
+// respect to each other), so each batch store is followed by a full wait.
+constexpr int kRingMaxSeg = 2;
+constexpr int kRingStash = 32;                // bitmap records per wave between stores
+constexpr int kRingAux = 2;                   // cache policy of the ring loads: nt
+// LDS: the accession mask, the weights (none with WREG), the ring, the stash
+__host__ __device__ constexpr size_t ring_lds_bytes(int W, int NBUF, int nch, bool wreg) {
+  return (size_t)nch * 64 + (wreg ? 0 : (size_t)nch * 8 * 64 * 16) + (size_t)W * NBUF * 4096 +
+         (size_t)W * kRingStash * 40;
+}
+
+template <int W, int NBUF, bool DD, bool WREG>
+__global__ __launch_bounds__(64 * W) void screen_ring(const ScreenDesc* __restrict__ sd,
+                                                      int nshards, int64_t total_blocks,
+                                                      int nch_max) {
+  extern __shared__ v4i scr_lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // scalar row arithmetic
+  const int g = lane >> 4, col = lane & 15;
+  const v4u* const mk_lds = reinterpret_cast<const v4u*>(scr_lds);
+  v4i* const wl = scr_lds + 4 * nch_max;
+  const int nwl = WREG ? 0 : 512 * nch_max;  // v4i of LDS weights
+  v4u* const ring = reinterpret_cast<v4u*>(wl + nwl) + (size_t)wave * NBUF * 256;
+  // stash: [W][kRingStash] 16-byte words, then their addresses, then (DD) dup
+  // words and addresses
+  v4u* const st_w = reinterpret_cast<v4u*>(wl + nwl) + (size_t)W * NBUF * 256 +
+                    (size_t)wave * kRingStash;
+  uint64_t* const st_a = reinterpret_cast<uint64_t*>(reinterpret_cast<v4u*>(wl + nwl) +
+                                                     (size_t)W * NBUF * 256 + W * kRingStash) +
+                         (size_t)wave * kRingStash;
+  uint64_t* const st_d = st_a + (size_t)W * kRingStash;
+  uint64_t* const st_da = st_d + (size_t)W * kRingStash;
+
+  int64_t w = blockIdx.x;
+  if (w >= total_blocks) return;
+  // consumer shard
+  int s = 0;
+  while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
+  // producer position: block, shard, tile, segment; `done` once past the last unit
+  int64_t pw = w;
+  int ps = s, pt = 0, pg = 0;
+  bool pdone = false;
+  // this lane's 16 bytes of load u (the KiB of rows 4u .. 4u + 3 of a unit):
+  // row 4u + rho, chunk C = lane / 16, part g, so that it lands in slot lane
+  int lrow[4], lcol[4];  // row within the unit; offset within the row (8-byte words)
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int x = ((lane & 15) - u) & 15;
+    lrow[u] = 4 * u + (x >> 2);
+    lcol[u] = 8 * (lane >> 4) + 2 * (x & 3);
+  }
+  // the producer's shard, reloaded when it changes: rows, row stride, last row,
+  // segments, first block, first block of the next shard; per-lane offsets
+  const uint64_t* p_bits = nullptr;
+  uint32_t p_rw = 0, p_last = 0;
+  int p_nseg = 1;
+  int64_t p_blk0 = 0, p_next0 = 0;
+  uint32_t loff[4];
+  auto load_p = [&]() {
+    const ScreenDesc& P = sd[ps];
+    p_bits = P.bits;
+    p_rw = (uint32_t)P.rw;
+    p_last = (uint32_t)(P.m - 1);
+    p_nseg = P.nchunks / 4;
+    p_blk0 = P.blk0;
+    p_next0 = (ps + 1 < nshards) ? sd[ps + 1].blk0 : total_blocks;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) loff[u] = (uint32_t)lrow[u] * p_rw + (uint32_t)lcol[u];
+  };
+  load_p();
+  auto issue = [&](int buf) {
+    const uint32_t kbase = (uint32_t)(((pw - p_blk0) * W + wave) * 64 + 16 * pt);
+    if (kbase + 15 <= p_last) {  // every row of the unit exists: one add per load
+      const uint64_t* rb = p_bits + (uint64_t)kbase * p_rw + 32 * pg;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        __builtin_amdgcn_global_load_lds(
+            (const void*)(rb + loff[u]),
+            (__attribute__((address_space(3))) void*)(ring + buf * 256 + u * 64), 16, 0, kRingAux);
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        uint32_t k = kbase + (uint32_t)lrow[u];
+        if (k > p_last) k = p_last;  // rows past m are read, never kept
+        const uint64_t* src = p_bits + (uint64_t)k * p_rw + 32 * pg + lcol[u];
+        __builtin_amdgcn_global_load_lds(
+            (const void*)src, (__attribute__((address_space(3))) void*)(ring + buf * 256 + u * 64),
+            16, 0, kRingAux);
+      }
+    }
+  };
+  auto advance = [&]() {
+    if (pdone) return;
+    if (++pg < p_nseg) return;
+    pg = 0;
+    if (++pt < 4) return;
+    pt = 0;
+    pw += gridDim.x;
+    if (pw >= total_blocks) {
+      pdone = true;  // the last unit is issued again (into a slot never read)
+      pw -= gridDim.x;
+      pt = 3;
+      pg = p_nseg - 1;
+      return;
+    }
+    if (pw >= p_next0) {
+      while (ps + 1 < nshards && sd[ps + 1].blk0 <= pw) ++ps;
+      load_p();
+    }
+  };
+  int jb = 0;  // ring slot of the unit the consumer takes next
+#pragma unroll
+  for (int i = 0; i < NBUF - 1; ++i) {
+    issue(i);
+    advance();
+  }
+
+  const void* staged = nullptr;
+  int nst = 0;  // stashed records of this wave
+  auto flush = [&]() {
+    if (lane < nst) {
+      *gptr_w(reinterpret_cast<v4u*>(st_a[lane])) = st_w[lane];
+      if (DD) *gptr_w(reinterpret_cast<uint64_t*>(st_da[lane])) = st_d[lane];
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // the stores, and every ring load in flight
+    nst = 0;
+  };
+  v4f acc[4][2];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = v4f{0.f, 0.f, 0.f, 0.f};
+  uint32_t ne[4] = {0, 0, 0, 0};
+  typedef int v8i __attribute__((ext_vector_type(8)));
+  v8i wa[WREG ? 4 : 1][WREG ? 4 : 1];  // WREG: fragments [chunk][shift] (MFMA operand form)
+
+  while (true) {
+    const ScreenDesc& D = sd[s];
+    const int nseg = WREG ? 1 : D.nchunks / 4;
+    if (WREG && D.wts != staged) {  // this wave's fragments, straight from global memory
+      const auto* gw = gptr(reinterpret_cast<const v4i*>(D.wts));
+#pragma unroll
+      for (int C = 0; C < 4; ++C)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const v4i a0 = gw[((C * 4 + k) * 64 + lane) * 2];
+          const v4i a1 = gw[((C * 4 + k) * 64 + lane) * 2 + 1];
+          wa[C][k] = v8i{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, 0, 0};
+        }
+      // every load in flight, the ring's included, has landed: the explicit
+      // counts of the ring waits then cover ring loads only
+      __builtin_amdgcn_s_waitcnt(0);
+      asm volatile("" ::: "memory");
+      staged = D.wts;
+    }
+    if (!WREG && D.wts != staged) {  // workgroup-uniform: every wave walks the same blocks
+      __syncthreads();
+      const v4i* __restrict__ gw = reinterpret_cast<const v4i*>(D.wts);
+      const v4u* __restrict__ gm = reinterpret_cast<const v4u*>(D.mask);
+      for (int i = threadIdx.x; i < D.nchunks * 512; i += 64 * W) wl[i] = gw[i];
+      for (int i = threadIdx.x; i < D.nchunks * 4; i += 64 * W)
+        reinterpret_cast<v4u*>(scr_lds)[i] = gm[i];
+      __syncthreads();
+      staged = D.wts;
+    }
+    v4u prev[kRingMaxSeg][4];  // DD: previous tile's rotated rows (lane 0: its row 15)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int sg = 0; sg < kRingMaxSeg; ++sg) {
+        if (sg < nseg) {
+          // the unit NBUF - 1 ahead, into the slot the previous unit was read
+          // from: those reads have returned first (lgkmcnt(0)), and the fence
+          // keeps the refill below that wait
+          __builtin_amdgcn_s_waitcnt(0xC07F);
+          asm volatile("" ::: "memory");
+          issue(jb == 0 ? NBUF - 1 : jb - 1);
+          advance();
+          constexpr int K = 4 * (NBUF - 1);
+          __builtin_amdgcn_s_waitcnt((K & 15) | (7 << 4) | (15 << 8) | ((K >> 4) << 14));
+          asm volatile("" ::: "memory");  // the slot's reads stay after the wait
+          const v4u* ub = ring + jb * 256 + 64 * (col >> 2);
+          v4u q[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) q[c] = ub[16 * c + ((4 * (col & 3) + g + (col >> 2)) & 15)];
+          jb = jb + 1 == NBUF ? 0 : jb + 1;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int C = 4 * sg + c;
+            if (DD) {
+              const v4u mk = mk_lds[4 * C + g];
+              const v4u rr = v4u{ror16(q[c].x), ror16(q[c].y), ror16(q[c].z), ror16(q[c].w)};
+              const v4u pv = (t > 0 && col == 0) ? prev[sg][c] : rr;
+              ne[t] |= ((q[c].x ^ pv.x) & mk.x) | ((q[c].y ^ pv.y) & mk.y) |
+                       ((q[c].z ^ pv.z) & mk.z) | ((q[c].w ^ pv.w) & mk.w);
+              prev[sg][c] = rr;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              v8i a;
+              if constexpr (WREG) {
+                a = wa[c][k];
+              } else {
+                const v4i a0 = wl[((C * 4 + k) * 64 + lane) * 2];
+                const v4i a1 = wl[((C * 4 + k) * 64 + lane) * 2 + 1];
+                a = v8i{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, 0, 0};
+              }
+              v8i b;
+              if (k < 3) {
+                const uint32_t M = 0x11111111u << k;
+                b = v8i{(int)(q[c].x & M), (int)(q[c].y & M), (int)(q[c].z & M),
+                        (int)(q[c].w & M), 0, 0, 0, 0};
+              } else {
+                b = v8i{(int)((q[c].x >> 1) & 0x44444444u), (int)((q[c].y >> 1) & 0x44444444u),
+                        (int)((q[c].z >> 1) & 0x44444444u), (int)((q[c].w >> 1) & 0x44444444u),
+                        0, 0, 0, 0};
+              }
+              acc[t][k >> 1] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                  a, b, acc[t][k >> 1], 2, 4, 0, 127, 0, 127);
+            }
+          }
+        }
+      }
+    }
+    // the block's decision (as screen_i8<4, ...>)
+    const int64_t kbase = ((w - D.blk0) * W + wave) * 64;
+    const int64_t kidx = kbase + 16 * g + col;
+    int c = 0;
+    bool keep = false, sure = false;
+    f6_decide<4>(acc, D, g, &c, &keep, &sure);
+    keep = keep && kidx < D.m && c > 0 && c < D.n;
+    unsigned long long dm = 0;
+    if (DD) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const unsigned long long b = __ballot(ne[r] != 0u);
+        const uint32_t d16 = (uint32_t)((b | (b >> 16) | (b >> 32) | (b >> 48)) & 0xFFFFull);
+        dm |= (unsigned long long)(~d16 & 0xFFFFu) << (16 * r);
+        ne[r] = 0;
+      }
+      dm &= ~1ull;
+      const int64_t nv = D.m - kbase;
+      if (nv < 64) dm &= nv <= 0 ? 0ull : ((1ull << nv) - 1ull);
+      keep = keep && ((dm >> lane) & 1ull) == 0;
+    }
+    const unsigned long long bal = __ballot(keep);
+    const unsigned long long pb = __ballot(keep && sure);
+    if (lane == 0) {
+      st_w[nst] = v4u{(uint32_t)bal, (uint32_t)(bal >> 32), (uint32_t)pb, (uint32_t)(pb >> 32)};
+      st_a[nst] = (uint64_t)(reinterpret_cast<v4u*>(D.bmap) + (kbase >> 6));
+      if (DD) {
+        st_d[nst] = dm;
+        st_da[nst] = (uint64_t)(D.dmap + (kbase >> 5));
+      }
+    }
+    ++nst;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = v4f{0.f, 0.f, 0.f, 0.f};
+    w += gridDim.x;
+    if (w >= total_blocks) break;
+    if (nst == kRingStash) flush();
+    while (s + 1 < nshards && sd[s + 1].blk0 <= w) ++s;
+  }
+  flush();
 }
 
 // Physical index of kept-bitmap word w: one trait interleaves each 64 k-mers'
@@ -1894,6 +2193,53 @@ hipError_t launch_screen(int t, int nch_max, int grid, hipStream_t s, const Scre
   return hipLaunchKernel(fn, g, blk, args, shm, s);
 }
 
+// The ring screen (one trait): W = 8 or 12 waves per workgroup, one workgroup
+// per CU; 12 waves take 4-chunk models only (LDS).
+bool screen_ring_fits(int nch, int W) {
+  return nch >= 4 && nch % 4 == 0 && nch <= 4 * kRingMaxSeg && (W == 8 || (W == 12 && nch == 4));
+}
+
+// Instantiation for (W, nch_max, dd): 8 waves on 4-chunk models keep the weights
+// in registers and four ring slots (with the duplicate check, whose registers
+// that leaves no room for, the weights in LDS and three slots); 8-chunk models
+// and 12 waves read the weights from LDS with two slots.
+struct RingCfg {
+  int nbuf;
+  bool wreg;
+};
+static RingCfg ring_cfg(int W, int nch_max, bool dd) {
+  if (W == 8 && nch_max <= 4) return dd ? RingCfg{3, false} : RingCfg{4, true};
+  return RingCfg{2, false};
+}
+size_t screen_ring_lds(int W, int nch_max, bool dd) {
+  const RingCfg c = ring_cfg(W, nch_max, dd);
+  return ring_lds_bytes(W, c.nbuf, nch_max, c.wreg);
+}
+
+template <int W, int NBUF, bool DD, bool WREG>
+static const void* ring_fn() {
+  return (const void*)screen_ring<W, NBUF, DD, WREG>;
+}
+
+static const void* ring_kernel(int W, int nch_max, bool dd) {
+  const RingCfg c = ring_cfg(W, nch_max, dd);
+  if (W == 8) {
+    if (c.wreg) return ring_fn<8, 4, false, true>();
+    if (c.nbuf == 3) return ring_fn<8, 3, true, false>();
+    return dd ? ring_fn<8, 2, true, false>() : ring_fn<8, 2, false, false>();
+  }
+  return dd ? ring_fn<12, 2, true, false>() : ring_fn<12, 2, false, false>();
+}
+
+hipError_t launch_screen_ring(int W, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
+                              int nshards, int64_t total_blocks, bool dd) {
+  if (!screen_ring_fits(nch_max, W)) return hipErrorInvalidValue;
+  const void* fn = ring_kernel(W, nch_max, dd);
+  dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
+  void* args[] = {(void*)&sd, (void*)&nshards, (void*)&total_blocks, (void*)&nch_max};
+  return hipLaunchKernel(fn, g, dim3(64 * W), args, screen_ring_lds(W, nch_max, dd), s);
+}
+
 // Workgroups of the screen one CU holds at once for a model of nch chunks.
 int screen_wgs_per_cu(int t, int nch, bool dd, bool w8) {
   int nb = 0;
@@ -2017,6 +2363,13 @@ hipError_t set_kernel_attributes() {
     if (e == hipSuccess)
       e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLdsBytes);
   const int scr = (int)scr_lds_bytes(kScrMaxChunks);
+  for (int W : {8, 12})
+    for (int nch : {4, 8})
+      for (int dd = 0; dd < 2; ++dd)
+        if (e == hipSuccess && screen_ring_fits(nch, W))
+          e = hipFuncSetAttribute(ring_kernel(W, nch, dd != 0),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)screen_ring_lds(W, nch, dd != 0));
   for (int t = 1; t <= kMaxTraits; ++t)
     for (int v = 0; v < 4; ++v)
       if (e == hipSuccess)

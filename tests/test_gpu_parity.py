@@ -305,7 +305,7 @@ def test_autotune_keeps_results(built):
         h1, n1 = k.scan(bits, prm)
     print(f"[autotune] {tune}", flush=True)
     assert tune["tiles_per_wave"] in (2, 4) and tune["workgroups_per_cu"] in (1, 2, 4)
-    assert tune["screen_wgs_per_cu"] in (1, 2)
+    assert tune["screen_wgs_per_cu"] in (1, 2, 3, 4)
     assert np.array_equal(n0, n1)
     assert np.array_equal(h0["idx"], h1["idx"]) and np.array_equal(h0["T"], h1["T"])
 
@@ -327,7 +327,7 @@ def test_every_launch_shape_gives_identical_results(built, t):
         dev = k.to_device(bits)
         for tiles in (0, 2):
             for gm in (1, 2, 4):
-                for sw in (1, 2):
+                for sw in (1, 2, 3):
                     for stw in (0, 1):
                         k.set_tune(tiles, gm, sw, stw)
                         for mode in ("on", "off"):
@@ -348,7 +348,7 @@ def test_every_launch_shape_gives_identical_results(built, t):
             assert np.array_equal(h[f], h0[f]), (key, f)
     print(f"[shapes] t={t}: {len(out)} shapes, (k-mers per workgroup, screen) {sorted(kpw)}",
           flush=True)
-    if t == 1:  # one trait: 4 or 2 tiles per wave, 16- or 8-wave screen workgroups
+    if t == 1:  # one trait: 4 or 2 tiles per wave, 16- or 8-wave screen workgroups, the ring
         assert len(kpw) >= 2
 
 
@@ -712,3 +712,69 @@ def test_screen_odd_shapes(built, n_acc, m, t):
         exp, _ = expected_hits(ref, 1.5, 3.0)
         assert n1[tr] == ref["n_passed"], tr
         assert np.array_equal(h1[h1["trait"] == tr]["idx"], exp), tr
+
+
+@pytest.mark.parametrize("n_acc", [2048, 4000])
+def test_ring_screen_identical(built, n_acc):
+    """The one-trait screen that streams its rows through an LDS ring (kg_tune screen shapes 3
+    and 4) keeps exactly the k-mers the register-loaded screen (shape 1) keeps: same survivor
+    counts, hits and pass counts, on uneven shards of two models with missing samples (4 and
+    8 chunks, partial last blocks, an empty shard), with and without the duplicate-row check,
+    and equal to the FP64 oracle per shard."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    from tests.test_gpu_dedup import planted_rows
+    base = synth.make_dataset(seed=90 + n_acc % 7, n_acc=n_acc, n_missing=0, npop=6, p=12,
+                              m=30000, n_causal=20, effect_sd=0.4)
+    rng = np.random.default_rng(n_acc)
+    rw = synth.row_words_for(n_acc)
+    models = []
+    for mi in range(2):
+        miss = rng.choice(n_acc, size=int(0.02 * (mi + 1) * n_acc), replace=False)
+        idxs = np.setdiff1d(np.arange(n_acc), miss).astype(np.int32)
+        U, _, _ = np.linalg.svd(base.U[np.isin(base.idxs, idxs)], full_matrices=False)
+        y = base.y[np.isin(base.idxs, idxs)] + 0.3 * rng.standard_normal(len(idxs))
+        models.append((idxs, U, y))
+    sizes = [7001, 1, 0, 12345, 4096, 577]
+    rows_all = planted_rows(base, 12000, seed=n_acc)[: sum(sizes)]
+    prm = ScanParams(0.15, 2.0, 3.0)
+    out = {}
+    with KgwasHip(screen="on") as k:
+        k.set_model(n_acc, *models[0])
+        k.add_model(n_acc, *models[1])
+        shards, devs, refs, k0 = [], [], [], 0
+        for si, m in enumerate(sizes):
+            bits = rows_all[k0:k0 + m]
+            k0 += m
+            dev = k.to_device(bits) if m else k.alloc(rw * 8)
+            devs.append(dev)
+            shards.append((dev.ptr, m, rw, si % 2))
+            refs.append(bits)
+        shapes = (1, 3, 4) if n_acc <= 2048 else (1, 3)
+        for dd in (False, True):
+            k.set_dedup(dd)
+            for sw in shapes:
+                k.set_tune(0, 1, sw, 0)
+                h, n = k.scan_shards(shards, prm)
+                out[(dd, sw)] = (h.copy(), n.copy(), k.timing()["survivors"], k.get_tune())
+        for d in devs:
+            d.free()
+    h0, n0, s0, _ = out[(False, 1)]
+    print(f"[ring screen] n_acc={n_acc} hits={len(h0)} passed={n0[:, 0].tolist()} survivors={s0}",
+          flush=True)
+    assert out[(False, 3)][3]["screen_wgs_per_cu"] == 3
+    for (dd, sw), (h, n, sv, tune) in out.items():
+        assert np.array_equal(n, n0), (dd, sw)
+        if not dd:
+            assert sv == s0, (dd, sw, sv, s0)
+        else:
+            assert sv == out[(True, 1)][2], (dd, sw)
+        for f in ("idx", "shard", "c", "T", "flags"):
+            assert np.array_equal(h[f], h0[f]), (dd, sw, f)
+    for si, bits in enumerate(refs):
+        if len(bits) == 0:
+            continue
+        idxs, U, y = models[si % 2]
+        ref = oracle.scan_f64(bits, idxs, y, U, 0.15)
+        assert n0[si, 0] == ref["n_passed"], si
+        exp, _ = expected_hits(ref, 2.0, 3.0)
+        assert np.array_equal(h0[h0["shard"] == si]["idx"], exp), si
