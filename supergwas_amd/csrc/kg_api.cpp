@@ -638,7 +638,7 @@ static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_o
   // column records {w0, w1, w2, k_0..k_7, E, h^2, 0}; zero past nb
   std::vector<double> rec((size_t)ncol * kColRec, 0.0);
   double sum_h2 = 0.0, sum_E2 = 0.0;
-  std::vector<double> sum_hbp(kMaxTraits, 0.0), sum_Ek(kMaxTraits, 0.0);
+  std::vector<double> sum_hbp(kMaxTraits, 0.0), sum_Ek(kMaxTraits, 0.0), sum_k2(kMaxTraits, 0.0);
   for (int b = 0; b < nb; ++b) {
     double* r = &rec[(size_t)b * kColRec];
     for (int l = 0; l < L; ++l) r[l] = colw[(size_t)b * L + l];
@@ -655,12 +655,13 @@ static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_o
         const double bpv = md.bprime[(size_t)tr * p + k];
         r[3 + tr] = bpv;
         sum_hbp[tr] += colh[b] * std::fabs(bpv);
+        sum_k2[tr] += bpv * bpv;
         sum_Ek[tr] += colE[b] * bpv;
       }
     }
   }
   // per-trait records {mean r, |y - ybar|, yyM, b'_0 (dropped d column), h of r_c,
-  // E of r_c, sum_b E_b k_tr}
+  // E of r_c, sum_b E_b k_tr, |k_tr| over the structure columns}
   std::vector<double> trc((size_t)t * kTrcRec, 0.0);
   for (int tr = 0; tr < t; ++tr) {
     double* r = &trc[(size_t)tr * kTrcRec];
@@ -671,6 +672,7 @@ static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_o
     r[4] = colh[tr];
     r[5] = colE[tr];
     r[6] = sum_Ek[tr];
+    r[7] = std::sqrt(sum_k2[tr]) * (1.0 + 1e-12);  // norm of the structure weights k_tr
   }
   auto up = [&](DBuf& d, const void* src, size_t bytes) -> hipError_t {
     hipError_t e = d.ensure(bytes);

@@ -77,7 +77,8 @@ struct EpiI8 {
   double inv_n;     // 1 / n
   const double* colrec; // [ncol][kColRec] {w0, w1, w2, k_0..k_7, E, h^2, 0}: staged in LDS
   const double* trc;    // [t][kTrcRec] {mean r, |y - ybar|, yyM, b'_0, h of r_c, E of r_c,
-                        //               sum_b E_b k_tr (the mean correction of S)}
+                        //               sum_b E_b k_tr (the mean correction of S),
+                        //               |k_tr| over the structure columns}
   double sum_h2;        // sum over z columns of h^2   (|z|^2 bound: 2c' |z| |h| + c'^2 |h|^2)
   double sum_hbp[8];    // per trait: sum over z columns of h |b'|
   double sum_E2;        // sum over z columns of E^2

@@ -268,6 +268,20 @@ __device__ __forceinline__ uint32_t swap32(uint32_t x, uint32_t y) {
   const auto v = __builtin_amdgcn_permlane32_swap(x, y, false, false);
   return v[0] + v[1];
 }
+__device__ __forceinline__ float swap16(float x, float y) {
+  return __uint_as_float(0u) + [&] {
+    const auto v = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false,
+                                                    false);
+    return __uint_as_float(v[0]) + __uint_as_float(v[1]);
+  }();
+}
+__device__ __forceinline__ float swap32(float x, float y) {
+  return __uint_as_float(0u) + [&] {
+    const auto v = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false,
+                                                    false);
+    return __uint_as_float(v[0]) + __uint_as_float(v[1]);
+  }();
+}
 __device__ __forceinline__ double swap16(double x, double y) {
   const uint64_t xb = (uint64_t)__double_as_longlong(x), yb = (uint64_t)__double_as_longlong(y);
   const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)xb, (uint32_t)yb, false, false);
@@ -421,6 +435,8 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         staged_w = D.wts;
         staged_cr = ep.colrec;
       }
+      // MFMA loop at raised priority: the other wave's epilogue fills its gaps
+      if constexpr (F6) __builtin_amdgcn_s_setprio(1);
       for (int C = 0; C < nchunks; ++C) {
         uint32_t wd[R][4];
 #pragma unroll
@@ -524,6 +540,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         }
         }  // int8 limbs
       }
+      if constexpr (F6) __builtin_amdgcn_s_setprio(0);
       // the next block's first chunk, under this block's epilogue
       if (wn < total_blocks) {
         rows_at<R>(sd[sn], nk, nridx, g, nrowp);
@@ -652,12 +669,49 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     const int rme = g % R;
     const bool primary = g < R;
     const bool est = ep.stat_est != 0;  // full-statistic mode: error estimate per k-mer
+    // FP6 digits (one trait): the column passes in FP32 -- z = w0 sum_l
+    // 30^(L-1-l) A_l from the exact digit sums, then |z|^2, z'E, the column d
+    // and the trait's own residual column, and (without a recheck pass) S; the
+    // per-k-mer totals go on in FP64, and the bounds take a relative 1e-5 for
+    // these FP32 roundings (under 4e-6 for sums of 32 terms).
+    constexpr bool F32E = F6;
+    float fzz[R], fzE[R], fz0[R], fsv[R], fS[R];
+    const bool f_S = F32E && D.rchk == nullptr;  // S of unproven k-mers decided here
+    if constexpr (F32E) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) fzz[r] = fzE[r] = fz0[r] = fsv[r] = fS[r] = 0.f;
+#pragma unroll
+      for (int jc = 0; jc < J; ++jc) {
+        const double* cr = colrec_lds + (4 * jc + g) * kColRec;
+        const float w0f = (float)cr[0], cEf = (float)cr[11], kcf = (float)cr[3];
+        const int b = 4 * jc + g;
+        const int sl = L * jc;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const float d0 = acc[r][sl >> 2][sl & 3], d1 = acc[r][(sl + 1) >> 2][(sl + 1) & 3];
+          float z;
+          if (L == 2) {
+            z = w0f * fmaf(30.0f, d0, d1);
+          } else {
+            const float d2 = acc[r][(sl + 2) >> 2][(sl + 2) & 3];
+            z = (float)(cr[0] * fma(900.0, (double)d0, fma(30.0, (double)d1, (double)d2)));
+          }
+          // column slot j < JSP: the trait's residual (b = 0) or column d (b = 1)
+          const float zs = (jc < JSP && b < t) ? 0.f : z;
+          fzz[r] = fmaf(zs, zs, fzz[r]);
+          fzE[r] = fmaf(zs, cEf, fzE[r]);
+          if (jc < JSP && b == t) fz0[r] = z;
+          if (jc < JSP && b == 0) fsv[r] = z;
+          if (f_S) fS[r] = fmaf(z, kcf, fS[r]);
+        }
+      }
+    }
     // this lane's column values z (FP64) of every tile, after which the limb
     // sums are dead: the per-trait passes below then need R registers of
     // partial sums each, not R x t
-    double zc[R][J];
+    double zc[F32E ? 1 : R][F32E ? 1 : J];
 #pragma unroll
-    for (int jc = 0; jc < J; ++jc) {
+    for (int jc = 0; jc < (F32E ? 0 : J); ++jc) {
       const double* cr = colrec_lds + (4 * jc + g) * kColRec;
       const double w0 = cr[0], w1 = cr[1], w2 = cr[2];
       const int sl = L * jc;
@@ -694,7 +748,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 #pragma unroll
     for (int r = 0; r < R; ++r) pzz[r] = pz0[r] = pzE[r] = pzh[r] = 0.0;
 #pragma unroll
-    for (int jc = 0; jc < J; ++jc) {
+    for (int jc = 0; jc < (F32E ? 0 : J); ++jc) {
       const double* cr = colrec_lds + (4 * jc + g) * kColRec;
       const double cE = cr[11], ch2 = cr[12];
       const int b = 4 * jc + g;
@@ -712,10 +766,13 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     const int c = (int)group_scatter<R>(cnt);
     const double cd = (double)c, cn = cd * ep.inv_n;
     // mean correction (kg_internal.h): z_k + (c/n) E_k for every column
-    const double kzE = group_scatter<R>(pzE);
-    const double kzz = group_scatter<R>(pzz) + cn * fma(cn, ep.sum_E2, 2.0 * kzE);
-    const double kz0 = group_scatter<R>(pz0) + cn * ep.E_d;
+    const double kzE = F32E ? (double)group_scatter<R>(fzE) : group_scatter<R>(pzE);
+    const double kzz = (F32E ? (double)group_scatter<R>(fzz) : group_scatter<R>(pzz)) +
+                       cn * fma(cn, ep.sum_E2, 2.0 * kzE);
+    const double kz0 = (F32E ? (double)group_scatter<R>(fz0) : group_scatter<R>(pz0)) + cn * ep.E_d;
     const double kzh = est ? group_scatter<R>(pzh) : 0.0;
+    // FP32 column sums: relative slack for their roundings
+    const double f32s = F32E ? 1e-5 : 0.0;
 
     const int64_t kidx = kbase + 16 * rme + col;
     const bool valid = primary && (kidx < D.m);
@@ -736,6 +793,8 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     }
     const double xxM = cd - cd * cd * ep.base_coef - cross - kzz;
     double bxx = bzz + bcross + 1e-11 * (cd + kzz + fabs(cross) + 1.0);
+    if (F32E)
+      bxx += f32s * (cd + kzz + fabs(cross) + 2.0 * cn * sqrt(fmax(kzz, 0.0) * ep.sum_E2) + 1.0);
     if (ep.dmax_drop > 0.0) {
       const double e0 = cd * ep.dmax_drop;
       bxx += 2.0 * cn * fabs(ep.sigma) * e0 + e0 * e0;
@@ -752,7 +811,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 #pragma unroll
         for (int r = 0; r < R; ++r) vS[r] = vsv[r] = 0.0;
 #pragma unroll
-        for (int jc = 0; jc < J; ++jc) {
+        for (int jc = 0; jc < (F32E ? 0 : J); ++jc) {
           const double kc = colrec_lds[(4 * jc + g) * kColRec + 3 + tr];
           const bool own = jc < JSP && 4 * jc + g == tr;
 #pragma unroll
@@ -762,15 +821,29 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
           }
         }
         const double* tc = trc_lds + tr * kTrcRec;
-        const double S = group_scatter<R>(vS) + cn * tc[6];
-        const double sv = group_scatter<R>(vsv) + cn * tc[5];
+        double S, sv;
+        if constexpr (F32E) {
+          // without S (recheck pass), an unproven pre-filter stays open: bS = inf
+          S = f_S ? (double)group_scatter<R>(fS) + cn * tc[6] : 0.0;
+          sv = (double)group_scatter<R>(fsv) + cn * tc[5];
+        } else {
+          S = group_scatter<R>(vS) + cn * tc[6];
+          sv = group_scatter<R>(vsv) + cn * tc[5];
+        }
         const double rbar = tc[0];
         const double es = cm * tc[4];
         double bS = cm * ep.sum_hbp[tr] + es;
         if (ep.dmax_drop > 0.0) bS += cd * ep.dmax_drop * fabs(tc[3]);
         bS += 1e-12 * (fabs(S) + fabs(sv) + 1.0);
+        if (F32E) {
+          // FP32 S = sum_k z_k k_k + x'r_c: its rounding is below 1e-5 (|z| |k| +
+          // |x'r_c| + |S|), |k| = the norm of the structure columns' weights k_tr
+          // (trait record tc[7])
+          bS = f_S ? bS + f32s * (sqrt(fmax(kzz, 0.0)) * tc[7] + fabs(S) + fabs(sv) + 1.0)
+                   : INFINITY;
+        }
         const double xyM = sv + cd * rbar;
-        const double bxy = es + 1e-12 * (fabs(sv) + cd * fabs(rbar) + 1.0);
+        const double bxy = es + (1e-12 + f32s) * (fabs(sv) + cd * fabs(rbar) + 1.0);
         bool pf_def = false;
         uint32_t fl = 0;
         if (valid) fl = classify_i8(c, xxM, bxx, S, bS, xyM, bxy, ep, tc[1], tc[2], pf_sure, &pf_def);
