@@ -1696,14 +1696,26 @@ __global__ __launch_bounds__(1024) void scr_scan(const uint32_t* __restrict__ cc
   if (tid == 1023) *total = part[1023];
 }
 
-// Wave v of chunk c owns a quarter of its words (at most 512: 8 per lane,
-// loaded at once); a wave pass covers two words (lane l: bit l % 32 of word
-// 2 j + l / 32 of the quarter), so the ballot order is row order and each
-// pass stores its rows contiguously.
+// Wave v of chunk c owns a quarter of its words (at most 512); a pass takes 64
+// words, one per lane: a wave prefix sum of their popcounts places each word's
+// rows, and each lane writes its own word's rows (lowest bit first), so the
+// list is in row order.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) {
+  const int lane = __lane_id();
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+    if (lane >= off) x += y;
+  }
+  *total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+  return x - v;
+}
+
 __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch,
                                                  const ScreenDesc* __restrict__ ss,
                                                  const int64_t* __restrict__ P) {
-  static_assert(kScrChunkWords == 4 * 512, "a wave's quarter is 8 words per lane");
+  static_assert(kScrChunkWords == 4 * 512, "a wave's quarter is 8 passes of 64 words");
   __shared__ uint32_t wcnt[4];
   const ScrChunk C = ch[blockIdx.x];
   const ScreenDesc& D = ss[C.shard];
@@ -1714,15 +1726,12 @@ __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch
   const uint32_t* __restrict__ bm = D.bmap;
   const bool dd = D.mult != nullptr;  // duplicate-row scan: write run lengths too
   const bool ilv = D.ilv != 0;        // one trait: proven passes set the entry's top bit
-  uint32_t wv[8], dv[8], pv[8];
+  uint32_t wv[8];
   uint32_t v = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int j = 64 * i + lane;
     wv[i] = (j < nw) ? bm[scr_word(a0 + j, ilv)] : 0u;
-    pv[i] = (ilv && j < nw) ? bm[scr_word(a0 + j, true) + 2] : 0u;
-    dv[i] = 0u;
-    if (dd && j < nw) dv[i] = D.dmap[a0 + j];
     v += __popc(wv[i]);
   }
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -1731,46 +1740,35 @@ __global__ __launch_bounds__(256) void scr_write(const ScrChunk* __restrict__ ch
   int64_t o = P[blockIdx.x] - P[D.chunk0];
   for (int i = 0; i < wave; ++i) o += wcnt[i];
   uint32_t* __restrict__ list = D.list;
-  const unsigned long long below = (1ull << lane) - 1ull;
-  const int h = lane >> 5;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     if (64 * i >= nw) break;
-    for (int j = 0; j < 32; ++j) {
-      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)wv[i], 2 * j);
-      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)wv[i], 2 * j + 1);
-      const uint32_t word = h ? hi : lo;
-      const bool set = ((word >> (lane & 31)) & 1u) != 0;
-      const unsigned long long bal = __ballot(set);
-      // the dup words, by every lane of the wave (dd is uniform)
-      uint32_t dlo = 0, dhi = 0;
+    const int j = 64 * i + lane;
+    const int64_t wi = a0 + j;  // this lane's word
+    uint32_t word = wv[i];
+    uint32_t total;
+    const uint32_t pre = wave_excl_scan((uint32_t)__popc(word), &total);
+    const uint32_t pw = (ilv && word != 0u) ? bm[scr_word(wi, true) + 2] : 0u;
+    const uint32_t dword = (dd && word != 0u) ? D.dmap[wi] : 0u;
+    int64_t pos = o + pre;
+    while (word != 0u) {
+      const int b = __builtin_ctz(word);
+      word &= word - 1u;
+      list[pos] = (uint32_t)(32 * wi + b) | (((pw >> b) & 1u) << 31);
       if (dd) {
-        dlo = (uint32_t)__builtin_amdgcn_readlane((int)dv[i], 2 * j);
-        dhi = (uint32_t)__builtin_amdgcn_readlane((int)dv[i], 2 * j + 1);
-      }
-      const uint32_t plo = (uint32_t)__builtin_amdgcn_readlane((int)pv[i], 2 * j);
-      const uint32_t phi = (uint32_t)__builtin_amdgcn_readlane((int)pv[i], 2 * j + 1);
-      if (set) {
-        const int64_t pos = o + __popcll(bal & below);
-        const int64_t wi = a0 + 64 * i + 2 * j + h;
-        const uint32_t proven = (((h ? phi : plo) >> (lane & 31)) & 1u) << 31;
-        list[pos] = (uint32_t)(32 * wi + (lane & 31)) | proven;
-        if (dd) {
-          // the run of dup bits after this row: to the end of its word, then
-          // into the next (a run never crosses a wave's first row, whose dup
-          // bit is clear, so two words hold it)
-          const uint32_t dword = h ? dhi : dlo;
-          const int b = lane & 31;
-          uint32_t run = b < 31 ? (uint32_t)__builtin_ctz(~(dword >> (b + 1))) : 0u;
-          if (run == (uint32_t)(31 - b) && (wi + 1) * 32 < D.m) {
-            const uint32_t nx = D.dmap[wi + 1];
-            run += nx == 0xFFFFFFFFu ? 32u : (uint32_t)__builtin_ctz(~nx);
-          }
-          D.mult[pos] = 1u + run;
+        // the run of dup bits after this row: to the end of its word, then
+        // into the next (a run never crosses a wave's first row, whose dup
+        // bit is clear, so two words hold it)
+        uint32_t run = b < 31 ? (uint32_t)__builtin_ctz(~(dword >> (b + 1))) : 0u;
+        if (run == (uint32_t)(31 - b) && (wi + 1) * 32 < D.m) {
+          const uint32_t nx = D.dmap[wi + 1];
+          run += nx == 0xFFFFFFFFu ? 32u : (uint32_t)__builtin_ctz(~nx);
         }
+        D.mult[pos] = 1u + run;
       }
-      o += __popcll(bal);
+      ++pos;
     }
+    o += total;
   }
 }
 
