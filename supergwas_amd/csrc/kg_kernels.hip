@@ -1099,22 +1099,35 @@ __device__ __forceinline__ void scr_group_f6(const v4u (&q)[DC][R], int C0, int 
 // proof |S~| is scaled down by 4e-6 and h was rounded up, so lo <= |S| before
 // its last rounding, and the roundings of lo^2 and of the threshold product
 // total below 5e-7 relative.
-template <int R>
-__device__ __forceinline__ void f6_decide(const v4f (&acc)[R][2], const ScreenDesc& D, int g,
-                                          int* c_out, bool* keep, bool* sure) {
-  uint32_t x01[R], x23[R], x4[R], cpart[R];
+// One tile's partials: a0 = shift 0 (lanes g < 2) or 1, a1 = shift 2 or 3.
+struct F6Part {
+  uint32_t x01, x23, x4, cp;
+};
+__device__ __forceinline__ F6Part f6_partial(const v4f& a0, const v4f& a1, int g) {
   const float m0 = g < 2 ? 16.0f : 8.0f;
   const bool lo_g = (g & 1) == 0;
+  int v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = (int)fmaf(a0[i], m0, a1[i] * 4.0f);
+  // D_01 = 30 D_0 + D_1 and D_23 = 30 D_2 + D_3 are exact in int32 (|.| <= 465 n)
+  F6Part p;
+  p.x01 = lo_g ? (uint32_t)(30 * v[0] + v[1]) : 0u;
+  p.x23 = lo_g ? (uint32_t)(30 * v[2] + v[3]) : 0u;
+  p.x4 = lo_g ? 0u : (uint32_t)v[0];
+  p.cp = lo_g ? 0u : (uint32_t)v[1];
+  return p;
+}
+
+template <int R>
+__device__ __forceinline__ void f6_finish(const F6Part (&pt)[R], const ScreenDesc& D,
+                                          int* c_out, bool* keep, bool* sure) {
+  uint32_t x01[R], x23[R], x4[R], cpart[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    int v[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = (int)fmaf(acc[r][0][i], m0, acc[r][1][i] * 4.0f);
-    // D_01 = 30 D_0 + D_1 and D_23 = 30 D_2 + D_3 are exact in int32 (|.| <= 465 n)
-    x01[r] = lo_g ? (uint32_t)(30 * v[0] + v[1]) : 0u;
-    x23[r] = lo_g ? (uint32_t)(30 * v[2] + v[3]) : 0u;
-    x4[r] = lo_g ? 0u : (uint32_t)v[0];
-    cpart[r] = lo_g ? 0u : (uint32_t)v[1];
+    x01[r] = pt[r].x01;
+    x23[r] = pt[r].x23;
+    x4[r] = pt[r].x4;
+    cpart[r] = pt[r].cp;
   }
   const int c = (int)group_scatter<R>(cpart);
   const int D01 = (int)group_scatter<R>(x01), D23 = (int)group_scatter<R>(x23);
@@ -1128,6 +1141,15 @@ __device__ __forceinline__ void f6_decide(const v4f (&acc)[R][2], const ScreenDe
   *keep = D.thrf[0] >= 0.0f && !(hi * hi < D.thrf[0] * vv * (1.0f - 1e-5f));
   *sure = D.thrf[0] >= 0.0f && lo > 0.0f && lo * lo >= D.thrf[0] * vv * (1.0f + 1e-5f);
   *c_out = c;
+}
+
+template <int R>
+__device__ __forceinline__ void f6_decide(const v4f (&acc)[R][2], const ScreenDesc& D, int g,
+                                          int* c_out, bool* keep, bool* sure) {
+  F6Part pt[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) pt[r] = f6_partial(acc[r][0], acc[r][1], g);
+  f6_finish<R>(pt, D, c_out, keep, sure);
 }
 
 // Software pipeline over load groups (block, first chunk): the next group's
@@ -1341,11 +1363,15 @@ void screen_i8(const ScreenDesc* __restrict__ sd, int nshards, int64_t total_blo
 // workgroup block W of those; units run tile by tile, segment (4 chunks) by
 // segment.  Models of 4 or 8 chunks (kRingMaxSeg segments).
 //
-// WREG (4-chunk models): the model's 16 weight fragments (4 chunks x 4 shifts)
-// stay in registers, loaded from global memory by each wave when its model
-// changes, so the LDS serves only the rows (and has room for a fourth ring
-// slot); the LDS weight reads would otherwise be 2 KiB per k-mer, since a ring
-// slot holds one tile and every fragment would be read once per tile.
+// WREG (4-chunk models): the model's weight fragments stay in registers,
+// loaded from global memory by each wave when its model changes, so the LDS
+// serves only the rows (and has room for more ring slots); the LDS weight reads
+// would otherwise be 2 KiB per k-mer, since a ring slot holds one tile and
+// every fragment would be read once per tile.  Shift k fills rows 8 (k % 2) ..
+// + 5 only, so a lane (row i) holds a nonzero fragment for shifts of parity
+// i / 8 only: it keeps those two per chunk, each shift accumulates on its own
+// (the rows of the other half then collect products that are never read), and
+// each tile is reduced to the decision's partials as soon as it is complete.
 //
 // The wave's bitmap words collect in an LDS stash and go out in batches: a
 // global store between the ring's loads would leave the vmcnt wait that
@@ -1476,12 +1502,18 @@ __global__ __launch_bounds__(64 * W) void screen_ring(const ScreenDesc* __restri
     __builtin_amdgcn_s_waitcnt(0);  // the stores, and every ring load in flight
     nst = 0;
   };
-  v4f acc[4][2];
+  v4f acc[WREG ? 1 : 4][2];  // LDS weights: the block's tiles, shifts 0/1 and 2/3
+  v4f acc4[WREG ? 4 : 1];     // WREG: the current tile, one accumulator per shift
+  F6Part part[4];             // WREG: the block's finished tiles
 #pragma unroll
-  for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int r = 0; r < (WREG ? 1 : 4); ++r) acc[r][0] = acc[r][1] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < (WREG ? 4 : 1); ++k) acc4[k] = v4f{0.f, 0.f, 0.f, 0.f};
   uint32_t ne[4] = {0, 0, 0, 0};
   typedef int v8i __attribute__((ext_vector_type(8)));
-  v8i wa[WREG ? 4 : 1][WREG ? 4 : 1];  // WREG: fragments [chunk][shift] (MFMA operand form)
+  // WREG: [chunk][shifts 0/1, 2/3] fragments of this lane's row half
+  v8i wm[WREG ? 4 : 1][WREG ? 2 : 1];
+  const int hrow = (lane & 15) >> 3;  // 1: this lane holds a row of the upper half
 
   while (true) {
     const ScreenDesc& D = sd[s];
@@ -1491,10 +1523,11 @@ __global__ __launch_bounds__(64 * W) void screen_ring(const ScreenDesc* __restri
 #pragma unroll
       for (int C = 0; C < 4; ++C)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int h = 0; h < 2; ++h) {
+          const int k = 2 * h + hrow;
           const v4i a0 = gw[((C * 4 + k) * 64 + lane) * 2];
           const v4i a1 = gw[((C * 4 + k) * 64 + lane) * 2 + 1];
-          wa[C][k] = v8i{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, 0, 0};
+          wm[C][h] = v8i{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, 0, 0};
         }
       // every load in flight, the ring's included, has landed: the explicit
       // counts of the ring waits then cover ring loads only
@@ -1548,7 +1581,7 @@ __global__ __launch_bounds__(64 * W) void screen_ring(const ScreenDesc* __restri
             for (int k = 0; k < 4; ++k) {
               v8i a;
               if constexpr (WREG) {
-                a = wa[c][k];
+                a = wm[c][k >> 1];
               } else {
                 const v4i a0 = wl[((C * 4 + k) * 64 + lane) * 2];
                 const v4i a1 = wl[((C * 4 + k) * 64 + lane) * 2 + 1];
@@ -1564,11 +1597,21 @@ __global__ __launch_bounds__(64 * W) void screen_ring(const ScreenDesc* __restri
                         (int)((q[c].z >> 1) & 0x44444444u), (int)((q[c].w >> 1) & 0x44444444u),
                         0, 0, 0, 0};
               }
-              acc[t][k >> 1] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                  a, b, acc[t][k >> 1], 2, 4, 0, 127, 0, 127);
+              if constexpr (WREG)
+                acc4[k] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, acc4[k], 2, 4,
+                                                                          0, 127, 0, 127);
+              else
+                acc[t][k >> 1] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                    a, b, acc[t][k >> 1], 2, 4, 0, 127, 0, 127);
             }
           }
         }
+      }
+      if constexpr (WREG) {
+        // rows 0-7 (lane groups 0, 1): shifts 0 and 2; rows 8-15: shifts 1 and 3
+        part[t] = f6_partial(g < 2 ? acc4[0] : acc4[1], g < 2 ? acc4[2] : acc4[3], g);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc4[k] = v4f{0.f, 0.f, 0.f, 0.f};
       }
     }
     // the block's decision (as screen_i8<4, ...>)
@@ -1576,7 +1619,10 @@ __global__ __launch_bounds__(64 * W) void screen_ring(const ScreenDesc* __restri
     const int64_t kidx = kbase + 16 * g + col;
     int c = 0;
     bool keep = false, sure = false;
-    f6_decide<4>(acc, D, g, &c, &keep, &sure);
+    if constexpr (WREG)
+      f6_finish<4>(part, D, &c, &keep, &sure);
+    else
+      f6_decide<4>(acc, D, g, &c, &keep, &sure);
     keep = keep && kidx < D.m && c > 0 && c < D.n;
     unsigned long long dm = 0;
     if (DD) {
@@ -1604,7 +1650,7 @@ __global__ __launch_bounds__(64 * W) void screen_ring(const ScreenDesc* __restri
     }
     ++nst;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int r = 0; r < (WREG ? 1 : 4); ++r) acc[r][0] = acc[r][1] = v4f{0.f, 0.f, 0.f, 0.f};
     w += gridDim.x;
     if (w >= total_blocks) break;
     if (nst == kRingStash) flush();
@@ -2270,16 +2316,19 @@ bool screen_ring_fits(int nch, int W) {
   return nch >= 4 && nch % 4 == 0 && nch <= 4 * kRingMaxSeg && (W == 8 || (W == 12 && nch == 4));
 }
 
-// Instantiation for (W, nch_max, dd): 8 waves on 4-chunk models keep the weights
-// in registers and four ring slots (with the duplicate check, whose registers
-// that leaves no room for, the weights in LDS and three slots); 8-chunk models
-// and 12 waves read the weights from LDS with two slots.
+// Instantiation for (W, nch_max, dd): on 4-chunk models, 8 waves keep the
+// weights in registers with four ring slots and 12 waves with three (with the
+// duplicate check, whose registers that leaves no room for, the weights in LDS
+// with three or two slots); 8-chunk models read the weights from LDS with two
+// slots.  No instantiation may spill: scratch stores would share the vmcnt
+// counter with the ring's loads and void its waits.
 struct RingCfg {
   int nbuf;
   bool wreg;
 };
 static RingCfg ring_cfg(int W, int nch_max, bool dd) {
   if (W == 8 && nch_max <= 4) return dd ? RingCfg{3, false} : RingCfg{4, true};
+  if (W == 12 && !dd) return RingCfg{3, true};
   return RingCfg{2, false};
 }
 size_t screen_ring_lds(int W, int nch_max, bool dd) {
@@ -2299,7 +2348,7 @@ static const void* ring_kernel(int W, int nch_max, bool dd) {
     if (c.nbuf == 3) return ring_fn<8, 3, true, false>();
     return dd ? ring_fn<8, 2, true, false>() : ring_fn<8, 2, false, false>();
   }
-  return dd ? ring_fn<12, 2, true, false>() : ring_fn<12, 2, false, false>();
+  return dd ? ring_fn<12, 2, true, false>() : ring_fn<12, 3, false, true>();
 }
 
 hipError_t launch_screen_ring(int W, int nch_max, int grid, hipStream_t s, const ScreenDesc* sd,
