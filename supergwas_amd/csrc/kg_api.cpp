@@ -50,7 +50,7 @@ hipError_t launch_scan_i8_res(int NT, int nch_max, int grid, hipStream_t s, cons
                               const int64_t* total_dev);
 bool i8_res_fits(int NT, int nch);
 int64_t i8_res_block_kmers();
-hipError_t launch_scan_f6_res(int NT, int D, int nch_max, int grid, hipStream_t s,
+hipError_t launch_scan_f6_res(int NT, int D, int cm, int nch_max, int grid, hipStream_t s,
                               const ShardDesc* sd, int nshards, int64_t total_blocks,
                               const ScanOut& o, const int64_t* total_dev);
 bool f6_res_fits(int NT, int nch);
@@ -183,6 +183,7 @@ struct Model {
   std::vector<double> rc;        // centred residuals (n x t)
   std::vector<double> rbar, ycn, ymean, yyM;
   std::vector<uint64_t> maskw;   // accession mask, 8 * nchunks_i8 words
+  bool mask_full = false;        // every accession of every 512-sample chunk is in the model
   int nch_i8 = 0, nch_f32 = 0;
 };
 
@@ -198,6 +199,7 @@ struct StatPatch {
 // operands, column and trait records, epilogue constants.
 struct I8Set {
   int NT = 0;
+  bool cc = false;  // FP6 digits: the last column slot is the count column
   DBuf wts, colrec, trc;
   EpiI8 epi{};
 };
@@ -528,7 +530,8 @@ constexpr int kNoFit = -100;  // a secondary weight set that does not fit its ke
 // otherwise kNoFit when the columns need more tiles than L allows.  f6: L FP6
 // digits per column in the resident kernel's operand layout instead.
 static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_out,
-                        DBuf& wts_b, DBuf& colrec_b, DBuf& trc_b, EpiI8& ep, bool f6 = false) {
+                        DBuf& wts_b, DBuf& colrec_b, DBuf& trc_b, EpiI8& ep, bool f6 = false,
+                        bool* cc_out = nullptr) {
   Model& md = M.md;
   const int n = md.n, p = md.p, t = md.t;
   // 32385 n < 2^31 keeps every limb sum (|sum| <= 127 n) and the epilogue's
@@ -552,6 +555,9 @@ static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_o
   const int NU = 16 * NT;
   const int ncol = i8_column_capacity(NT, L);
   NT_out = NT;
+  // FP6 digits: a spare column slot becomes the count column (the last slot)
+  const bool cc = f6 && ncol > nb;
+  if (cc_out) *cc_out = cc;
   if (primary) {
     M.nz = nz;
     M.nb = nb;
@@ -578,6 +584,10 @@ static int build_i8_set(kg_ctx* ctx, ModelDev& M, int L, bool primary, int& NT_o
       for (int j = 0; j < n; ++j) ucol[(size_t)j * NU + u] = lim[(size_t)j * L + l];
     }
   }
+  // count column: digit 8 (the FP6 value 1) for every sample, so its digit-0
+  // sum is the allele count; its column record stays zero
+  if (cc)
+    for (int j = 0; j < n; ++j) ucol[(size_t)j * NU + urow(ncol - 1, 0)] = 8;
   // accession -> sample
   std::vector<int> samp(md.n_acc, -1);
   for (int j = 0; j < n; ++j) samp[md.idxs[j]] = j;
@@ -721,7 +731,7 @@ static int build_f6(kg_ctx* ctx, ModelDev& M) {
   const int D = ctx->res_digits;
   if (M.md.t != 1 || D == 0) return KG_OK;
   const int rc = build_i8_set(ctx, M, D, false, M.f6.NT, M.f6.wts, M.f6.colrec, M.f6.trc,
-                              M.f6.epi, true);
+                              M.f6.epi, true, &M.f6.cc);
   if (rc == kNoFit) return KG_OK;
   if (rc != KG_OK) return rc;
   M.f6_ok = f6_res_fits(M.f6.NT, M.md.nch_i8);
@@ -1045,6 +1055,8 @@ static int add_model(kg_ctx* ctx, int32_t n_acc, const int32_t* idxs, int32_t n,
   md.nch_f32 = (n_acc + 255) / 256;
   md.maskw.assign((size_t)8 * md.nch_i8, 0ull);
   for (int j = 0; j < n; ++j) md.maskw[idxs[j] >> 6] |= 1ull << (idxs[j] & 63);
+  md.mask_full = true;
+  for (uint64_t w : md.maskw) md.mask_full = md.mask_full && w == ~0ull;
   prepare_model(md);
   int rc = (ctx->precision == KG_PREC_MIXED) ? build_i8(ctx, *M) : build_f32(ctx, *M);
   if (rc == KG_OK && ctx->precision == KG_PREC_MIXED) rc = build_screen(ctx, *M);
@@ -1157,6 +1169,7 @@ struct ScreenRun {
   bool res = false;             // survivors through the resident-weight kernel
   int res_nt = 0;               // its weight tiles
   int res_d = 0;                // its FP6 digits per column (0: two int8 limbs)
+  int res_cm = kCntMasked;      // FP6: how the allele count is formed (kCnt*)
   // recheck pass of the k-mers the two-digit kernel leaves open: the streaming
   // kernel (primary weights, `rchk_L` limbs, two tiles per wave) over the
   // device-built table d_sd3 (d_total3 blocks of bk3 k-mers)
@@ -1190,7 +1203,8 @@ static int enqueue_i8(kg_ctx* ctx, int NT, int t, int nsh, int64_t total_blocks,
     KG_CK(ctx, launch_screen_compact(ctx->stream, sr.d_ch, sr.ncc, sr.d_ss, sr.ccount, sr.P,
                                      sr.d_sd2, nsh, sr.bk, sr.d_total));
     if (sr.res && sr.res_d > 0) {
-      KG_CK(ctx, launch_scan_f6_res(sr.res_nt, sr.res_d, sr.nch_max, ctx->cus, ctx->stream,
+      KG_CK(ctx, launch_scan_f6_res(sr.res_nt, sr.res_d, sr.res_cm, sr.nch_max, ctx->cus,
+                                    ctx->stream,
                                     sr.d_sd2, nsh, sr.sblocks_full, o, sr.d_total));
       if (sr.rchk) {
         KG_CK(ctx, launch_rchk_tables(ctx->stream, sr.d_sd3, nsh, sr.bk3, sr.d_total3));
@@ -1347,6 +1361,20 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     if (res_d > 0 && !f6_res_fits(res_nt, nch_max)) res_d = 0;
     if (res_d == 0) res_nt = 0;
   }
+  // FP6 allele count: from the count column when every model has one, else a
+  // bit count, of the bits themselves when every model covers every accession
+  // of its chunks; the shards' own masks (none in screened scans) need the
+  // masked count
+  int res_cm = kCntMasked;
+  if (res_d > 0) {
+    bool all_cc = true, all_full = true;
+    for (const auto& s : sh) {
+      const ModelDev& M = *ctx->models[s.model];
+      all_cc = all_cc && M.f6.cc && s.mask == nullptr;
+      all_full = all_full && M.md.mask_full && s.mask == nullptr;
+    }
+    res_cm = all_cc ? kCntColumn : (all_full ? kCntBits : kCntMasked);
+  }
   if (use_res && res_d == 0) {
     for (const auto& s : sh) {
       const ModelDev& M = *ctx->models[s.model];
@@ -1378,6 +1406,7 @@ static int scan_i8_shards(kg_ctx* ctx, const std::vector<HShard>& sh, const kg_s
     sr.res = use_res;
     sr.res_nt = res_nt;
     sr.res_d = res_d;
+    sr.res_cm = res_cm;
     sr.rchk = rchk;
     sr.bk = use_res ? i8_res_block_kmers() : bk;
     sr.bk3 = i8_block_kmers(NT, ctx->limbs, t, 2);

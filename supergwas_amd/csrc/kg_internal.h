@@ -30,6 +30,13 @@ constexpr size_t kResLdsBytes = 152 * 1024;
 // lanes of 24-byte A operands (32 six-bit digits), stored as a 16-byte part
 // [chunk][shift][tile][lane] followed by an 8-byte part in the same order.
 constexpr size_t kF6OperandBytes = 24;
+// Allele count of the FP6 resident kernel: a bit count of the masked bits
+// (any launch), of the bits themselves (every accession of every chunk is in
+// the model: no mask), or the count column of the weights (every model of the
+// launch has a spare column slot: no mask, no bit count).
+constexpr int kCntMasked = 0;
+constexpr int kCntBits = 1;
+constexpr int kCntColumn = 2;
 constexpr uint32_t kPfPass = 1u;  // pre-filter decided pass
 constexpr uint32_t kPfAmb = 2u;   // pre-filter within the error bound of tau
 constexpr uint32_t kSigDef = 4u;  // 1D threshold decided pass

@@ -340,7 +340,9 @@ __host__ __device__ constexpr int i8_tiles(int NT, int TT) { return (TT == 1 && 
 // FP32.  Element j = 8 d + m of lane group g, shift k, is accession
 // 512 C + 128 g + 32 d + 4 m + k.  With D.rchk set, k-mers whose decision the
 // digit bounds leave open go to the recheck list instead of the candidates.
-template <int NT, int L, int TT, int R, bool RES, bool F6 = false>
+// CM: the allele count from a bit count of the masked bits, of the unmasked
+// bits, or from the count column (kCnt*); the MFMA operands are never masked.
+template <int NT, int L, int TT, int R, bool RES, bool F6 = false, int CM = kCntMasked>
 __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __restrict__ sd,
                                                         int nshards, int64_t total_blocks,
                                                         ScanOut out,
@@ -353,6 +355,9 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
   static_assert(4 * J <= kMaxCols, "column records exceed the LDS table");
   static_assert(!RES || TT == 1, "resident weights: single-trait epilogue");
   static_assert(!F6 || (RES && (L == 2 || L == 3)), "FP6 digits: resident form, 2 or 3 digits");
+  static_assert(F6 || CM == kCntMasked, "count modes: FP6 digits only");
+  // the bits need the accession mask only for a popcount of masked bits
+  constexpr bool MASKED = CM == kCntMasked;
   typedef int v8i __attribute__((ext_vector_type(8)));
   typedef int v2i __attribute__((ext_vector_type(2)));
   using AccV = std::conditional_t<F6, v4f, v4i>;
@@ -384,7 +389,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
   if constexpr (RES) {
 #pragma unroll
     for (int r = 0; r < R; ++r) rcur[r] = *gptr(reinterpret_cast<const v4u*>(rowp[r]));
-    mcur = gptr(reinterpret_cast<const v4u*>(sd[s].mask))[g];
+    if constexpr (MASKED) mcur = gptr(reinterpret_cast<const v4u*>(sd[s].mask))[g];
   } else {
     dma_bits<R>(rowp, 0, mybits, lane);
     dma_mask(sd[s].mask, 0, g, mybits + R * 64);
@@ -438,14 +443,31 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
       // MFMA loop at raised priority: the other wave's epilogue fills its gaps
       if constexpr (F6) __builtin_amdgcn_s_setprio(1);
       for (int C = 0; C < nchunks; ++C) {
+        // FP6 operands need no mask: accessions outside the model have zero
+        // weights in every row (the count column's included)
         uint32_t wd[R][4];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          wd[r][0] = rcur[r].x & mcur.x;
-          wd[r][1] = rcur[r].y & mcur.y;
-          wd[r][2] = rcur[r].z & mcur.z;
-          wd[r][3] = rcur[r].w & mcur.w;
-          cnt[r] += __popc(wd[r][0]) + __popc(wd[r][1]) + __popc(wd[r][2]) + __popc(wd[r][3]);
+          if constexpr (!MASKED) {
+            wd[r][0] = rcur[r].x;
+            wd[r][1] = rcur[r].y;
+            wd[r][2] = rcur[r].z;
+            wd[r][3] = rcur[r].w;
+            if constexpr (CM == kCntBits) {
+              // one bit count with accumulate per word (the compiler would
+              // pair the counts and add them with a three-input add)
+              uint32_t cc = cnt[r];
+#pragma unroll
+              for (int i = 0; i < 4; ++i) asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(cc) : "v"(wd[r][i]));
+              cnt[r] = cc;
+            }
+          } else {
+            wd[r][0] = rcur[r].x & mcur.x;
+            wd[r][1] = rcur[r].y & mcur.y;
+            wd[r][2] = rcur[r].z & mcur.z;
+            wd[r][3] = rcur[r].w & mcur.w;
+            cnt[r] += __popc(wd[r][0]) + __popc(wd[r][1]) + __popc(wd[r][2]) + __popc(wd[r][3]);
+          }
         }
         // the next chunk loads under this one's MFMAs (the last chunk reloads
         // itself: unconditional loads into the same registers keep every wait
@@ -455,7 +477,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 #pragma unroll
           for (int r = 0; r < R; ++r)
             rcur[r] = *gptr(reinterpret_cast<const v4u*>(rowp[r] + 8 * Cn));
-          mcur = gptr(reinterpret_cast<const v4u*>(D.mask))[4 * Cn + g];
+          if constexpr (MASKED) mcur = gptr(reinterpret_cast<const v4u*>(D.mask))[4 * Cn + g];
         }
         __builtin_amdgcn_sched_barrier(0);
         if constexpr (F6) {
@@ -546,7 +568,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         rows_at<R>(sd[sn], nk, nridx, g, nrowp);
 #pragma unroll
         for (int r = 0; r < R; ++r) rcur[r] = *gptr(reinterpret_cast<const v4u*>(nrowp[r]));
-        mcur = gptr(reinterpret_cast<const v4u*>(sd[sn].mask))[g];
+        if constexpr (MASKED) mcur = gptr(reinterpret_cast<const v4u*>(sd[sn].mask))[g];
       }
     }
     if constexpr (!RES) {
@@ -675,11 +697,14 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     // per-k-mer totals go on in FP64, and the bounds take a relative 1e-5 for
     // these FP32 roundings (under 4e-6 for sums of 32 terms).
     constexpr bool F32E = F6;
-    float fzz[R], fzE[R], fz0[R], fsv[R], fS[R];
+    // kCntColumn: the allele count is the digit-0 sum of the count column
+    // (the last column slot, b = 4 J - 1: digit 1 for every sample of the
+    // model, zero column record), exact in FP32.
+    float fzz[R], fzE[R], fz0[R], fsv[R], fS[R], fcn[R];
     const bool f_S = F32E && D.rchk == nullptr;  // S of unproven k-mers decided here
     if constexpr (F32E) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) fzz[r] = fzE[r] = fz0[r] = fsv[r] = fS[r] = 0.f;
+      for (int r = 0; r < R; ++r) fzz[r] = fzE[r] = fz0[r] = fsv[r] = fS[r] = fcn[r] = 0.f;
 #pragma unroll
       for (int jc = 0; jc < J; ++jc) {
         const double* cr = colrec_lds + (4 * jc + g) * kColRec;
@@ -689,6 +714,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const float d0 = acc[r][sl >> 2][sl & 3], d1 = acc[r][(sl + 1) >> 2][(sl + 1) & 3];
+          if (CM == kCntColumn && jc == J - 1) fcn[r] = g == 3 ? d0 : 0.f;
           float z;
           if (L == 2) {
             z = w0f * fmaf(30.0f, d0, d1);
@@ -763,7 +789,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         if (jc < JSP && b == t) pz0[r] = z;
       }
     }
-    const int c = (int)group_scatter<R>(cnt);
+    const int c = CM == kCntColumn ? (int)group_scatter<R>(fcn) : (int)group_scatter<R>(cnt);
     const double cd = (double)c, cn = cd * ep.inv_n;
     // mean correction (kg_internal.h): z_k + (c/n) E_k for every column
     const double kzE = F32E ? (double)group_scatter<R>(fzE) : group_scatter<R>(pzE);
@@ -2146,13 +2172,14 @@ bool i8_res_fits(int NT, int nch) {
 // k-mers per workgroup block of the resident kernel
 int64_t i8_res_block_kmers() { return (int64_t)kI8Waves * 16 * 4; }
 
-// FP6-digit resident instantiations: D = 2 or 3 digits per column, NT = 1..6.
+// FP6-digit resident instantiations: D = 2 or 3 digits per column, NT = 1..6,
+// count mode CM (kCnt*).
 constexpr int kF6MaxTiles = 6;
-template <int NT, int D>
+template <int NT, int D, int CM>
 static hipError_t launch_f6_nt(dim3 grid, size_t shm, hipStream_t s, const ShardDesc* sd,
                                int nshards, int64_t total_blocks, const ScanOut& o,
                                const int64_t* total_dev) {
-  auto kfn = scan_i8<NT, D, 1, 4, true, true>;
+  auto kfn = scan_i8<NT, D, 1, 4, true, true, CM>;
   hipLaunchKernelGGL(kfn, grid, dim3(kI8Threads), shm, s, sd, nshards, total_blocks, o, total_dev);
   return hipGetLastError();
 }
@@ -2163,16 +2190,21 @@ bool f6_res_fits(int NT, int nch) {
   return NT >= 1 && NT <= kF6MaxTiles && f6_res_lds_bytes(NT, nch) <= kResLdsBytes;
 }
 
-hipError_t launch_scan_f6_res(int NT, int D, int nch_max, int grid, hipStream_t s,
+hipError_t launch_scan_f6_res(int NT, int D, int cm, int nch_max, int grid, hipStream_t s,
                               const ShardDesc* sd, int nshards, int64_t total_blocks,
                               const ScanOut& o, const int64_t* total_dev) {
-  if (!f6_res_fits(NT, nch_max) || (D != 2 && D != 3)) return hipErrorInvalidValue;
+  if (!f6_res_fits(NT, nch_max) || (D != 2 && D != 3) || cm < kCntMasked || cm > kCntColumn)
+    return hipErrorInvalidValue;
   dim3 g((unsigned)(total_blocks < grid ? (total_blocks > 0 ? total_blocks : 1) : grid));
   const size_t shm = f6_res_lds_bytes(NT, nch_max);
-  switch (NT * 4 + D) {
-#define KG_CASE(X, Y) \
-  case X * 4 + Y:     \
-    return launch_f6_nt<X, Y>(g, shm, s, sd, nshards, total_blocks, o, total_dev);
+  switch ((NT * 4 + D) * 4 + cm) {
+#define KG_CASE(X, Y)                                                                       \
+  case (X * 4 + Y) * 4 + kCntMasked:                                                        \
+    return launch_f6_nt<X, Y, kCntMasked>(g, shm, s, sd, nshards, total_blocks, o, total_dev); \
+  case (X * 4 + Y) * 4 + kCntBits:                                                          \
+    return launch_f6_nt<X, Y, kCntBits>(g, shm, s, sd, nshards, total_blocks, o, total_dev);   \
+  case (X * 4 + Y) * 4 + kCntColumn:                                                        \
+    return launch_f6_nt<X, Y, kCntColumn>(g, shm, s, sd, nshards, total_blocks, o, total_dev);
     KG_CASE(1, 2) KG_CASE(2, 2) KG_CASE(3, 2) KG_CASE(4, 2) KG_CASE(5, 2) KG_CASE(6, 2)
     KG_CASE(1, 3) KG_CASE(2, 3) KG_CASE(3, 3) KG_CASE(4, 3) KG_CASE(5, 3) KG_CASE(6, 3)
 #undef KG_CASE
@@ -2472,13 +2504,14 @@ hipError_t set_kernel_attributes() {
   for (const void* f : res_fns)
     if (e == hipSuccess)
       e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLdsBytes);
-  const void* f6_fns[] = {
-      (const void*)scan_i8<1, 2, 1, 4, true, true>, (const void*)scan_i8<2, 2, 1, 4, true, true>,
-      (const void*)scan_i8<3, 2, 1, 4, true, true>, (const void*)scan_i8<4, 2, 1, 4, true, true>,
-      (const void*)scan_i8<5, 2, 1, 4, true, true>, (const void*)scan_i8<6, 2, 1, 4, true, true>,
-      (const void*)scan_i8<1, 3, 1, 4, true, true>, (const void*)scan_i8<2, 3, 1, 4, true, true>,
-      (const void*)scan_i8<3, 3, 1, 4, true, true>, (const void*)scan_i8<4, 3, 1, 4, true, true>,
-      (const void*)scan_i8<5, 3, 1, 4, true, true>, (const void*)scan_i8<6, 3, 1, 4, true, true>};
+#define KG_F6(X, Y)                                                              \
+  (const void*)scan_i8<X, Y, 1, 4, true, true, kCntMasked>,                     \
+      (const void*)scan_i8<X, Y, 1, 4, true, true, kCntBits>,                   \
+      (const void*)scan_i8<X, Y, 1, 4, true, true, kCntColumn>
+  const void* f6_fns[] = {KG_F6(1, 2), KG_F6(2, 2), KG_F6(3, 2), KG_F6(4, 2),
+                          KG_F6(5, 2), KG_F6(6, 2), KG_F6(1, 3), KG_F6(2, 3),
+                          KG_F6(3, 3), KG_F6(4, 3), KG_F6(5, 3), KG_F6(6, 3)};
+#undef KG_F6
   for (const void* f : f6_fns)
     if (e == hipSuccess)
       e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLdsBytes);

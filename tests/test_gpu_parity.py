@@ -37,6 +37,8 @@ CASES = {
                        effect_sd=0.3), 12000, 0.2, 4.0, 8.0),
     "n333_nointercept": (dict(seed=3, n_acc=333, n_missing=7, npop=4, p=6, m=5001, n_causal=5,
                               effect=0.5, intercept=False), 5001, 0.1, 2.0, 4.0),
+    "n2048_p32_missing": (dict(seed=4, n_acc=2048, n_missing=60, npop=8, p=32, m=12000,
+                               n_causal=50, effect_sd=0.3), 12000, 0.2, 4.0, 8.0),
 }
 
 
@@ -352,12 +354,16 @@ def test_every_launch_shape_gives_identical_results(built, t):
         assert len(kpw) >= 2
 
 
-@pytest.mark.parametrize("name", ["n2048_p32", "n1024_p20", "n333_nointercept"])
+@pytest.mark.parametrize("name", ["n2048_p32", "n2048_p32_missing", "n1024_p20",
+                                  "n333_nointercept"])
 def test_resident_forms_identical(built, name):
     """The resident kernel's operand forms -- two int8 limbs, two FP6 digits against FP4 bits
     (with the three-limb recheck pass of the k-mers its bounds leave open), three FP6 digits --
     and the streaming kernel report the same hits, values and counts, equal to the FP64
-    oracle outside the +-1e-6 band.  The p threshold sits on a passing k-mer's own p-value near
+    oracle outside the +-1e-6 band.  The FP6 kernel's allele count comes three ways: a bit count
+    of the unmasked bits (n2048_p32: every accession phenotyped, 32 columns fill 4 tiles), of
+    the masked bits (n2048_p32_missing: 60 accessions without phenotype), or the count column
+    in a spare column slot (n1024_p20, n333_nointercept).  The p threshold sits on a passing k-mer's own p-value near
     T = 2, in the bulk of the statistic's distribution, so that many k-mers fall inside the
     two-digit bounds and the recheck pass has work."""
     from supergwas_amd.api import KgwasHip, ScanParams
