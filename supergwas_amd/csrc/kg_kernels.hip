@@ -367,6 +367,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
   __shared__ v4i buf1[RES ? 1 : FRAG2];
   __shared__ v4i bitsbuf[RES ? 1 : kI8Waves * (R + 1) * 64];  // [wave][r | mask][lane]
   __shared__ double colrec_lds[4 * J * kColRec];
+  __shared__ __attribute__((aligned(16))) float colf_lds[F6 ? 4 * J * 4 : 4];  // F6: {w0, E, k_0}, FP32
   __shared__ double trc_lds[kMaxTraits * kTrcRec];
   extern __shared__ v4i wres[];  // RES: [chunk][step][tile][lane] of the staged model
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -436,6 +437,15 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         const double* __restrict__ gcr = ep.colrec;
         for (int i = threadIdx.x; i < ep.ncol * kColRec; i += kI8Threads) colrec_lds[i] = gcr[i];
         for (int i = threadIdx.x; i < ep.t * kTrcRec; i += kI8Threads) trc_lds[i] = ep.trc[i];
+        if constexpr (F6) {
+          for (int i = threadIdx.x; i < 4 * J; i += kI8Threads) {
+            const bool in = i < ep.ncol;
+            colf_lds[4 * i + 0] = in ? (float)gcr[i * kColRec + 0] : 0.f;
+            colf_lds[4 * i + 1] = in ? (float)gcr[i * kColRec + 11] : 0.f;
+            colf_lds[4 * i + 2] = in ? (float)gcr[i * kColRec + 3] : 0.f;
+            colf_lds[4 * i + 3] = 0.f;
+          }
+        }
         __syncthreads();
         staged_w = D.wts;
         staged_cr = ep.colrec;
@@ -708,7 +718,8 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
 #pragma unroll
       for (int jc = 0; jc < J; ++jc) {
         const double* cr = colrec_lds + (4 * jc + g) * kColRec;
-        const float w0f = (float)cr[0], cEf = (float)cr[11], kcf = (float)cr[3];
+        const v4f cf4 = *reinterpret_cast<const v4f*>(colf_lds + 4 * (4 * jc + g));
+        const float w0f = cf4.x, cEf = cf4.y, kcf = cf4.z;
         const int b = 4 * jc + g;
         const int sl = L * jc;
 #pragma unroll
@@ -790,15 +801,14 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
       }
     }
     const int c = CM == kCntColumn ? (int)group_scatter<R>(fcn) : (int)group_scatter<R>(cnt);
-    const double cd = (double)c, cn = cd * ep.inv_n;
-    // mean correction (kg_internal.h): z_k + (c/n) E_k for every column
-    const double kzE = F32E ? (double)group_scatter<R>(fzE) : group_scatter<R>(pzE);
-    const double kzz = (F32E ? (double)group_scatter<R>(fzz) : group_scatter<R>(pzz)) +
-                       cn * fma(cn, ep.sum_E2, 2.0 * kzE);
-    const double kz0 = (F32E ? (double)group_scatter<R>(fz0) : group_scatter<R>(pz0)) + cn * ep.E_d;
-    const double kzh = est ? group_scatter<R>(pzh) : 0.0;
-    // FP32 column sums: relative slack for their roundings
-    const double f32s = F32E ? 1e-5 : 0.0;
+    // FP32 column totals (FP6 digits): this lane's k-mer
+    float tzE = 0.f, tzz = 0.f, tz0 = 0.f, tsv = 0.f;
+    if constexpr (F32E) {
+      tzE = group_scatter<R>(fzE);
+      tzz = group_scatter<R>(fzz);
+      tz0 = group_scatter<R>(fz0);
+      tsv = group_scatter<R>(fsv);
+    }
 
     const int64_t kidx = kbase + 16 * rme + col;
     const bool valid = primary && (kidx < D.m);
@@ -808,6 +818,75 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
     // (single-trait lists only: a multi-trait screen proves nothing)
     const bool pf_sure = (RES || TT == 1) && (lrow & ~kRowMask) != 0u;
     const uint32_t kmult = (D.mult != nullptr && valid) ? gptr(D.mult)[kidx] : 1u;
+
+    // FP6 digits, hits-only scans: a decision in FP32 first.  A k-mer is
+    // decided here when its pre-filter pass is proven (or the threshold is
+    // trivial) and r^2 < r2star is certain with the bounds of the FP64 pass
+    // below, evaluated in FP32 with 3e-5 relative slack for the column sums'
+    // and these totals' roundings (each under 1e-6) and a 1e-4 margin on the
+    // comparison; degenerate k-mers (c = 0, c = n, |y - ybar| = 0) are
+    // decided as classify_i8 decides them.  A wave whose k-mers are all
+    // decided skips the FP64 pass: it only counts its passes.
+    bool fast_all = false, fpd = false;
+    if constexpr (F32E) {
+      const double* tc = trc_lds;  // the one trait
+      bool fast = !valid;
+      if (valid && !ep.pairs && out.st_c == nullptr && !ep.all_sig) {
+        if (c <= 0 || c >= n || !(tc[1] >= 1e-12)) {
+          fast = true;  // classify_i8: not reported, not counted
+        } else if (pf_sure || ep.tau_trivial) {
+          if (!(tc[2] > 1e-12)) {
+            fast = fpd = true;  // compute_hit gives p = 1
+          } else {
+            const float cf = (float)c, nf = (float)n;
+            const float cnf = cf * (float)ep.inv_n, cmf = fminf(cf, nf - cf);
+            const float sh2 = (float)ep.sum_h2, sE2 = (float)ep.sum_E2;
+            const float kzzf = tzz + cnf * fmaf(cnf, sE2, 2.f * tzE);
+            const float kzzp = fmaxf(kzzf, 0.f);
+            float crossf = 0.f, bcrf = 0.f;
+            if (ep.has_d) {
+              const float sg = (float)ep.sigma;
+              crossf = 2.f * cnf * sg * (tz0 + cnf * (float)ep.E_d);
+              bcrf = 2.f * cnf * fabsf(sg) * cmf * (float)ep.h_d;
+            }
+            const float c2b = cf * cf * (float)ep.base_coef;
+            const float xxf = cf - c2b - crossf - kzzf;
+            float bxxf = 2.f * cmf * sqrtf(kzzp * sh2) + cmf * cmf * sh2 + bcrf +
+                         3e-5f * (cf + c2b + kzzp + fabsf(crossf) +
+                                  2.f * cnf * sqrtf(kzzp * sE2) + 1.f);
+            if (ep.dmax_drop > 0.0) {
+              const float e0 = cf * (float)ep.dmax_drop;
+              bxxf += 2.f * cnf * fabsf((float)ep.sigma) * e0 + e0 * e0;
+            }
+            const float svf = tsv + cnf * (float)tc[5], rbf = (float)tc[0];
+            const float xyf = fmaf(cf, rbf, svf);
+            const float bxyf = cmf * (float)tc[4] + 3e-5f * (fabsf(svf) + cf * fabsf(rbf) + 1.f);
+            const float lo = xxf - bxxf, hi = fabsf(xyf) + bxyf;
+            if (lo > 1e-6f * cf && hi * hi < (float)ep.r2star * 0.9999f * lo * (float)tc[2])
+              fast = fpd = true;
+          }
+        }
+      }
+      fast_all = __ballot(!fast) == 0ull;
+    }
+    if (fast_all) {
+      if (D.mult == nullptr) {
+        wpass[0] += (uint32_t)__popcll(__ballot(fpd));
+      } else {
+        uint32_t v = fpd ? kmult : 0u;
+        for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+        wpass[0] += v;
+      }
+    } else {
+    const double cd = (double)c, cn = cd * ep.inv_n;
+    // mean correction (kg_internal.h): z_k + (c/n) E_k for every column
+    const double kzE = F32E ? (double)tzE : group_scatter<R>(pzE);
+    const double kzz = (F32E ? (double)tzz : group_scatter<R>(pzz)) +
+                       cn * fma(cn, ep.sum_E2, 2.0 * kzE);
+    const double kz0 = (F32E ? (double)tz0 : group_scatter<R>(pz0)) + cn * ep.E_d;
+    const double kzh = est ? group_scatter<R>(pzh) : 0.0;
+    // FP32 column sums: relative slack for their roundings
+    const double f32s = F32E ? 1e-5 : 0.0;
     // |dz_k| <= c' h_k (c' = min(c, n - c))  =>
     //   |d |z|^2| <= 2c' sum|z_k| h_k + c'^2 sum h_k^2 <= 2c' |z| |h| + c'^2 |h|^2
     const double cm = fmin(cd, (double)ep.n - cd);
@@ -851,7 +930,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         if constexpr (F32E) {
           // without S (recheck pass), an unproven pre-filter stays open: bS = inf
           S = f_S ? (double)group_scatter<R>(fS) + cn * tc[6] : 0.0;
-          sv = (double)group_scatter<R>(fsv) + cn * tc[5];
+          sv = (double)tsv + cn * tc[5];
         } else {
           S = group_scatter<R>(vS) + cn * tc[6];
           sv = group_scatter<R>(vsv) + cn * tc[5];
@@ -917,6 +996,7 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         append_cand(fl != 0, rec, out);
       }
     }
+    }  // FP64 pass
     // the shard's passes counted by this wave, when the next block is another
     // shard's (or there is none)
     if (sn != s || wn >= total_blocks) {
