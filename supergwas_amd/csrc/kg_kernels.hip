@@ -809,6 +809,15 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
       tz0 = group_scatter<R>(fz0);
       tsv = group_scatter<R>(fsv);
     }
+    const double cd = (double)c, cn = cd * ep.inv_n;
+    // mean correction (kg_internal.h): z_k + (c/n) E_k for every column
+    const double kzE = F32E ? (double)tzE : group_scatter<R>(pzE);
+    const double kzz = (F32E ? (double)tzz : group_scatter<R>(pzz)) +
+                       cn * fma(cn, ep.sum_E2, 2.0 * kzE);
+    const double kz0 = (F32E ? (double)tz0 : group_scatter<R>(pz0)) + cn * ep.E_d;
+    const double kzh = est ? group_scatter<R>(pzh) : 0.0;
+    // FP32 column sums: relative slack for their roundings
+    const double f32s = F32E ? 1e-5 : 0.0;
 
     const int64_t kidx = kbase + 16 * rme + col;
     const bool valid = primary && (kidx < D.m);
@@ -878,15 +887,6 @@ __global__ __launch_bounds__(kI8Threads, 2) void scan_i8(const ShardDesc* __rest
         wpass[0] += v;
       }
     } else {
-    const double cd = (double)c, cn = cd * ep.inv_n;
-    // mean correction (kg_internal.h): z_k + (c/n) E_k for every column
-    const double kzE = F32E ? (double)tzE : group_scatter<R>(pzE);
-    const double kzz = (F32E ? (double)tzz : group_scatter<R>(pzz)) +
-                       cn * fma(cn, ep.sum_E2, 2.0 * kzE);
-    const double kz0 = (F32E ? (double)tz0 : group_scatter<R>(pz0)) + cn * ep.E_d;
-    const double kzh = est ? group_scatter<R>(pzh) : 0.0;
-    // FP32 column sums: relative slack for their roundings
-    const double f32s = F32E ? 1e-5 : 0.0;
     // |dz_k| <= c' h_k (c' = min(c, n - c))  =>
     //   |d |z|^2| <= 2c' sum|z_k| h_k + c'^2 sum h_k^2 <= 2c' |z| |h| + c'^2 |h|^2
     const double cm = fmin(cd, (double)ep.n - cd);
