@@ -240,7 +240,10 @@ def main():
                 traffic_per[kname.strip()] = float(val)
     row_bytes = n_acc / 8.0
     n_surv = float(np.mean(surv))
-    kern = {"screen_i8": {"ms": float(np.mean(scr_ms)), "kmers": args.m},
+    # the one-trait screen streams its rows through an LDS ring (screen shapes 3
+    # and 4) or loads them into registers (screen_i8)
+    scr_name = "screen_ring" if mixed.get_tune()["screen_wgs_per_cu"] in (3, 4) else "screen_i8"
+    kern = {scr_name: {"ms": float(np.mean(scr_ms)), "kmers": args.m},
             "scan_i8": {"ms": float(np.mean(bulk_ms)), "kmers": n_surv}}
     for kname, kv in kern.items():
         kv["achieved_GBs"] = (kv["kmers"] * row_bytes / (kv["ms"] / 1e3) / 1e9
