@@ -11,7 +11,9 @@
     launch modes;
   * models past the screen's 8192 accessions (the unscreened fallback over more
     than 16 chunks of 512 samples) and a U.bin with r = 0
-    (src/kgwas_utils.hpp:135-157 accepts it).
+    (src/kgwas_utils.hpp:135-157 accepts it);
+  * the headline workload's shape (n=2048, p=32) at 1.2e6 k-mers, where every
+    workgroup of the persistent kernels walks several blocks, against the oracle.
 """
 import numpy as np
 import pytest
@@ -213,3 +215,51 @@ def test_model_without_structure_columns(built):
     assert np.array_equal(st["c"], ref["c"])
     Tb = T_from(st["xxM"], st["xyM"][:, 0], oracle.yyM(ds.y, U0), ds.n)
     assert rel_err_T(Tb, ref["T"]).max() <= STAT_TOL
+
+
+def test_headline_shape_over_many_blocks(built):
+    """n=2048, p=32, 1.2e6 k-mers in one screened scan: every workgroup of the persistent
+    kernels walks several blocks (about 9 screen blocks and 1-2 survivor blocks each), so the
+    ring's refills across blocks, the survivor kernel's next-block row loads and the recheck
+    pass all run as at the benchmark's size.  Pre-filter count and hits against the FP64
+    oracle outside the +-1e-6 band (the oracle in pieces of 1e5 k-mers; the |cor| band from
+    oracle runs at tau (1 -+ 1e-6))."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    m, corr, l10, l2d = 1_200_000, 0.2, 4.0, 8.0
+    ds = synth.make_dataset(seed=11, **dict(synth.CONFIGS["mixed"], m=m))
+    bits = ds.fill_bits(0, m)
+    prm = ScanParams(corr, l10, l2d)
+    with KgwasHip(screen="on") as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.y)
+        dev = k.to_device(bits)
+        h, n = k.scan_device(dev.ptr, m, bits.shape[1], prm)
+        tm, tune = k.timing(), k.get_tune()
+        dev.free()
+    print(f"[headline shape] m={m} hits={len(h)} passed={int(n[0])} survivors={tm['survivors']} "
+          f"rechecked={tm['rechecked']} candidates={tm['candidates']}", flush=True)
+    assert tune["resident_active"] == 1 and tune["screen_wgs_per_cu"] == 3
+    assert tm["survivors"] < m // 4
+    exp_all, near_all, c_all, T_all = [], np.zeros(m, bool), np.zeros(m, np.int64), np.zeros(m)
+    npass, nnear_c = 0, 0
+    for a in range(0, m, 100_000):
+        b = bits[a:a + 100_000]
+        ref = oracle.scan_f64(b, ds.idxs, ds.y, ds.U, corr)
+        lo = oracle.scan_f64(b, ds.idxs, ds.y, ds.U, corr * (1.0 - BAND))["passed"]
+        hi = oracle.scan_f64(b, ds.idxs, ds.y, ds.U, corr * (1.0 + BAND))["passed"]
+        near_c = lo & ~hi
+        with np.errstate(divide="ignore", invalid="ignore"):
+            near_p = np.abs(ref["p"] / 10.0 ** -l10 - 1.0) <= BAND
+        exp, _ = expected_hits(ref, l10, l2d)
+        exp_all.append(exp + a)
+        near_all[a:a + len(b)] = near_c | near_p
+        c_all[a:a + len(b)] = ref["c"]
+        T_all[a:a + len(b)] = ref["T"]
+        npass += int(ref["n_passed"])
+        nnear_c += int(near_c.sum())
+    exp_all = np.concatenate(exp_all)
+    assert abs(int(n[0]) - npass) <= nnear_c
+    sym = np.setxor1d(exp_all, h["idx"])
+    assert np.all(near_all[sym]), sym[:10]
+    assert np.array_equal(h["c"], c_all[h["idx"]])
+    assert np.allclose(h["T"], T_all[h["idx"]], rtol=1e-9, atol=0)
+    assert len(exp_all) > 10
