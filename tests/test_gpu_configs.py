@@ -12,8 +12,9 @@
   * models past the screen's 8192 accessions (the unscreened fallback over more
     than 16 chunks of 512 samples) and a U.bin with r = 0
     (src/kgwas_utils.hpp:135-157 accepts it);
-  * the headline workload's shape (n=2048, p=32) at 1.2e6 k-mers, where every
-    workgroup of the persistent kernels walks several blocks, against the oracle.
+  * the headline workload's shape (n=2048, p=32) at 1.2e6 k-mers, and three
+    traits at 4e5 k-mers, where every workgroup of the persistent kernels walks
+    several blocks, against the oracle.
 """
 import numpy as np
 import pytest
@@ -263,3 +264,42 @@ def test_headline_shape_over_many_blocks(built):
     assert np.array_equal(h["c"], c_all[h["idx"]])
     assert np.allclose(h["T"], T_all[h["idx"]], rtol=1e-9, atol=0)
     assert len(exp_all) > 10
+
+
+def test_three_traits_over_many_blocks(built):
+    """Three traits at n=2048, p=32, 4e5 k-mers in one screened scan at tau = 0.1 (most
+    k-mers survive for some trait): the multi-trait screen and the streaming full kernel
+    (weight stages through LDS) walk several blocks per workgroup, so the next block's first
+    bits chunk and weight stage load during a block's last stage.  Per-trait pre-filter
+    counts and hits against the FP64 oracle outside the +-1e-6 band."""
+    from supergwas_amd.api import KgwasHip, ScanParams
+    m, corr, l10, l2d = 400_000, 0.1, 4.0, 8.0
+    ds = synth.make_dataset(seed=12, n_acc=2048, n_missing=0, npop=8, p=32, m=m, n_causal=50,
+                            effect_sd=0.3, traits=3, trait_corr=0.2)
+    bits = ds.fill_bits(0, m)
+    prm = ScanParams(corr, l10, l2d)
+    with KgwasHip(screen="on") as k:
+        k.set_model(ds.n_acc, ds.idxs, ds.U, ds.Y)
+        dev = k.to_device(bits)
+        h, n = k.scan_device(dev.ptr, m, bits.shape[1], prm)
+        tm = k.timing()
+        dev.free()
+    print(f"[three traits] m={m} hits={len(h)} passed={list(map(int, n))} "
+          f"survivors={tm['survivors']} candidates={tm['candidates']}", flush=True)
+    assert 256 * 256 * 2 < tm["survivors"] < m   # more than two 256-k-mer blocks per workgroup
+    for tr in range(3):
+        y = ds.Y[:, tr]
+        ref = oracle.scan_f64(bits, ds.idxs, y, ds.U, corr)
+        lo = oracle.scan_f64(bits, ds.idxs, y, ds.U, corr * (1.0 - BAND))["passed"]
+        hi = oracle.scan_f64(bits, ds.idxs, y, ds.U, corr * (1.0 + BAND))["passed"]
+        near_c = lo & ~hi
+        with np.errstate(divide="ignore", invalid="ignore"):
+            near_p = np.abs(ref["p"] / 10.0 ** -l10 - 1.0) <= BAND
+        assert abs(int(n[tr]) - ref["n_passed"]) <= int(near_c.sum()), tr
+        exp, _ = expected_hits(ref, l10, l2d)
+        got = h[h["trait"] == tr]
+        sym = np.setxor1d(exp, got["idx"])
+        assert np.all((near_c | near_p)[sym]), (tr, sym[:10])
+        assert np.array_equal(got["c"], ref["c"][got["idx"]]), tr
+        assert np.allclose(got["T"], ref["T"][got["idx"]], rtol=1e-9, atol=0), tr
+        assert len(exp) > 0, tr
